@@ -1,0 +1,232 @@
+/*
+ * maos_tomo.h -- C ABI of the MI355X tomography solver.
+ *
+ * Two layers, both plain C (no torch or HIP types in any signature):
+ *
+ *  1. Host setup (FP64, C++ inside): builds the reconstruction geometry and
+ *     operators exactly as MAOS's CPU setup does before it calls
+ *     gpu_setup_recon():  ploc/xloc grids, saloc + subaperture area, the GP
+ *     average-gradient operator, the NEA weighting W (saneai), the TT
+ *     projector PTT, the Laplacian regulariser L2 and the FDPCG block
+ *     preconditioner.  Result: an mt_recon_t, a POD view of plain arrays that
+ *     plays the role of MAOS's RECON_T on this path.
+ *
+ *  2. GPU solve (HIP, gfx950): consumes an mt_recon_t (from (1) or from a
+ *     caller that built the same arrays itself) and runs the FDPCG-PCG
+ *     tomography solve.  These calls replace the reference's C entry points
+ *     gpu_setup_recon / gpu_update_recon_cn2 / gpu_recon_reset / gpu_tomo
+ *     (cuda/gpu.h:49-56, cuda/recon.cu:477-582) and the C++ operator entry
+ *     points cutomo_grid::R/L (cuda/tomo.cu:607-677), cufdpcg_t::P
+ *     (cuda/fdpcg.cu:168-226) and gpu_pcg (cuda/pcg.cu:87-242).
+ *
+ * Every function returns an int status (MT_OK == 0, negative on error) and
+ * never aborts the process; mt_last_error() returns a message for the last
+ * failure on the calling thread.
+ *
+ * Vector layouts (identical to MAOS dcell layouts):
+ *   gradient vector : per WFS w (in order, skipped WFS have 0 entries),
+ *                     2*nsa[w] doubles: all x slopes then all y slopes.
+ *   layer vector    : per layer l, xnx[l]*xny[l] values, index ix+iy*xnx[l]
+ *                     (the square xloc of tomo.square=1).
+ */
+#ifndef MAOS_TOMO_H
+#define MAOS_TOMO_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* status codes                                                        */
+/* ------------------------------------------------------------------ */
+#define MT_OK          0
+#define MT_EINVAL     -1   /* invalid argument or dimension            */
+#define MT_ENOMEM     -2   /* host or device allocation failed         */
+#define MT_EHIP       -3   /* HIP runtime error                        */
+#define MT_ENOTFINITE -4   /* non-finite input (NaN/Inf gradients)     */
+#define MT_ESTATE     -5   /* call not valid in the current state      */
+#define MT_EFULL      -6   /* admission refused: over memory budget    */
+#define MT_NOTCONV     1   /* positive flag: solve ran but residual > 1 */
+
+const char *mt_last_error(void);
+
+/* ------------------------------------------------------------------ */
+/* 1. host setup                                                       */
+/* ------------------------------------------------------------------ */
+
+/* One wavefront-sensor type (MAOS powfs.* keys on this path). */
+typedef struct {
+    double dsa;       /* powfs.dsa: subaperture size (m)                    */
+    double dx;        /* powfs.dx: OPD sampling inside a subaperture (m)    */
+    double saat;      /* powfs.saat: relative-area threshold                */
+    double nearecon;  /* powfs.nearecon: NEA for reconstruction (mas)       */
+    double hs;        /* powfs.hs: guide star height (m); INFINITY for NGS  */
+    int    trs;       /* powfs.trs: remove tip/tilt from this WFS' slopes   */
+    int    lo;        /* powfs.lo: low-order WFS (not in tomography)        */
+    int    skip;      /* powfs.skip                                         */
+    int    dtrat;     /* powfs.dtrat                                        */
+} mt_powfs_t;
+
+/* One wavefront sensor (MAOS wfs.* keys). */
+typedef struct {
+    double thetax, thetay;  /* direction (radian)            */
+    int    powfs;           /* index into mt_parms_t.powfs    */
+} mt_wfs_t;
+
+/* The .conf parameters the tomography path reads (PARMS_T subset). */
+typedef struct {
+    double aper_d, aper_din;       /* aper.d = [d din]                       */
+    int    npsr;                   /* number of reconstructed layers         */
+    const double *ht;              /* atmr.ht (m)                            */
+    const double *wt;              /* atmr.wt                                */
+    const int    *os;              /* atmr.os                                */
+    double atmr_dx;                /* atmr.dx; 0 = smallest high-order dsa   */
+    double r0z;                    /* atmr.r0z (m at 500 nm, zenith)         */
+    double L0;                     /* atmr.L0 (m)                            */
+    double za;                     /* sim.za (radian)                        */
+    int    tomo_pos;               /* tomo.pos                               */
+    int    tomo_guard;             /* tomo.guard                             */
+    int    tomo_cone;              /* tomo.cone                              */
+    double tomo_cxxscale;          /* tomo.cxxscale                          */
+    int    tomo_precond;           /* tomo.precond: 0 none, 1 FDPCG          */
+    int    recon_split;            /* recon.split                            */
+    int    tomo_splitlrt;          /* tomo.splitlrt                          */
+    int    npowfs;
+    const mt_powfs_t *powfs;
+    int    nwfs;
+    const mt_wfs_t *wfs;
+    int    ndir;                   /* extra beam directions (evl + fit)      */
+    const double *dir_thetax;      /* radian                                 */
+    const double *dir_thetay;      /* radian                                 */
+    const double *dir_hs;          /* m; INFINITY for science                */
+    double tomoscale;              /* TOMOSCALE (MAOS default 1e-12)         */
+} mt_parms_t;
+
+/*
+ * POD view of the reconstruction operators (the RECON_T fields that
+ * gpu_setup_recon reads, cuda/recon.cu:477-488, cuda/tomo.cu:174-312).
+ * All pointers are owned by whoever produced the struct.
+ */
+typedef struct {
+    int    npsr;
+    const long   *xnx, *xny;       /* layer grid sizes (recon->xmap)         */
+    const double *xox, *xoy, *xdx; /* layer grid origin / spacing (m)        */
+    const double *ht;              /* layer heights (m)                      */
+    long   pnx, pny;               /* ploc square grid (recon->pmap)         */
+    double pox, poy, pdx;
+    int    nwfs;
+    const double *thetax, *thetay, *hs;   /* per WFS                         */
+    const int    *skip;            /* per WFS: 1 = not in tomography         */
+    const long   *nsa;             /* per WFS subaperture count              */
+    const double *dsa;             /* per WFS subaperture size               */
+    const double * const *saloc;   /* per WFS: locx[nsa] then locy[nsa]      */
+    const double * const *saa;     /* per WFS: normalized subaperture area   */
+    /* GP per WFS in MAOS dsp (CSC) layout: 2*nsa rows x pnx*pny columns      */
+    const long   * const *gp_p;    /* ncol+1                                 */
+    const long   * const *gp_i;    /* row indices                            */
+    const double * const *gp_x;    /* values                                 */
+    int    pos;                    /* tomo.pos (GP stencil is (pos+1)^2)     */
+    const double * const *neai;    /* per WFS 3*nsa: [xx yy xy] per subap    */
+    const int    *trs;             /* per WFS: RHS removes TT                */
+    const double * const *ptt;     /* per WFS (trs): 2 x 2nsa column-major   */
+    int    ptt_lhs;                /* 1: LHS also removes TT (!split||lrt)   */
+    const double *l2c;             /* per layer: (0.25*laplacian_coef)^2
+                                      * cxxscale * TOMOSCALE                 */
+    /* FDPCG */
+    int    precond;                /* 0 none, 1 FDPCG                         */
+    int    fd_bs;                  /* block size = sum os^2                   */
+    long   fd_nb;                  /* number of blocks (full spectrum)        */
+    long   fd_nbhalf;              /* blocks of the half spectrum             */
+    int    fd_scale;               /* 1: apply 1/sqrt(N) per FFT (needscale)  */
+    const double *fd_Minv;         /* fd_nb blocks, bs*bs complex (re,im),
+                                      column-major; first fd_nbhalf blocks are
+                                      the half-spectrum ones                  */
+    const long   *fd_perm;         /* fd_nb*bs, full spectrum, FFT order      */
+    const long   *fd_permhf;       /* fd_nbhalf*bs, half spectrum; negative =
+                                      conjugate of element -v                 */
+    double tomoscale;
+} mt_recon_t;
+
+typedef struct mt_setup mt_setup_t;
+
+/* Build geometry + operators + FDPCG (setup_recon_prep, setup_recon_tomo,
+ * fdpcg_prepare: maos/setup_recon_prep.c:1043, maos/setup_recon.c:1212,
+ * maos/fdpcg.c:331). */
+int  mt_setup_create(const mt_parms_t *parms, mt_setup_t **out);
+/* Re-weight layers / r0 and rebuild L2 and FDPCG (setup_recon_tomo_update,
+ * maos/setup_recon.c:747). */
+int  mt_setup_update_cn2(mt_setup_t *s, const double *wt, double r0z);
+/* Fill a POD view; pointers stay valid until the next update or free. */
+int  mt_setup_recon(const mt_setup_t *s, mt_recon_t *out);
+void mt_setup_free(mt_setup_t *s);
+/* Pre-inversion FDPCG blocks (Mhat, same layout as fd_Minv), for checking. */
+int  mt_setup_fdpcg_mhat(const mt_setup_t *s, const double **mhat);
+/* Recon-grid sizes: nx (sum of layer sizes), ng (sum of gradient sizes). */
+int  mt_setup_sizes(const mt_setup_t *s, long *nx, long *ng);
+
+/* ------------------------------------------------------------------ */
+/* 2. GPU solve                                                        */
+/* ------------------------------------------------------------------ */
+
+typedef struct gpu_tomo gpu_tomo_t;
+
+/* Upload operators and allocate device state for `nbatch` independent
+ * systems sharing this geometry (nbatch = 1 is the reference's one solve).
+ * Replaces gpu_setup_recon (cuda/recon.cu:477). */
+int  gpu_tomo_setup(const mt_recon_t *recon, int device, int nbatch,
+                    int maxit, int warm, gpu_tomo_t **out);
+/* Re-upload L2 coefficients and FDPCG blocks after a Cn2 update.
+ * Replaces gpu_update_recon_cn2 (cuda/recon.cu:526). */
+int  gpu_tomo_update_cn2(gpu_tomo_t *g, const mt_recon_t *recon);
+/* Zero the warm-start state. Replaces gpu_recon_reset (cuda/recon.cu:538). */
+int  gpu_tomo_reset(gpu_tomo_t *g);
+void gpu_tomo_free(gpu_tomo_t *g);
+
+/* Full solve with host buffers (double, MAOS dcell layout), one system:
+ * grad -> RHS -> PCG (maxit, warm start from the resident solution) -> x.
+ * cgres receives sqrt(r'r / b'b).  Replaces gpu_tomo (cuda/recon.cu:564). */
+int  gpu_tomo_solve(gpu_tomo_t *g, int isys, const double *grad,
+                    double *x, double *cgres);
+
+/* Batched solve with device buffers already resident in HBM (FP32):
+ * d_grad = nbatch stacked gradient vectors, d_x = nbatch stacked layer
+ * vectors (in: warm start when warm=1; out: solution).  Enqueued on
+ * `stream` (a hipStream_t, 0 = the plan's own stream); no host sync.
+ * mode: 0 = one launch per kernel (plain), 1 = replay a captured hipGraph. */
+int  gpu_tomo_solve_dev(gpu_tomo_t *g, const float *d_grad, float *d_x,
+                        int nbatch, int mode, void *stream);
+/* Residual sqrt(r'r/b'b) of each system after the last solve (host copy). */
+int  gpu_tomo_residuals(gpu_tomo_t *g, double *cgres, int nbatch);
+
+/* Operator applies for parity checks (host double buffers, one system):
+ *   L : xout = beta*xout + alpha*(HX'G'WGHX + L2'L2) xin  (cutomo_grid::L)
+ *   R : xout = alpha * HX'G'W(I-TT*PTT) g                 (cutomo_grid::R)
+ *   P : xout = FDPCG^-1 xin                                (cufdpcg_t::P)
+ */
+int  gpu_tomo_L(gpu_tomo_t *g, const double *xin, double *xout,
+                double alpha, double beta);
+int  gpu_tomo_R(gpu_tomo_t *g, const double *grad, double *xout, double alpha);
+int  gpu_tomo_P(gpu_tomo_t *g, const double *xin, double *xout);
+
+/* Precision of the residual recurrence: mode 1 (default) keeps r, A*p and
+ * the right-hand side in FP64 (operators, FFTs, x, p, z stay FP32); mode 0
+ * keeps them in FP32.  Dot products and alpha/beta are FP64 in both.  A
+ * 30-iteration FDPCG solve of the 30 m system drifts ~5e-4 from the FP64
+ * solve with mode 0 and stays ~1e-5 with mode 1. */
+int  gpu_tomo_precision(gpu_tomo_t *g, int mode);
+
+/* Sizes of one system's vectors. */
+int  gpu_tomo_sizes(const gpu_tomo_t *g, long *nx, long *ng);
+
+/* Average device duration (ms) per launch of each kernel class over the
+ * last gpu_tomo_solve_dev call made with profiling enabled (HIP events on the
+ * stream the kernels run on), and the number of launches of that class.
+ * names/ms/calls are parallel arrays of length *n (in: capacity). */
+int  gpu_tomo_profile(gpu_tomo_t *g, int enable);
+int  gpu_tomo_profile_read(gpu_tomo_t *g, const char **names, double *ms,
+                           double *calls, int *n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MAOS_TOMO_H */

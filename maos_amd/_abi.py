@@ -1,0 +1,121 @@
+"""ctypes view of include/maos_tomo.h and the loader of libmaos_tomo.so.
+
+The library is built in-tree by maos_amd.build; importing this module never
+falls back to anything else: a missing library raises at load time.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libmaos_tomo.so")
+
+MT_OK = 0
+MT_EINVAL = -1
+MT_ENOMEM = -2
+MT_EHIP = -3
+MT_ENOTFINITE = -4
+MT_ESTATE = -5
+MT_EFULL = -6
+MT_NOTCONV = 1
+
+dp = C.POINTER(C.c_double)
+lp = C.POINTER(C.c_long)
+ip = C.POINTER(C.c_int)
+
+
+class MtPowfs(C.Structure):
+    _fields_ = [("dsa", C.c_double), ("dx", C.c_double), ("saat", C.c_double),
+                ("nearecon", C.c_double), ("hs", C.c_double), ("trs", C.c_int),
+                ("lo", C.c_int), ("skip", C.c_int), ("dtrat", C.c_int)]
+
+
+class MtWfs(C.Structure):
+    _fields_ = [("thetax", C.c_double), ("thetay", C.c_double), ("powfs", C.c_int)]
+
+
+class MtParms(C.Structure):
+    _fields_ = [("aper_d", C.c_double), ("aper_din", C.c_double), ("npsr", C.c_int),
+                ("ht", dp), ("wt", dp), ("os", ip), ("atmr_dx", C.c_double),
+                ("r0z", C.c_double), ("L0", C.c_double), ("za", C.c_double),
+                ("tomo_pos", C.c_int), ("tomo_guard", C.c_int), ("tomo_cone", C.c_int),
+                ("tomo_cxxscale", C.c_double), ("tomo_precond", C.c_int),
+                ("recon_split", C.c_int), ("tomo_splitlrt", C.c_int),
+                ("npowfs", C.c_int), ("powfs", C.POINTER(MtPowfs)),
+                ("nwfs", C.c_int), ("wfs", C.POINTER(MtWfs)),
+                ("ndir", C.c_int), ("dir_thetax", dp), ("dir_thetay", dp), ("dir_hs", dp),
+                ("tomoscale", C.c_double)]
+
+
+class MtRecon(C.Structure):
+    _fields_ = [("npsr", C.c_int), ("xnx", lp), ("xny", lp), ("xox", dp), ("xoy", dp),
+                ("xdx", dp), ("ht", dp), ("pnx", C.c_long), ("pny", C.c_long),
+                ("pox", C.c_double), ("poy", C.c_double), ("pdx", C.c_double),
+                ("nwfs", C.c_int), ("thetax", dp), ("thetay", dp), ("hs", dp), ("skip", ip),
+                ("nsa", lp), ("dsa", dp), ("saloc", C.POINTER(dp)), ("saa", C.POINTER(dp)),
+                ("gp_p", C.POINTER(lp)), ("gp_i", C.POINTER(lp)), ("gp_x", C.POINTER(dp)),
+                ("pos", C.c_int), ("neai", C.POINTER(dp)), ("trs", ip), ("ptt", C.POINTER(dp)),
+                ("ptt_lhs", C.c_int), ("l2c", dp), ("precond", C.c_int), ("fd_bs", C.c_int),
+                ("fd_nb", C.c_long), ("fd_nbhalf", C.c_long), ("fd_scale", C.c_int),
+                ("fd_Minv", dp), ("fd_perm", lp), ("fd_permhf", lp), ("tomoscale", C.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libmaos_tomo.so (built by maos_amd.build); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libmaos_tomo.so is not built (%s); run `python -m maos_amd.build`"
+                           % LIB_PATH)
+    L = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    L.mt_last_error.restype = C.c_char_p
+    L.mt_setup_create.argtypes = [C.POINTER(MtParms), C.POINTER(vp)]
+    L.mt_setup_update_cn2.argtypes = [vp, dp, C.c_double]
+    L.mt_setup_recon.argtypes = [vp, C.POINTER(MtRecon)]
+    L.mt_setup_free.argtypes = [vp]
+    L.mt_setup_free.restype = None
+    L.mt_setup_fdpcg_mhat.argtypes = [vp, C.POINTER(dp)]
+    L.mt_setup_sizes.argtypes = [vp, lp, lp]
+    L.gpu_tomo_setup.argtypes = [C.POINTER(MtRecon), C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
+    L.gpu_tomo_update_cn2.argtypes = [vp, C.POINTER(MtRecon)]
+    L.gpu_tomo_reset.argtypes = [vp]
+    L.gpu_tomo_free.argtypes = [vp]
+    L.gpu_tomo_free.restype = None
+    L.gpu_tomo_solve.argtypes = [vp, C.c_int, dp, dp, dp]
+    L.gpu_tomo_solve_dev.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp]
+    L.gpu_tomo_residuals.argtypes = [vp, dp, C.c_int]
+    L.gpu_tomo_L.argtypes = [vp, dp, dp, C.c_double, C.c_double]
+    L.gpu_tomo_R.argtypes = [vp, dp, dp, C.c_double]
+    L.gpu_tomo_P.argtypes = [vp, dp, dp]
+    L.gpu_tomo_sizes.argtypes = [vp, lp, lp]
+    L.gpu_tomo_precision.argtypes = [vp, C.c_int]
+    L.gpu_tomo_profile.argtypes = [vp, C.c_int]
+    L.gpu_tomo_profile_read.argtypes = [vp, C.POINTER(C.c_char_p), dp, dp, ip]
+    _lib = L
+    return L
+
+
+# Every symbol include/maos_tomo.h declares (checked by the CPU tests).
+EXPORTS = [
+    "mt_last_error", "mt_setup_create", "mt_setup_update_cn2", "mt_setup_recon",
+    "mt_setup_free", "mt_setup_fdpcg_mhat", "mt_setup_sizes", "gpu_tomo_setup",
+    "gpu_tomo_update_cn2", "gpu_tomo_reset", "gpu_tomo_free", "gpu_tomo_solve",
+    "gpu_tomo_solve_dev", "gpu_tomo_residuals", "gpu_tomo_L", "gpu_tomo_R", "gpu_tomo_P",
+    "gpu_tomo_sizes", "gpu_tomo_profile", "gpu_tomo_profile_read", "gpu_tomo_precision",
+]
+
+
+class MtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (status %d)" % (msg, code))
+        self.code = code
+
+
+def check(rc, what):
+    if rc < 0:
+        raise MtError(rc, "%s: %s" % (what, (lib().mt_last_error() or b"").decode()))
+    return rc

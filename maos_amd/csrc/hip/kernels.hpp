@@ -1,0 +1,92 @@
+// CDNA4 (gfx950) kernels of the tomography PCG/FDPCG solve.
+//
+// Data layout in HBM, per system s of a batch (systems share one geometry):
+//   layer vectors  x, r, z, p, Ap, b : s*nxtot + xoff[l] + ix + iy*xnx[l]
+//   gradients      g                 : s*ngtot + goff[w] + {x slopes, y slopes}
+//   WFS fields     q                 : (s*nwfs + w)*nploc + ix + iy*pnx
+//   spectra        S (complex)       : s*nspec + soff[l] + kx*ny + ky,
+//                                      kx in [0, nx/2] (half spectrum,
+//                                      column-major so FDPCG columns are
+//                                      contiguous); forward spectrum of r
+//                                      in the residual precision, inverse
+//                                      spectrum in FP32
+// All gathers, no atomics: every output element is written by exactly one
+// thread, so results do not depend on scheduling (a system computes the same
+// bits alone or inside a batch).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace mtk {
+
+constexpr int MAXL = 16;      // layers
+constexpr int MAXW = 16;      // wavefront sensors (active)
+constexpr int MAXST = 16;     // FFT stages
+constexpr int TILE = 16;      // output tile edge of the WFS / layer passes
+constexpr int TAPS = 4;       // max ploc samples landing on one layer node per axis
+
+// Geometry shared by all systems of a plan (device copy).
+struct Geom {
+    int npsr, nwfs, pos, np;      // np = pos+1
+    int pnx, pny, nploc;
+    long nxtot, ngtot;
+    int xnx[MAXL], xny[MAXL];
+    long xoff[MAXL + 1];
+    int nsa[MAXW];
+    long goff[MAXW + 1];
+    float l2c[MAXL];
+    int ttr[MAXW];
+    // ray tables, per (w,l): forward per ploc index, transposed per layer index
+    // forward x: hx_nx[(w*MAXL+l)*pnx + i] = (n0, n1) (n0 < 0: no ray), hx_fx: fraction
+    const int2 *hx_nx, *hx_ny;
+    const float *hx_fx, *hx_fy;
+    const int4 *hxt_x, *hxt_y;    // per (w,l) per layer index: up to 4 ploc indices (-1 pad)
+    long hxt_xoff[MAXL + 1];      // offset of layer l in a (w,*) transposed x table
+    // subapertures
+    const int *saidx;             // per WFS per ploc point: subaperture with that lower-left corner, -1
+    const float *gpx, *gpy;       // per WFS per subap np*np coefficients
+    const float4 *neai;           // per WFS per subap (xx, yy, xy, 0)
+    const float *ptt;             // per WFS 2 x 2nsa (column-major), ttr only
+    long saoff[MAXW + 1];         // subap offset of WFS w in gpx/neai (units of subaps)
+    // FDPCG
+    int precond, bs, scale;
+    int zero_dc;                  // zero the DC bin of the residual spectrum (see k_col)
+    int fnx, fny;                 // FFT sizes (all layers equal)
+    long nspec;                   // complex entries per system (sum over layers (nx/2+1)*ny)
+    long soff[MAXL + 1];
+    float fscale;                 // 1/sqrt(nx*ny) when scale
+    int nstx, nsty;               // FFT stages
+    int radx[MAXST], rady[MAXST];
+    const float2 *twx, *twy;      // exp(-2 pi i k/n)
+    const double2 *twx64, *twy64; // same in FP64 (forward transform of the FP64 residual)
+    const float2 *Minv;           // nbhalf blocks bs*bs, column-major
+    int ngroup;                   // column groups of the block apply
+    const int *grp_col;           // per group: first column index into colkx
+    const int *colkx;             // column kx list (per group, contiguous)
+    const int *grp_blk;           // per group: first block index into blkid (ngroup+1)
+    const int *blkid;             // block ids ordered by group
+    const int *blkpos;            // per block (nbhalf*bs): LDS position | (conj << 30) | (nowrite << 29)
+    int maxcol;                   // max columns in one group
+    // row blocks of the row FFT kernels
+    int nrowblk;
+    const int2 *rowblk;           // (layer, first row), ROWS rows each
+};
+
+constexpr int ROWS = 16;          // rows per row-FFT workgroup (8 complex FFTs)
+
+// Per-batch scalar state (device, FP64).
+struct Scal {
+    double *part_pap;   // [nsys][maxpart]
+    double *part_rz;    // [nsys][maxpart]
+    double *part_rr;    // [nsys][maxpart]
+    double *part_bb;    // [nsys][maxpart]
+    double *rz;         // [nsys][maxit+2]
+    double *res;        // [nsys]
+    float *tt;          // [nsys][nwfs][2]
+    int maxpart, maxit;
+    int npart_layer;    // partial count written by the layer kernel
+    int npart_row;      // partial count written by the inverse row kernel
+};
+
+}  // namespace mtk

@@ -1,0 +1,228 @@
+"""Host-side mirror of the reference's tomography interface.
+
+  Recon        ~ setup_recon_prep + setup_recon_tomo + fdpcg_prepare
+                 (maos/setup_recon_prep.c:1043, maos/setup_recon.c:1212,
+                 maos/fdpcg.c:331): FP64 operators in C++ (libmaos_tomo.so).
+  GpuTomo      ~ gpu_setup_recon / gpu_tomo / gpu_recon_reset /
+                 gpu_update_recon_cn2 (cuda/recon.cu:477-582) and the operator
+                 entry points cutomo_grid::L/R (cuda/tomo.cu:607-677),
+                 cufdpcg_t::P (cuda/fdpcg.cu:168-226).
+
+Vectors are numpy float64 arrays in MAOS dcell order (see include/maos_tomo.h).
+The GPU path runs only through the HIP library; nothing here computes the
+solve on the CPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from .configs import Parms
+
+
+def _arr(ptr, n, dtype=np.float64):
+    if not ptr or n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_abi.dp)
+
+
+class Recon:
+    """Reconstruction geometry + operators built from .conf parameters."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self._parms = Parms(cfg)
+        L = _abi.lib()
+        h = C.c_void_p()
+        _abi.check(L.mt_setup_create(C.byref(self._parms.p), C.byref(h)), "mt_setup_create")
+        self._h = h
+        self._refresh()
+
+    def _refresh(self):
+        L = _abi.lib()
+        self.rec = _abi.MtRecon()
+        _abi.check(L.mt_setup_recon(self._h, C.byref(self.rec)), "mt_setup_recon")
+        nx, ng = C.c_long(), C.c_long()
+        L.mt_setup_sizes(self._h, C.byref(nx), C.byref(ng))
+        self.nx, self.ng = nx.value, ng.value
+
+    def update_cn2(self, wt, r0z):
+        """setup_recon_tomo_update: new layer weights / r0, rebuild L2 + FDPCG."""
+        w = np.ascontiguousarray(wt, dtype=np.float64)
+        _abi.check(_abi.lib().mt_setup_update_cn2(self._h, _dp(w), float(r0z)), "update_cn2")
+        self._refresh()
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _abi.lib().mt_setup_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- numpy views (copies) of the operators -------------------------
+    @property
+    def npsr(self):
+        return self.rec.npsr
+
+    @property
+    def nwfs(self):
+        return self.rec.nwfs
+
+    def layer_shapes(self):
+        r = self.rec
+        return [(r.xny[l], r.xnx[l]) for l in range(r.npsr)]
+
+    def layer_offsets(self):
+        off = [0]
+        for ny, nx in self.layer_shapes():
+            off.append(off[-1] + nx * ny)
+        return off
+
+    def grad_offsets(self):
+        off = [0]
+        for w in range(self.nwfs):
+            off.append(off[-1] + 2 * self.rec.nsa[w])
+        return off
+
+    def nsa(self, w):
+        return self.rec.nsa[w]
+
+    def saloc(self, w):
+        n = self.rec.nsa[w]
+        a = _arr(self.rec.saloc[w], 2 * n)
+        return a[:n], a[n:]
+
+    def saa(self, w):
+        return _arr(self.rec.saa[w], self.rec.nsa[w])
+
+    def gp(self, w):
+        """GP of WFS w as scipy.sparse.csc_matrix (2nsa x nploc), MAOS dsp."""
+        import scipy.sparse as sp
+        r = self.rec
+        nploc = r.pnx * r.pny
+        p = _arr(r.gp_p[w], nploc + 1, np.int64)
+        i = _arr(r.gp_i[w], p[-1], np.int64)
+        x = _arr(r.gp_x[w], p[-1])
+        return sp.csc_matrix((x, i, p), shape=(2 * r.nsa[w], nploc))
+
+    def neai(self, w):
+        return _arr(self.rec.neai[w], 3 * self.rec.nsa[w]).reshape(-1, 3)
+
+    def ptt(self, w):
+        if not self.rec.trs[w] or not self.rec.ptt[w]:
+            return None
+        return _arr(self.rec.ptt[w], 4 * self.rec.nsa[w]).reshape(-1, 2).T
+
+    def l2c(self):
+        return _arr(self.rec.l2c, self.rec.npsr)
+
+    def fdpcg(self):
+        r = self.rec
+        bs, nb, nbh = r.fd_bs, r.fd_nb, r.fd_nbhalf
+        M = _arr(r.fd_Minv, nb * bs * bs * 2).view(np.complex128).reshape(nb, bs, bs)
+        Minv = np.transpose(M, (0, 2, 1)).copy()  # column-major -> [b][row][col]
+        mh = _abi.dp()
+        _abi.lib().mt_setup_fdpcg_mhat(self._h, C.byref(mh))
+        Mh = _arr(mh, nb * bs * bs * 2).view(np.complex128).reshape(nb, bs, bs)
+        Mhat = np.transpose(Mh, (0, 2, 1)).copy()
+        return dict(bs=bs, nb=nb, nbhalf=nbh, scale=r.fd_scale, Minv=Minv, Mhat=Mhat,
+                    perm=_arr(r.fd_perm, nb * bs, np.int64),
+                    permhf=_arr(r.fd_permhf, nbh * bs, np.int64))
+
+
+class GpuTomo:
+    """GPU tomography plan for `nbatch` systems sharing one geometry."""
+
+    def __init__(self, recon, device=0, nbatch=1, maxit=30, warm=0):
+        self.recon = recon
+        L = _abi.lib()
+        h = C.c_void_p()
+        _abi.check(L.gpu_tomo_setup(C.byref(recon.rec), device, nbatch, maxit, int(warm),
+                                    C.byref(h)), "gpu_tomo_setup")
+        self._h = h
+        self.nbatch, self.maxit, self.warm = nbatch, maxit, warm
+        nx, ng = C.c_long(), C.c_long()
+        L.gpu_tomo_sizes(h, C.byref(nx), C.byref(ng))
+        self.nx, self.ng = nx.value, ng.value
+
+    def precision(self, mode):
+        """1: FP64 residual recurrence (default); 0: FP32 residual."""
+        _abi.check(_abi.lib().gpu_tomo_precision(self._h, int(mode)), "gpu_tomo_precision")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _abi.lib().gpu_tomo_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def tomo(self, grad):
+        """gpu_tomo: one solve from host gradients. Returns (x, cgres, status)."""
+        g = np.ascontiguousarray(grad, dtype=np.float64)
+        if g.size != self.ng:
+            raise ValueError("gradient vector has %d entries, expected %d" % (g.size, self.ng))
+        x = np.zeros(self.nx)
+        res = C.c_double()
+        rc = _abi.lib().gpu_tomo_solve(self._h, 0, _dp(g), _dp(x), C.byref(res))
+        _abi.check(rc, "gpu_tomo_solve")
+        return x, res.value, rc
+
+    def reset(self):
+        _abi.check(_abi.lib().gpu_tomo_reset(self._h), "gpu_tomo_reset")
+
+    def update_cn2(self, recon):
+        _abi.check(_abi.lib().gpu_tomo_update_cn2(self._h, C.byref(recon.rec)), "update_cn2")
+
+    def L(self, xin, alpha=1.0, beta=0.0, xout=None):
+        xi = np.ascontiguousarray(xin, dtype=np.float64)
+        xo = np.zeros(self.nx) if xout is None else np.ascontiguousarray(xout, dtype=np.float64)
+        _abi.check(_abi.lib().gpu_tomo_L(self._h, _dp(xi), _dp(xo), alpha, beta), "gpu_tomo_L")
+        return xo
+
+    def R(self, grad, alpha=1.0):
+        g = np.ascontiguousarray(grad, dtype=np.float64)
+        xo = np.zeros(self.nx)
+        _abi.check(_abi.lib().gpu_tomo_R(self._h, _dp(g), _dp(xo), alpha), "gpu_tomo_R")
+        return xo
+
+    def P(self, xin):
+        xi = np.ascontiguousarray(xin, dtype=np.float64)
+        xo = np.zeros(self.nx)
+        _abi.check(_abi.lib().gpu_tomo_P(self._h, _dp(xi), _dp(xo)), "gpu_tomo_P")
+        return xo
+
+    def solve_dev(self, d_grad, d_x, nbatch=None, mode=0, stream=None):
+        """Batched solve on device buffers (int device pointers, float32)."""
+        nb = self.nbatch if nbatch is None else nbatch
+        _abi.check(_abi.lib().gpu_tomo_solve_dev(self._h, C.c_void_p(d_grad), C.c_void_p(d_x), nb,
+                                                 mode, C.c_void_p(stream or 0)),
+                   "gpu_tomo_solve_dev")
+
+    def residuals(self, nbatch=None):
+        nb = self.nbatch if nbatch is None else nbatch
+        out = np.zeros(nb)
+        _abi.check(_abi.lib().gpu_tomo_residuals(self._h, _dp(out), nb), "gpu_tomo_residuals")
+        return out
+
+    def profile(self, enable=True):
+        _abi.lib().gpu_tomo_profile(self._h, 1 if enable else 0)
+
+    def profile_read(self):
+        n = C.c_int(16)
+        names = (C.c_char_p * 16)()
+        ms = (C.c_double * 16)()
+        calls = (C.c_double * 16)()
+        _abi.lib().gpu_tomo_profile_read(self._h, names, ms, calls, C.byref(n))
+        return {names[i].decode(): (ms[i], int(calls[i])) for i in range(n.value)}
