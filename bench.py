@@ -1,0 +1,203 @@
+"""Benchmark: tomography solves/s and p99 latency on the 30 m / 6 LGS /
+6 layer / 30-iteration FDPCG-PCG solve, one MI355X.
+
+A step is one call of the solve on one batch of synthetic inputs already
+resident in HBM.  Modes measured back to back on the same inputs:
+  plain    one system per call, one launch per kernel (the "plain GPU" solve)
+  graph    one system per call, the whole solve replayed as one hipGraph
+  batched  --batch systems per call, one hipGraph (the headline `value`)
+Latency is per call (a system in a batch waits for its whole batch).
+
+Also reported: relative L2 of the GPU solution against the FP64 CPU oracle
+(same inputs, same iteration count), a roofline entry for the dominant kernel
+(algorithmic bytes per launch / HIP-event duration on the launch stream), and
+the FP64 CPU oracle timed on the host cores (cpu_baseline).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "tomo solves/s + p99 latency (30m, 6 LGS, 6 layers, 30 PCG it) vs plain GPU"
+
+
+def algorithmic_bytes(r, G):
+    """Unique bytes each kernel class must move per launch for one system
+    (FP32 x/p/z, FP64 r/Ap/b/q, complex128 forward spectrum, complex64
+    inverse spectrum).  Sizes from the plan geometry."""
+    rec = r.rec
+    nx = r.nx
+    nw = sum(1 for w in range(rec.nwfs) if not rec.skip[w])
+    nploc = rec.pnx * rec.pny
+    nsa = sum(rec.nsa[w] for w in range(rec.nwfs))
+    np2 = (rec.pos + 1) ** 2
+    nxl, nyl = rec.xnx[0], rec.xny[0]
+    nspec = rec.npsr * (nxl // 2 + 1) * nyl
+    bs, nbh = rec.fd_bs, rec.fd_nbhalf
+    return {
+        # p (read once) + GP coefs + NEA + subap map, q written (FP64)
+        "k_wfs": 4 * nx + nsa * (2 * np2 * 4 + 16) + 4 * nw * nploc + 8 * nw * nploc,
+        # q (FP64) + p read, Ap (FP64) written
+        "k_layer": 8 * nw * nploc + 4 * nx + 8 * nx,
+        # x, p, r, Ap read; x, r, forward spectrum (complex128) written
+        "k_row": (4 + 4 + 8 + 8) * nx + (4 + 8) * nx + 16 * nspec,
+        # forward spectrum read, half blocks read, inverse spectrum written
+        "k_col": 16 * nspec + 8 * nbh * bs * bs + 8 * nspec,
+        # inverse spectrum + r read, z written
+        "k_irow": 8 * nspec + 8 * nx + 4 * nx,
+        # z, p read, p written
+        "k_p": 12 * nx,
+    }
+
+
+def cpu_baseline(r, grads, maxit, seconds):
+    from oracle.oracle import Oracle
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    o = Oracle(r, nthread=cores)
+    x0, _ = o.solve(grads[0], maxit)  # warm-up + reference solution
+    o.solve(grads[1 % len(grads)], maxit)
+    t0 = time.perf_counter()
+    n = 0
+    lat = []
+    while True:
+        t = time.perf_counter()
+        o.solve(grads[n % len(grads)], maxit)
+        lat.append(time.perf_counter() - t)
+        n += 1
+        if time.perf_counter() - t0 > seconds and n >= 3:
+            break
+    el = time.perf_counter() - t0
+    return dict(value=n / el, unit="solves/s", cores=cores, kind="port",
+                sample="%d cold 30-iteration FDPCG solves of the 30 m system (RHS + PCG, FP64, "
+                       "OpenMP over WFS/layers/blocks), %.1f s; p50 %.1f ms p99 %.1f ms"
+                       % (n, el, 1e3 * np.percentile(lat, 50), 1e3 * np.percentile(lat, 99))), x0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the solve runs on one MI355X; "
+              "independent solves would shard across GPUs with no communication). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from maos_amd import configs, tomo, workload
+
+    cfg = configs.large30()
+    maxit = cfg["maxit"]
+    B = args.batch
+    r = tomo.Recon(cfg)
+    grads = [workload.make_inputs(cfg, r, seed) for seed in range(1, B + 1)]
+    dev = torch.device("cuda:0")
+    d_g = torch.tensor(np.stack(grads).astype(np.float32), device=dev).contiguous()
+    d_x = torch.zeros((B, r.nx), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    sp = stream.cuda_stream
+
+    def run_mode(plan, nb, mode, steps, warmup):
+        lat = []
+        for i in range(warmup + steps):
+            j = (i * nb) % B
+            if j + nb > B:
+                j = 0
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            plan.solve_dev(d_g[j].data_ptr(), d_x[j].data_ptr(), nbatch=nb, mode=mode, stream=sp)
+            torch.cuda.synchronize()
+            if i >= warmup:
+                lat.append(time.perf_counter() - t)
+        lat = np.array(lat)
+        return dict(solves_per_s=nb * len(lat) / lat.sum(), p50_ms=1e3 * float(np.percentile(lat, 50)),
+                    p99_ms=1e3 * float(np.percentile(lat, 99)), ms_per_step=1e3 * float(lat.mean()))
+
+    plain = tomo.GpuTomo(r, nbatch=1, maxit=maxit)
+    batched = tomo.GpuTomo(r, nbatch=B, maxit=maxit)
+    res_plain = run_mode(plain, 1, 0, args.steps, args.warmup)
+    res_graph = run_mode(plain, 1, 1, args.steps, args.warmup)
+    nsteps_b = max(10, args.steps // 4)
+    # timed region of the headline mode: barrier + synchronize on both sides
+    torch.cuda.synchronize()
+    res_batch = run_mode(batched, B, 1, nsteps_b, max(2, args.warmup // 4))
+    torch.cuda.synchronize()
+
+    # accuracy of system 0 (batched path) against the FP64 CPU oracle
+    x_gpu = d_x[0].double().cpu().numpy()
+    res_gpu = batched.residuals()[0]
+
+    # per-kernel profile of one batched solve (plain launches, HIP events on
+    # the launch stream)
+    batched.profile(True)
+    batched.solve_dev(d_g.data_ptr(), d_x.data_ptr(), nbatch=B, mode=0, stream=sp)
+    prof = batched.profile_read()
+    batched.profile(False)
+    torch.cuda.synchronize()
+    ab = algorithmic_bytes(r, batched)
+    kernels = {}
+    for k, (ms, calls) in prof.items():
+        by = ab.get(k, 0) * B
+        kernels[k] = dict(avg_us=1e3 * ms, calls=calls, total_ms=ms * calls,
+                          alg_MB=by / 1e6, GBs=(by / (ms * 1e-3) / 1e9) if ms > 0 else 0.0)
+    dom = max((k for k in kernels if k in ab), key=lambda k: kernels[k]["total_ms"])
+    kd = kernels[dom]
+    roofline = dict(bound="hbm", achieved=round(kd["GBs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(kd["GBs"] / HBM_PEAK_GBS, 4), traffic=None, kernel=dom,
+                    avg_us=round(kd["avg_us"], 2), alg_bytes_per_launch=int(ab[dom] * B))
+
+    cpu = None
+    err = None
+    if not args.no_cpu:
+        cpu, x_ref = cpu_baseline(r, grads, maxit, args.cpu_seconds)
+        err = float(np.linalg.norm(x_gpu - x_ref) / np.linalg.norm(x_ref))
+
+    out = {
+        "metric": METRIC,
+        "value": round(res_batch["solves_per_s"], 2),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": nsteps_b,
+        "warmup": max(2, args.warmup // 4),
+        "ms_per_step": round(res_batch["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 operators/FFT-inverse, f64 residual/forward-FFT/reductions",
+        "data": "synthetic (seeded von Karman screens, geometric SH slopes + 0.05\" noise)",
+        "config": {"workload": "large30: D=30 m, 6 LGS at 90 km, 6 layers 0-16 km, 60x60 subaps, "
+                               "xloc 6x96x96, 30 FDPCG-PCG iterations, cold start",
+                   "batch": B, "global_batch": B, "parallelism": "batch of independent solves"},
+        "p50_ms": round(res_batch["p50_ms"], 4),
+        "p99_ms": round(res_batch["p99_ms"], 4),
+        "modes": {"plain": res_plain, "graph": res_graph, "batched": res_batch},
+        "speedup_vs_plain": round(res_batch["solves_per_s"] / res_plain["solves_per_s"], 2),
+        "accuracy": {"rel_l2_vs_fp64_oracle": err, "cg_residual": float(res_gpu)},
+        "roofline": roofline,
+        "kernels": kernels,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()
