@@ -164,6 +164,38 @@ int  mt_setup_fdpcg_mhat(const mt_setup_t *s, const double **mhat);
 /* Recon-grid sizes: nx (sum of layer sizes), ng (sum of gradient sizes). */
 int  mt_setup_sizes(const mt_setup_t *s, long *nx, long *ng);
 
+/* Lower-level FDPCG construction from explicit inputs: the pieces that
+ * fdpcg_prepare (maos/fdpcg.c:331-580) reads from RECON_T / POWFS_T.
+ * mt_setup_create calls the same code.  Outputs are allocated here and
+ * released with mt_fdpcg_free. */
+typedef struct {
+    int    nps;                    /* layers                                  */
+    const long   *nx, *ny, *os;    /* layer grid sizes and oversampling       */
+    int    pos;                    /* tomo.pos                                */
+    double dxp;                    /* ploc spacing                            */
+    const double *ht;              /* layer heights                           */
+    const double *l2cf;            /* laplacian_coef * sqrt(cxxscale*TOMOSCALE) */
+    long   nsa;                    /* subapertures of the high-order powfs    */
+    double dsa;
+    const double *salocx, *salocy, *saa;
+    int    nwfs;                   /* WFS of the high-order powfs             */
+    const double *thetax, *thetay; /* radian                                  */
+    const double *neam;            /* mean NEA / sqrt(TOMOSCALE) (recon->neam) */
+} mt_fdpcg_in_t;
+
+typedef struct {
+    int    bs;
+    long   nb, nbhalf;
+    int    scale;
+    double *Mhat;                  /* nb*bs*bs complex, before inversion      */
+    double *Minv;                  /* nb*bs*bs complex                        */
+    long   *perm;                  /* nb*bs                                   */
+    long   *permhf;                /* nbhalf*bs                               */
+} mt_fdpcg_out_t;
+
+int  mt_fdpcg_prepare(const mt_fdpcg_in_t *in, mt_fdpcg_out_t *out);
+void mt_fdpcg_free(mt_fdpcg_out_t *out);
+
 /* ------------------------------------------------------------------ */
 /* 2. GPU solve                                                        */
 /* ------------------------------------------------------------------ */

@@ -59,6 +59,18 @@ class MtRecon(C.Structure):
                 ("fd_Minv", dp), ("fd_perm", lp), ("fd_permhf", lp), ("tomoscale", C.c_double)]
 
 
+class MtFdpcgIn(C.Structure):
+    _fields_ = [("nps", C.c_int), ("nx", lp), ("ny", lp), ("os", lp), ("pos", C.c_int),
+                ("dxp", C.c_double), ("ht", dp), ("l2cf", dp), ("nsa", C.c_long), ("dsa", C.c_double),
+                ("salocx", dp), ("salocy", dp), ("saa", dp), ("nwfs", C.c_int), ("thetax", dp),
+                ("thetay", dp), ("neam", dp)]
+
+
+class MtFdpcgOut(C.Structure):
+    _fields_ = [("bs", C.c_int), ("nb", C.c_long), ("nbhalf", C.c_long), ("scale", C.c_int),
+                ("Mhat", dp), ("Minv", dp), ("perm", lp), ("permhf", lp)]
+
+
 _lib = None
 
 
@@ -80,6 +92,9 @@ def lib():
     L.mt_setup_free.restype = None
     L.mt_setup_fdpcg_mhat.argtypes = [vp, C.POINTER(dp)]
     L.mt_setup_sizes.argtypes = [vp, lp, lp]
+    L.mt_fdpcg_prepare.argtypes = [C.POINTER(MtFdpcgIn), C.POINTER(MtFdpcgOut)]
+    L.mt_fdpcg_free.argtypes = [C.POINTER(MtFdpcgOut)]
+    L.mt_fdpcg_free.restype = None
     L.gpu_tomo_setup.argtypes = [C.POINTER(MtRecon), C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
     L.gpu_tomo_update_cn2.argtypes = [vp, C.POINTER(MtRecon)]
     L.gpu_tomo_reset.argtypes = [vp]
@@ -102,7 +117,8 @@ def lib():
 # Every symbol include/maos_tomo.h declares (checked by the CPU tests).
 EXPORTS = [
     "mt_last_error", "mt_setup_create", "mt_setup_update_cn2", "mt_setup_recon",
-    "mt_setup_free", "mt_setup_fdpcg_mhat", "mt_setup_sizes", "gpu_tomo_setup",
+    "mt_setup_free", "mt_setup_fdpcg_mhat", "mt_setup_sizes", "mt_fdpcg_prepare",
+    "mt_fdpcg_free", "gpu_tomo_setup",
     "gpu_tomo_update_cn2", "gpu_tomo_reset", "gpu_tomo_free", "gpu_tomo_solve",
     "gpu_tomo_solve_dev", "gpu_tomo_residuals", "gpu_tomo_L", "gpu_tomo_R", "gpu_tomo_P",
     "gpu_tomo_sizes", "gpu_tomo_profile", "gpu_tomo_profile_read", "gpu_tomo_precision",

@@ -16,7 +16,9 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <cstdlib>
 #include <memory>
+#include <type_traits>
 
 #include "setup_internal.hpp"
 
@@ -450,5 +452,58 @@ int mt_setup_sizes(const mt_setup_t *s, long *nx, long *ng) {
 }
 
 void mt_setup_free(mt_setup_t *s) { delete s; }
+
+int mt_fdpcg_prepare(const mt_fdpcg_in_t *in, mt_fdpcg_out_t *out) {
+    if (!in || !out || in->nps <= 0 || !in->nx || !in->ny || !in->os || !in->ht || !in->l2cf ||
+        in->nsa <= 0 || !in->salocx || !in->salocy || !in->saa || in->nwfs <= 0 || !in->thetax ||
+        !in->thetay || !in->neam)
+        return MT_EINVAL;
+    std::memset(out, 0, sizeof(*out));
+    return guarded([&] {
+        SaGeom sa;
+        sa.dsa = in->dsa;
+        sa.locx.assign(in->salocx, in->salocx + in->nsa);
+        sa.locy.assign(in->salocy, in->salocy + in->nsa);
+        sa.saa.assign(in->saa, in->saa + in->nsa);
+        FdpcgIn f;
+        f.nps = in->nps;
+        f.nx.assign(in->nx, in->nx + in->nps);
+        f.ny.assign(in->ny, in->ny + in->nps);
+        f.os.assign(in->os, in->os + in->nps);
+        f.pos = in->pos;
+        f.dxp = in->dxp;
+        f.ht.assign(in->ht, in->ht + in->nps);
+        f.l2cf.assign(in->l2cf, in->l2cf + in->nps);
+        f.sa = &sa;
+        f.thetax.assign(in->thetax, in->thetax + in->nwfs);
+        f.thetay.assign(in->thetay, in->thetay + in->nwfs);
+        f.neam.assign(in->neam, in->neam + in->nwfs);
+        FdpcgOut o = fdpcg_prepare(f);
+        auto dup = [](const auto &v) {
+            using T = typename std::decay<decltype(v)>::type::value_type;
+            T *p = (T *)std::malloc(sizeof(T) * (v.size() ? v.size() : 1));
+            if (!p) throw std::bad_alloc();
+            std::copy(v.begin(), v.end(), p);
+            return p;
+        };
+        out->bs = o.bs;
+        out->nb = o.nb;
+        out->nbhalf = o.nbhalf;
+        out->scale = o.scale;
+        out->Mhat = dup(o.Mhat);
+        out->Minv = dup(o.Minv);
+        out->perm = dup(o.perm);
+        out->permhf = dup(o.permhf);
+    });
+}
+
+void mt_fdpcg_free(mt_fdpcg_out_t *out) {
+    if (!out) return;
+    std::free(out->Mhat);
+    std::free(out->Minv);
+    std::free(out->perm);
+    std::free(out->permhf);
+    std::memset(out, 0, sizeof(*out));
+}
 
 }  // extern "C"

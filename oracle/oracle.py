@@ -41,6 +41,8 @@ def lib():
         L.or_hx.restype = None
         L.or_hxt.argtypes = [vp, C.c_int, dp, dp]
         L.or_hxt.restype = None
+        L.or_l2.argtypes = [vp, C.c_int, dp, dp, C.c_double]
+        L.or_l2.restype = None
         L.or_fft1d.argtypes = [dp, dp, C.c_int, C.c_int]
         L.or_fft1d.restype = None
         _lib = L
@@ -102,6 +104,12 @@ class Oracle:
         x = np.zeros(self.nx) if x0 is None else np.array(x0, dtype=np.float64)
         res = lib().or_solve(self._h, _dp(gg), _dp(x), 0 if x0 is None else 1, int(maxit))
         return x, res
+
+    def l2(self, l, xl, alpha=1.0):
+        xi = np.ascontiguousarray(xl, dtype=np.float64)
+        out = np.zeros_like(xi)
+        lib().or_l2(self._h, int(l), _dp(xi), _dp(out), alpha)
+        return out
 
     def hx(self, w, x):
         r = self.recon.rec

@@ -531,6 +531,13 @@ static void l2_apply(const or_t *o, int l, const double *xin, double *xout, doub
     }
 }
 
+/* Exported for the tests: xout += alpha * L2'L2 xin on layer l. */
+void or_l2(const or_t *o, int l, const double *xin, double *xout, double alpha) {
+    double *tmp = (double *)malloc(sizeof(double) * o->xnx[l] * o->xny[l]);
+    l2_apply(o, l, xin, xout, alpha, tmp);
+    free(tmp);
+}
+
 /* iprop for layer l: xout_l += sum_w HX_wl' gg_w (+ alpha L2'L2 xin_l) */
 static void iprop_layer(const or_t *o, int l, double *const *gg, const double *xin, double *xout,
                         double alpha) {
