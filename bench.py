@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--modes", default="plain,graph,batched",
+                    help="comma list of modes to time (profiling runs use one)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the solve runs on one MI355X; "
@@ -132,10 +134,12 @@ def main():
         return dict(solves_per_s=nb * len(lat) / lat.sum(), p50_ms=1e3 * float(np.percentile(lat, 50)),
                     p99_ms=1e3 * float(np.percentile(lat, 99)), ms_per_step=1e3 * float(lat.mean()))
 
+    modes = set(args.modes.split(","))
     plain = tomo.GpuTomo(r, nbatch=1, maxit=maxit)
     batched = tomo.GpuTomo(r, nbatch=B, maxit=maxit)
-    res_plain = run_mode(plain, 1, 0, args.steps, args.warmup)
-    res_graph = run_mode(plain, 1, 1, args.steps, args.warmup)
+    empty = dict(solves_per_s=float("nan"), p50_ms=float("nan"), p99_ms=float("nan"), ms_per_step=float("nan"))
+    res_plain = run_mode(plain, 1, 0, args.steps, args.warmup) if "plain" in modes else empty
+    res_graph = run_mode(plain, 1, 1, args.steps, args.warmup) if "graph" in modes else empty
     nsteps_b = max(10, args.steps // 4)
     # timed region of the headline mode: barrier + synchronize on both sides
     torch.cuda.synchronize()
