@@ -7,6 +7,8 @@ files and converted to radian here (as setup_parms does).
   small8    8 m, 4 NGS at (+-15",+-15"), 2 layers (0, 8 km), tomo.pos=2, os=2
   large30   30 m, 6 LGS at 90 km (on-axis + 35" pentagon, config/maos/
             shwfs_lgs.conf:51-52), 6 layers 0-16 km, tomo.pos=1, os=1
+  lgs8_flat small8 with 4 LGS at 90 km and tomo.cone=0: rays above ground
+            sample the layers off-grid (generic ray path)
   batch     D in {8,10,20,30}, 1-6 LGS, 2-6 layers (large30 truncated)
 """
 import ctypes as C
@@ -54,6 +56,17 @@ def large30(**kw):
     return c
 
 
+def lgs8_flat(**kw):
+    """4 LGS at 90 km on the 8 m aperture without cone coordinates
+    (tomo.cone=0): layer grids keep atmr.dx, so the LGS rays of every layer
+    above ground sample them at a ratio 1 - h/hs (generic ray path)."""
+    c = small8(name="lgs8_flat", tomo_cone=0,
+               powfs=[dict(dsa=0.5, dx=1 / 64, saat=0.3, nearecon=30.0, hs=90e3, trs=1, lo=0,
+                           skip=0, dtrat=1)])
+    c.update(kw)
+    return c
+
+
 def batch_member(D, nwfs, nlayer, **kw):
     """One system of the uneven batch: large30 truncated to nwfs LGS and
     nlayer layers on a D-m aperture (weights renormalised by the setup)."""
@@ -76,7 +89,7 @@ def batch64(seed=3):
     return out
 
 
-CONFIGS = {"small8": small8, "large30": large30}
+CONFIGS = {"small8": small8, "large30": large30, "lgs8_flat": lgs8_flat}
 
 
 class Parms:

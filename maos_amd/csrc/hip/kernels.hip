@@ -368,6 +368,177 @@ __global__ __launch_bounds__(256) void k_layer(Geom G, const RT *q, const float 
 }
 
 // ---------------------------------------------------------------------------
+// Matched-sampling fast path (ploc spacing * cone scale == layer spacing for
+// every WFS and layer: LGS in cone coordinates, NGS on an NGS-only system).
+// The ray of (w, l) is then a constant integer offset plus a constant
+// bilinear fraction, so HX is a 2x2 stencil on a shifted window and HX' its
+// mirror.  Layer windows (k_wfs_m) and WFS-field windows (k_layer_m) are
+// staged in LDS with coalesced loads.
+// ---------------------------------------------------------------------------
+template <int POS, typename RT>
+__global__ __launch_bounds__(256) void k_wfs_m(Geom G, const float *x, RT *q) {
+    constexpr int H = TILE + 2 * POS;
+    constexpr int H1 = H + 1;
+    constexpr int S = TILE + POS;
+    constexpr int NP = POS + 1;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    float *xs = reinterpret_cast<float *>(lds_raw);  // [npsr][H1*H1]
+    __shared__ RT opd[H * H];
+    __shared__ RT wgx[S * S], wgy[S * S];
+    __shared__ int sidx[S * S];
+    const int w = blockIdx.y, s = blockIdx.z;
+    const int pnx = G.pnx, pny = G.pny;
+    const int ntx = (pnx + TILE - 1) / TILE;
+    const int tx0 = (blockIdx.x % ntx) * TILE, ty0 = (blockIdx.x / ntx) * TILE;
+    const float *xsys = x + (long)s * G.nxtot;
+    for (int l = 0; l < G.npsr; l++) {
+        const MRay m = G.mray[w * MAXL + l];
+        const int nxl = G.xnx[l], nyl = G.xny[l];
+        const float *xl = xsys + G.xoff[l];
+        const int gx0 = tx0 - POS + m.cx, gy0 = ty0 - POS + m.cy;
+        for (int i = threadIdx.x; i < H1 * H1; i += blockDim.x) {
+            int gx = gx0 + i % H1, gy = gy0 + i / H1;
+            if (gx == nxl && m.wx) gx = 0;  // edge ray: second node wraps
+            if (gy == nyl && m.wy) gy = 0;
+            xs[l * H1 * H1 + i] = (gx >= 0 && gx < nxl && gy >= 0 && gy < nyl) ? xl[gy * nxl + gx] : 0.f;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < H * H; i += blockDim.x) {
+        const int a = i % H, b = i / H;
+        const int px = tx0 - POS + a, py = ty0 - POS + b;
+        RT v = 0;
+        if (px >= 0 && px < pnx && py >= 0 && py < pny) {
+            for (int l = 0; l < G.npsr; l++) {
+                const MRay m = G.mray[w * MAXL + l];
+                if (px >= m.ax0 && px < m.ax1 && py >= m.ay0 && py < m.ay1) {
+                    const float *t = xs + l * H1 * H1 + b * H1 + a;
+                    const RT fx = m.fx, fy = m.fy;
+                    v += (RT(1) - fy) * ((RT(1) - fx) * (RT)t[0] + fx * (RT)t[1]) +
+                         fy * ((RT(1) - fx) * (RT)t[H1] + fx * (RT)t[H1 + 1]);
+                }
+            }
+        }
+        opd[i] = v;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < S * S; i += blockDim.x) {
+        const int a = i % S, b = i / S;
+        const int cx = tx0 - POS + a, cy = ty0 - POS + b;
+        int sa = -1;
+        if (cx >= 0 && cx < pnx && cy >= 0 && cy < pny) sa = G.saidx[(long)w * G.nploc + cy * pnx + cx];
+        sidx[i] = sa;
+        RT ox = 0, oy = 0;
+        if (sa >= 0) {
+            const long so = G.saoff[w] + sa;
+            RT gx = 0, gy = 0;
+            const float *cxp = G.gpx + so * NP * NP, *cyp = G.gpy + so * NP * NP;
+#pragma unroll
+            for (int zy = 0; zy < NP; zy++)
+#pragma unroll
+                for (int zx = 0; zx < NP; zx++) {
+                    const RT o = opd[(b + zy) * H + a + zx];
+                    gx += (RT)cxp[zy * NP + zx] * o;
+                    gy += (RT)cyp[zy * NP + zx] * o;
+                }
+            const float4 ne = G.neai[so];
+            ox = (RT)ne.x * gx + (RT)ne.z * gy;
+            oy = (RT)ne.z * gx + (RT)ne.y * gy;
+        }
+        wgx[i] = ox;
+        wgy[i] = oy;
+    }
+    __syncthreads();
+    const int ox = threadIdx.x % TILE, oy = threadIdx.x / TILE;
+    const int px = tx0 + ox, py = ty0 + oy;
+    if (threadIdx.x < TILE * TILE && px < pnx && py < pny) {
+        RT acc = 0;
+#pragma unroll
+        for (int zy = 0; zy < NP; zy++)
+#pragma unroll
+            for (int zx = 0; zx < NP; zx++) {
+                const int si = (oy + POS - zy) * S + (ox + POS - zx);
+                const int sa = sidx[si];
+                if (sa >= 0) {
+                    const long so = G.saoff[w] + sa;
+                    acc += (RT)G.gpx[so * NP * NP + zy * NP + zx] * wgx[si] +
+                           (RT)G.gpy[so * NP * NP + zy * NP + zx] * wgy[si];
+                }
+            }
+        q[((long)s * G.nwfs + w) * G.nploc + py * pnx + px] = acc;
+    }
+}
+
+template <bool LHS, typename RT>
+__global__ __launch_bounds__(256) void k_layer_m(Geom G, const RT *q, const float *p, RT *out,
+                                                 double *part, int partstride) {
+    constexpr int Q = TILE + 1;      // WFS-field window edge
+    constexpr int PH = TILE + 4;     // p window edge (L2'L2 reach is 2)
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    RT *qs = reinterpret_cast<RT *>(lds_raw);                       // [nwfs][Q*Q]
+    float *ps = reinterpret_cast<float *>(qs + G.nwfs * Q * Q);     // [PH*PH]
+    __shared__ double sh[8];
+    const int l = blockIdx.y, s = blockIdx.z;
+    const int nxl = G.xnx[l], nyl = G.xny[l];
+    const int ntx = (nxl + TILE - 1) / TILE, nty = (nyl + TILE - 1) / TILE;
+    const bool active = (int)blockIdx.x < ntx * nty;
+    const int jx0 = (blockIdx.x % ntx) * TILE, jy0 = (blockIdx.x / ntx) * TILE;
+    if (active) {
+        for (int w = 0; w < G.nwfs; w++) {
+            const MRay m = G.mray[w * MAXL + l];
+            const RT *qw = q + ((long)s * G.nwfs + w) * G.nploc;
+            const int ix0 = jx0 - m.cx - 1, iy0 = jy0 - m.cy - 1;
+            for (int i = threadIdx.x; i < Q * Q; i += blockDim.x) {
+                int ix = ix0 + i % Q, iy = iy0 + i / Q;
+                // node 0 is the wrapped second node of the edge ray a1 - 1;
+                // slot i = -c - 1 (node 0's second-node ray) is otherwise empty
+                if (m.wx && ix == -m.cx - 1) ix = m.ax1 - 1;
+                if (m.wy && iy == -m.cy - 1) iy = m.ay1 - 1;
+                RT v = 0;
+                if (ix >= m.ax0 && ix < m.ax1 && iy >= m.ay0 && iy < m.ay1) v = qw[iy * G.pnx + ix];
+                qs[w * Q * Q + i] = v;
+            }
+        }
+        if (LHS) {
+            const float *pl = p + (long)s * G.nxtot + G.xoff[l];
+            for (int i = threadIdx.x; i < PH * PH; i += blockDim.x) {
+                // periodic; the last tile's window may run past a small layer
+                const int xx = (jx0 - 2 + i % PH + nxl) % nxl, yy = (jy0 - 2 + i / PH + nyl) % nyl;
+                ps[i] = pl[yy * nxl + xx];
+            }
+        }
+    }
+    __syncthreads();
+    double dot = 0;
+    const int tx = threadIdx.x % TILE, ty = threadIdx.x / TILE;
+    const int jx = jx0 + tx, jy = jy0 + ty;
+    if (active && jx < nxl && jy < nyl) {
+        RT acc = 0;
+        for (int w = 0; w < G.nwfs; w++) {
+            const MRay m = G.mray[w * MAXL + l];
+            const RT fx = m.fx, fy = m.fy;
+            const RT *t = qs + w * Q * Q + ty * Q + tx;
+            acc += fy * ((RT(1) - fx) * t[1] + fx * t[0]) + (RT(1) - fy) * ((RT(1) - fx) * t[Q + 1] + fx * t[Q]);
+        }
+        const long o = (long)s * G.nxtot + G.xoff[l];
+        if (LHS) {
+            auto P = [&](int dx, int dy) { return (RT)ps[(ty + 2 + dy) * PH + tx + 2 + dx]; };
+            const RT c = P(0, 0);
+            const RT s4 = P(-1, 0) + P(1, 0) + P(0, -1) + P(0, 1);
+            const RT sd = P(-1, -1) + P(1, -1) + P(-1, 1) + P(1, 1);
+            const RT s2 = P(-2, 0) + P(2, 0) + P(0, -2) + P(0, 2);
+            acc += (RT)G.l2c[l] * (RT(20) * c - RT(8) * s4 + RT(2) * sd + s2);
+            dot = (double)c * (double)acc;
+        } else {
+            dot = (double)acc * (double)acc;
+        }
+        out[o + jy * nxl + jx] = acc;
+    }
+    dot = block_sum(dot, sh);
+    if (threadIdx.x == 0 && part) part[(long)s * partstride + blockIdx.y * gridDim.x + blockIdx.x] = dot;
+}
+
+// ---------------------------------------------------------------------------
 // Row kernel: PCG vector update fused with the forward real-to-complex row
 // FFT of r.  Two real rows are packed into one complex FFT.
 // ---------------------------------------------------------------------------
@@ -633,8 +804,22 @@ static void launch_wfs_t(const Geom &G, int nsys, bool rhs, const float *x, cons
         else hipLaunchKernelGGL((k_wfs<2, false, RT>), grid, dim3(256), 0, st, G, x, grad, tt, q);
     }
 }
+template <typename RT>
+static void launch_wfs_m(const Geom &G, int nsys, const float *x, RT *q, hipStream_t st) {
+    const int nt = ((G.pnx + TILE - 1) / TILE) * ((G.pny + TILE - 1) / TILE);
+    dim3 grid(nt, G.nwfs, nsys);
+    const int H1 = TILE + 2 * G.pos + 1;
+    const size_t lds = sizeof(float) * G.npsr * H1 * H1;
+    if (G.pos == 1) hipLaunchKernelGGL((k_wfs_m<1, RT>), grid, dim3(256), lds, st, G, x, q);
+    else hipLaunchKernelGGL((k_wfs_m<2, RT>), grid, dim3(256), lds, st, G, x, q);
+}
 void launch_wfs(const Geom &G, int nsys, bool rhs, int rt64, const float *x, const float *grad,
                 const float *tt, void *q, hipStream_t st) {
+    if (!rhs && G.matched) {
+        if (rt64) launch_wfs_m<double>(G, nsys, x, (double *)q, st);
+        else launch_wfs_m<float>(G, nsys, x, (float *)q, st);
+        return;
+    }
     if (rt64) launch_wfs_t<double>(G, nsys, rhs, x, grad, tt, (double *)q, st);
     else launch_wfs_t<float>(G, nsys, rhs, x, grad, tt, (float *)q, st);
 }
@@ -649,6 +834,23 @@ int layer_tiles(const Geom &G) {
 void launch_layer(const Geom &G, int nsys, bool lhs, int rt64, const void *q, const float *p, void *out,
                   double *part, int partstride, hipStream_t st) {
     dim3 grid(layer_tiles(G), G.npsr, nsys);
+    if (G.matched) {
+        const size_t Q2 = (TILE + 1) * (TILE + 1), P2 = (TILE + 4) * (TILE + 4);
+        if (rt64) {
+            const size_t lds = sizeof(double) * G.nwfs * Q2 + sizeof(float) * P2;
+            double *o = (double *)out;
+            const double *qq = (const double *)q;
+            if (lhs) hipLaunchKernelGGL((k_layer_m<true, double>), grid, dim3(256), lds, st, G, qq, p, o, part, partstride);
+            else hipLaunchKernelGGL((k_layer_m<false, double>), grid, dim3(256), lds, st, G, qq, p, o, part, partstride);
+        } else {
+            const size_t lds = sizeof(float) * G.nwfs * Q2 + sizeof(float) * P2;
+            float *o = (float *)out;
+            const float *qq = (const float *)q;
+            if (lhs) hipLaunchKernelGGL((k_layer_m<true, float>), grid, dim3(256), lds, st, G, qq, p, o, part, partstride);
+            else hipLaunchKernelGGL((k_layer_m<false, float>), grid, dim3(256), lds, st, G, qq, p, o, part, partstride);
+        }
+        return;
+    }
     if (rt64) {
         double *o = (double *)out;
         const double *qq = (const double *)q;

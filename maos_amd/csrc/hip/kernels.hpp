@@ -26,6 +26,15 @@ constexpr int MAXST = 16;     // FFT stages
 constexpr int TILE = 16;      // output tile edge of the WFS / layer passes
 constexpr int TAPS = 4;       // max ploc samples landing on one layer node per axis
 
+// Matched-sampling ray of one (WFS, layer) pair: ploc index i maps to layer
+// nodes i + c and i + c + 1 with a constant fraction f, for i in [a0, a1)
+// (per axis).  wrap: the last ray's second node is node 0 (edge case).
+struct MRay {
+    int cx, cy, ax0, ax1, ay0, ay1;
+    int wx, wy;
+    float fx, fy;
+};
+
 // Geometry shared by all systems of a plan (device copy).
 struct Geom {
     int npsr, nwfs, pos, np;      // np = pos+1
@@ -43,6 +52,8 @@ struct Geom {
     const float *hx_fx, *hx_fy;
     const int4 *hxt_x, *hxt_y;    // per (w,l) per layer index: up to 4 ploc indices (-1 pad)
     long hxt_xoff[MAXL + 1];      // offset of layer l in a (w,*) transposed x table
+    int matched;                  // every (w,l) ray is matched sampling (fast path)
+    const MRay *mray;             // [w*MAXL + l]
     // subapertures
     const int *saidx;             // per WFS per ploc point: subaperture with that lower-left corner, -1
     const float *gpx, *gpy;       // per WFS per subap np*np coefficients

@@ -92,6 +92,7 @@ struct gpu_tomo {
     DevBuf<int2> hx_nx, hx_ny;
     DevBuf<float> hx_fx, hx_fy;
     DevBuf<int4> hxt_x, hxt_y;
+    DevBuf<MRay> mray;
     DevBuf<int> saidx;
     DevBuf<float> gpx, gpy, ptt;
     DevBuf<float4> neai;
@@ -388,6 +389,56 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
             }
         }
     }
+    // matched-sampling detection: on each axis the valid ploc indices form
+    // one interval [a0, a1) whose rays land on node i + c with the next node
+    // i + c + 1 and one fraction (float copies of FP64 values that differ by
+    // rounding only).  The last ray may instead be the reference's edge case
+    // (node n-1, next node wrapped to 0): flagged as wrap.
+    std::vector<MRay> hm((size_t)nw * MAXL);
+    int matched = 1;
+    auto simple_axis = [&](const std::vector<int2> &nn, const std::vector<float> &ff, size_t o, long n,
+                           int nl, int &c, float &f, int &a0, int &a1, int &wrap) {
+        a0 = -1;
+        a1 = -1;
+        wrap = 0;
+        for (long i = 0; i < n; i++) {
+            const int2 v = nn[o + i];
+            if (v.x < 0) {
+                if (a0 >= 0 && a1 < 0) a1 = (int)i;
+                continue;
+            }
+            if (a1 >= 0 || wrap) return false;  // second interval, or a ray after the edge
+            if (a0 < 0) {
+                a0 = (int)i;
+                c = v.x - (int)i;
+                f = ff[o + i];
+            }
+            if (v.x - (int)i != c || std::fabs(ff[o + i] - f) > 1e-6f) return false;
+            if (v.y != v.x + 1) {
+                if (v.x == nl - 1 && v.y == 0) wrap = 1;
+                else return false;
+            }
+        }
+        if (a0 < 0) {  // ray misses the layer entirely
+            a0 = a1 = 0;
+            c = 0;
+            f = 0.f;
+        } else if (a1 < 0) {
+            a1 = (int)n;
+        }
+        return true;
+    };
+    for (int k = 0; k < nw && matched; k++)
+        for (int l = 0; l < R->npsr && matched; l++) {
+            const size_t wl = (size_t)k * MAXL + l;
+            MRay &m = hm[wl];
+            if (!simple_axis(hnx, hfx, wl * pnx, pnx, R->xnx[l], m.cx, m.fx, m.ax0, m.ax1, m.wx) ||
+                !simple_axis(hny, hfy, wl * pny, pny, R->xny[l], m.cy, m.fy, m.ay0, m.ay1, m.wy))
+                matched = 0;
+        }
+    G.matched = matched;
+    g->mray.upload(hm);
+    G.mray = g->mray.p;
     g->hx_nx.upload(hnx);
     g->hx_ny.upload(hny);
     g->hx_fx.upload(hfx);

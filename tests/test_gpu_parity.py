@@ -8,7 +8,7 @@ solution to 1e-4 relative L2 (the accuracy gate of this path).
 import numpy as np
 import pytest
 
-from maos_amd import tomo, workload
+from maos_amd import configs, tomo, workload
 from oracle.oracle import Oracle
 
 pytestmark = pytest.mark.gpu
@@ -18,9 +18,13 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.fixture(scope="module", params=["small8", "large30"])
+@pytest.fixture(scope="module", params=["small8", "large30", "lgs8_flat"])
 def sys_(request):
-    cfg, r = request.getfixturevalue(request.param)
+    if request.param == "lgs8_flat":
+        cfg = configs.lgs8_flat()
+        r = tomo.Recon(cfg)
+    else:
+        cfg, r = request.getfixturevalue(request.param)
     print("case", cfg["name"], flush=True)
     return cfg, r, tomo.GpuTomo(r, maxit=cfg["maxit"]), Oracle(r, nthread=8)
 
