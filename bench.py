@@ -30,32 +30,31 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "tomo solves/s + p99 latency (30m, 6 LGS, 6 layers, 30 PCG it) vs plain GPU"
 
 
-def algorithmic_bytes(r, G):
-    """Unique bytes each kernel class must move per launch for one system
-    (FP32 x/p/z, FP64 r/Ap/b/q, complex128 forward spectrum, complex64
-    inverse spectrum).  Sizes from the plan geometry."""
+def algorithmic_bytes(r):
+    """Unique HBM bytes each kernel must move in one launch of an UPDATE
+    iteration: (per system, shared by all systems of the launch).  Vectors:
+    x/p FP32, r/Ap/q FP64, forward spectrum complex128, inverse spectrum
+    complex64; shared: operator geometry and FDPCG blocks, read once."""
     rec = r.rec
     nx = r.nx
     nw = sum(1 for w in range(rec.nwfs) if not rec.skip[w])
     nploc = rec.pnx * rec.pny
-    nsa = sum(rec.nsa[w] for w in range(rec.nwfs))
+    nsa = sum(rec.nsa[w] for w in range(rec.nwfs) if not rec.skip[w])
     np2 = (rec.pos + 1) ** 2
     nxl, nyl = rec.xnx[0], rec.xny[0]
     nspec = rec.npsr * (nxl // 2 + 1) * nyl
     bs, nbh = rec.fd_bs, rec.fd_nbhalf
     return {
-        # p (read once) + GP coefs + NEA + subap map, q written (FP64)
-        "k_wfs": 4 * nx + nsa * (2 * np2 * 4 + 16) + 4 * nw * nploc + 8 * nw * nploc,
-        # q (FP64) + p read, Ap (FP64) written
-        "k_layer": 8 * nw * nploc + 4 * nx + 8 * nx,
-        # x, p, r, Ap read; x, r, forward spectrum (complex128) written
-        "k_row": (4 + 4 + 8 + 8) * nx + (4 + 8) * nx + 16 * nspec,
-        # forward spectrum read, half blocks read, inverse spectrum written
-        "k_col": 16 * nspec + 8 * nbh * bs * bs + 8 * nspec,
-        # inverse spectrum + r read, z written
-        "k_irow": 8 * nspec + 8 * nx + 4 * nx,
-        # z, p read, p written
-        "k_p": 12 * nx,
+        # read p, write q; GP stencils (x, y), NEA and the subaperture map
+        "k_wfs": (4 * nx + 8 * nw * nploc, nsa * (2 * np2 * 4 + 16) + 4 * nw * nploc),
+        # read q and p (L2), write Ap
+        "k_layer": (8 * nw * nploc + 4 * nx + 8 * nx, 0),
+        # read r, Ap; write r and the forward row spectrum
+        "k_row": (8 * nx + 8 * nx + 8 * nx + 16 * nspec, 0),
+        # read the row spectrum, write the inverse-column spectrum; blocks
+        "k_col": (16 * nspec + 8 * nspec, 8 * nbh * bs * bs),
+        # read the spectrum, x and p; write x and p
+        "k_irow": (8 * nspec + 8 * nx + 8 * nx, 0),
     }
 
 
@@ -157,17 +156,18 @@ def main():
     prof = batched.profile_read()
     batched.profile(False)
     torch.cuda.synchronize()
-    ab = algorithmic_bytes(r, batched)
+    ab = algorithmic_bytes(r)
     kernels = {}
     for k, (ms, calls) in prof.items():
-        by = ab.get(k, 0) * B
+        per, shared = ab.get(k, (0, 0))
+        by = per * B + shared
         kernels[k] = dict(avg_us=1e3 * ms, calls=calls, total_ms=ms * calls,
                           alg_MB=by / 1e6, GBs=(by / (ms * 1e-3) / 1e9) if ms > 0 else 0.0)
     dom = max((k for k in kernels if k in ab), key=lambda k: kernels[k]["total_ms"])
     kd = kernels[dom]
     roofline = dict(bound="hbm", achieved=round(kd["GBs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(kd["GBs"] / HBM_PEAK_GBS, 4), traffic=None, kernel=dom,
-                    avg_us=round(kd["avg_us"], 2), alg_bytes_per_launch=int(ab[dom] * B))
+                    avg_us=round(kd["avg_us"], 2), alg_bytes_per_launch=int(ab[dom][0] * B + ab[dom][1]))
 
     cpu = None
     err = None

@@ -220,6 +220,14 @@ void gpu_tomo_free(gpu_tomo_t *g);
 int  gpu_tomo_solve(gpu_tomo_t *g, int isys, const double *grad,
                     double *x, double *cgres);
 
+/* The same for nsys <= nbatch systems in one batched solve: grad is nsys
+ * stacked gradient vectors (ng each), x nsys stacked layer vectors (nx
+ * each, in: ignored, the warm start is the resident solution of each slot),
+ * cgres nsys residuals.  Returns MT_NOTCONV if any residual exceeds 1.
+ * Batch form of gpu_tomo (cuda/recon.cu:564) for independent loops. */
+int  gpu_tomo_solve_batch(gpu_tomo_t *g, int nsys, const double *grad,
+                          double *x, double *cgres);
+
 /* Batched solve with device buffers already resident in HBM (FP32):
  * d_grad = nbatch stacked gradient vectors, d_x = nbatch stacked layer
  * vectors (in: warm start when warm=1; out: solution).  Enqueued on
@@ -243,8 +251,8 @@ int  gpu_tomo_P(gpu_tomo_t *g, const double *xin, double *xout);
 /* Precision of the residual recurrence: mode 1 (default) keeps r, A*p and
  * the right-hand side in FP64 (operators, FFTs, x, p, z stay FP32); mode 0
  * keeps them in FP32.  Dot products and alpha/beta are FP64 in both.  A
- * 30-iteration FDPCG solve of the 30 m system drifts ~5e-4 from the FP64
- * solve with mode 0 and stays ~1e-5 with mode 1. */
+ * 30-iteration FDPCG solve of the 30 m system drifts ~2e-3 (relative L2)
+ * from the FP64 solve with mode 0 and stays ~1e-5 with mode 1. */
 int  gpu_tomo_precision(gpu_tomo_t *g, int mode);
 
 /* Sizes of one system's vectors. */

@@ -101,6 +101,7 @@ def lib():
     L.gpu_tomo_free.argtypes = [vp]
     L.gpu_tomo_free.restype = None
     L.gpu_tomo_solve.argtypes = [vp, C.c_int, dp, dp, dp]
+    L.gpu_tomo_solve_batch.argtypes = [vp, C.c_int, dp, dp, dp]
     L.gpu_tomo_solve_dev.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp]
     L.gpu_tomo_residuals.argtypes = [vp, dp, C.c_int]
     L.gpu_tomo_L.argtypes = [vp, dp, dp, C.c_double, C.c_double]
@@ -120,7 +121,7 @@ EXPORTS = [
     "mt_setup_free", "mt_setup_fdpcg_mhat", "mt_setup_sizes", "mt_fdpcg_prepare",
     "mt_fdpcg_free", "gpu_tomo_setup",
     "gpu_tomo_update_cn2", "gpu_tomo_reset", "gpu_tomo_free", "gpu_tomo_solve",
-    "gpu_tomo_solve_dev", "gpu_tomo_residuals", "gpu_tomo_L", "gpu_tomo_R", "gpu_tomo_P",
+    "gpu_tomo_solve_batch", "gpu_tomo_solve_dev", "gpu_tomo_residuals", "gpu_tomo_L", "gpu_tomo_R", "gpu_tomo_P",
     "gpu_tomo_sizes", "gpu_tomo_profile", "gpu_tomo_profile_read", "gpu_tomo_precision",
 ]
 

@@ -572,6 +572,9 @@ __global__ __launch_bounds__(256) void k_row(Geom G, Scal SC, int iter, float *x
         }
         alpha = a;
     }
+    // x += alpha p of an UPDATE iteration is applied by the inverse row
+    // kernel, which reads p anyway for p = z + beta p.
+    if (MODE == ROW_UPDATE && blockIdx.x == 0 && threadIdx.x == 0) SC.alpha[s] = alpha;
     const long base = (long)s * G.nxtot + G.xoff[l] + (long)row0 * n;
     FR *bufA = reinterpret_cast<FR *>(lds);
     double dot = 0;
@@ -587,7 +590,7 @@ __global__ __launch_bounds__(256) void k_row(Geom G, Scal SC, int iter, float *x
             rv = b[idx] - Ap[idx];
             r[idx] = rv;
         } else {
-            x[idx] += (float)alpha * p[idx];
+            if (MODE == ROW_LAST) x[idx] += (float)alpha * p[idx];
             rv = r[idx] - (RT)alpha * Ap[idx];
             r[idx] = rv;
         }
@@ -623,8 +626,9 @@ __global__ __launch_bounds__(256) void k_row(Geom G, Scal SC, int iter, float *x
 // inverse column FFTs.  Everything in LDS.
 // ---------------------------------------------------------------------------
 template <typename FT>
-__global__ __launch_bounds__(256) void k_col(Geom G, const FT *specf, float2 *speci) {
+__global__ __launch_bounds__(256) void k_col(Geom G, Scal SC, const FT *specf, float2 *speci) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ double sh[8];
     typedef typename CT<FT>::R FR;
     const int g = blockIdx.x, s = blockIdx.z;
     const int ny = G.fny;
@@ -666,8 +670,13 @@ __global__ __launch_bounds__(256) void k_col(Geom G, const FT *specf, float2 *sp
         }
         __syncthreads();
     }
+    // Block multiply, and the FP64 partial of r.z taken on the spectrum:
+    // sum_xy r z = sc * sum_k conj(R_k) Z_k over the full spectrum (Parseval;
+    // sc = fscale when scaled, and R here already carries that fscale), i.e.
+    // the half-spectrum sum with weight 2 off the kx = 0 and kx = n/2 columns.
     const int bs = G.bs;
     const int b0 = G.grp_blk[g], b1 = G.grp_blk[g + 1];
+    double dot = 0;
     for (int t = threadIdx.x; t < (b1 - b0) * bs; t += blockDim.x) {
         const int bi = G.blkid[b0 + t / bs];
         const int m = t % bs;
@@ -683,9 +692,16 @@ __global__ __launch_bounds__(256) void k_col(Geom G, const FT *specf, float2 *sp
         }
         const int pm = bp[m];
         if ((pm >> 30) & 1) y.y = -y.y;
-        if (!((pm >> 29) & 1)) C[pm & 0x1fffffff] = make_float2((float)y.x, (float)y.y);
+        if (!((pm >> 29) & 1)) {
+            const int pos = pm & 0x1fffffff;
+            C[pos] = make_float2((float)y.x, (float)y.y);
+            const int kx = G.colkx[c0 + pos / ny / G.npsr];
+            const double wgt = (kx == 0 || 2 * kx == G.fnx) ? 1.0 : 2.0;
+            dot += wgt * ((double)res[pos].x * (double)y.x + (double)res[pos].y * (double)y.y);
+        }
     }
-    __syncthreads();
+    dot = block_sum(dot, sh);  // contains the barrier before the inverse FFT
+    if (threadIdx.x == 0) SC.part_rz[(long)s * SC.maxpart + g] = dot;
     float2 *res2 = fft_run<1, float2>(C, D, ny, nf, G.nsty, G.rady, G.twy);
     for (int i = threadIdx.x; i < tot; i += blockDim.x) {
         const int f = i / ny, ky = i - f * ny;
@@ -696,16 +712,35 @@ __global__ __launch_bounds__(256) void k_col(Geom G, const FT *specf, float2 *sp
 }
 
 // ---------------------------------------------------------------------------
-// Inverse row kernel: complex-to-real row FFT -> z, and the FP64 partial of
-// r.z.  The imaginary parts of the DC and Nyquist bins are ignored (C2R).
+// Inverse row kernel: complex-to-real row FFT -> z (the imaginary parts of
+// the DC and Nyquist bins are ignored, as C2R does), then by mode
+//   IROW_Z       out = z                                  (P alone)
+//   IROW_FIRST   p = z; rz[slot] = r.z                    (start / restart)
+//   IROW_UPDATE  beta = r.z / rz[slot]; rz[slot+1] = r.z;
+//                x += alpha p; p = z + beta p             (lib/pcg.c:703,714)
+// r.z is reduced from the column kernel's spectral partials (fixed order,
+// the same value in every workgroup).
 // ---------------------------------------------------------------------------
-template <typename RT>
-__global__ __launch_bounds__(256) void k_irow(Geom G, Scal SC, const float2 *spec, const RT *r,
-                                              float *z) {
+enum { IROW_Z = 0, IROW_FIRST = 1, IROW_UPDATE = 2 };
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_irow(Geom G, Scal SC, int slot, const float2 *spec, float *x,
+                                              float *out) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     float2 *lds = reinterpret_cast<float2 *>(lds_raw);
     __shared__ double sh[8];
     const int s = blockIdx.z;
+    double beta = 0, alpha = 0;
+    if (MODE != IROW_Z) {
+        const double rz = reduce_partials(SC.part_rz + (long)s * SC.maxpart, SC.npart_col, sh);
+        double *hist = SC.rz + (long)s * (SC.maxit + 2);
+        if (MODE == IROW_UPDATE) {
+            beta = rz / hist[slot];
+            if (!isfinite(beta)) beta = 0;
+            alpha = SC.alpha[s];
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) hist[MODE == IROW_FIRST ? slot : slot + 1] = rz;
+    }
     const int2 rb = G.rowblk[blockIdx.x];
     const int l = rb.x, row0 = rb.y;
     const int n = G.fnx, nyl = G.xny[l];
@@ -733,39 +768,21 @@ __global__ __launch_bounds__(256) void k_irow(Geom G, Scal SC, const float2 *spe
     __syncthreads();
     float2 *res = fft_run<1, float2>(A, B, n, nf, G.nstx, G.radx, G.twx);
     const float sc = G.scale ? G.fscale : 1.f;
+    const float fa = (float)alpha, fb = (float)beta;
     const long base = (long)s * G.nxtot + G.xoff[l] + (long)row0 * n;
-    double dot = 0;
     for (int i = threadIdx.x; i < nrows * n; i += blockDim.x) {
         const int row = i / n, col = i - row * n;
         const float2 v = res[(row >> 1) * n + col];
         const float zv = ((row & 1) ? v.y : v.x) * sc;
         const long idx = base + (long)row * n + col;
-        z[idx] = zv;
-        dot += (double)r[idx] * (double)zv;
+        if (MODE == IROW_UPDATE) {
+            const float pv = out[idx];
+            x[idx] += fa * pv;
+            out[idx] = zv + fb * pv;
+        } else {
+            out[idx] = zv;
+        }
     }
-    dot = block_sum(dot, sh);
-    if (threadIdx.x == 0 && SC.part_rz) SC.part_rz[(long)s * SC.maxpart + blockIdx.x] = dot;
-}
-
-// ---------------------------------------------------------------------------
-// p = z + beta p, beta = (r.z)_new / (r.z)_old; records (r.z)_new.
-// ---------------------------------------------------------------------------
-template <bool FIRST>
-__global__ __launch_bounds__(256) void k_p(Geom G, Scal SC, int iter, const float *z, float *p) {
-    __shared__ double sh[8];
-    const int s = blockIdx.z;
-    const double rz = reduce_partials(SC.part_rz + (long)s * SC.maxpart, SC.npart_row, sh);
-    double beta = 0;
-    double *hist = SC.rz + (long)s * (SC.maxit + 2);
-    if (!FIRST) {
-        beta = rz / hist[iter];
-        if (!isfinite(beta)) beta = 0;
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) hist[FIRST ? 0 : iter + 1] = rz;
-    const float fb = (float)beta;
-    const long o = (long)s * G.nxtot;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < G.nxtot; i += (long)gridDim.x * blockDim.x)
-        p[o + i] = FIRST ? z[o + i] : z[o + i] + fb * p[o + i];
 }
 
 // Residual sqrt(r.r / b.b) per system (NaN when the solve stopped early).
@@ -896,28 +913,26 @@ void launch_row(const Geom &G, const Scal &SC, int nsys, int mode, int rt64, int
 size_t col_lds(const Geom &G, int f64) {
     return ((f64 ? sizeof(double2) : sizeof(float2)) * 2 + sizeof(float2) * 2) * (size_t)G.maxcol * G.npsr * G.fny;
 }
-void launch_col(const Geom &G, int nsys, int f64, const void *specf, float2 *speci, hipStream_t st) {
+void launch_col(const Geom &G, const Scal &SC, int nsys, int f64, const void *specf, float2 *speci,
+                hipStream_t st) {
     if (f64)
-        hipLaunchKernelGGL(k_col<double2>, dim3(G.ngroup, 1, nsys), dim3(256), col_lds(G, 1), st, G,
+        hipLaunchKernelGGL(k_col<double2>, dim3(G.ngroup, 1, nsys), dim3(256), col_lds(G, 1), st, G, SC,
                            (const double2 *)specf, speci);
     else
-        hipLaunchKernelGGL(k_col<float2>, dim3(G.ngroup, 1, nsys), dim3(256), col_lds(G, 0), st, G,
+        hipLaunchKernelGGL(k_col<float2>, dim3(G.ngroup, 1, nsys), dim3(256), col_lds(G, 0), st, G, SC,
                            (const float2 *)specf, speci);
 }
-void launch_irow(const Geom &G, const Scal &SC, int nsys, int rt64, const float2 *spec, const void *r, float *z,
-                 hipStream_t st) {
-    if (rt64)
-        hipLaunchKernelGGL(k_irow<double>, dim3(G.nrowblk, 1, nsys), dim3(256), row_lds(G, 0), st, G, SC, spec,
-                           (const double *)r, z);
-    else
-        hipLaunchKernelGGL(k_irow<float>, dim3(G.nrowblk, 1, nsys), dim3(256), row_lds(G, 0), st, G, SC, spec,
-                           (const float *)r, z);
-}
-void launch_p(const Geom &G, const Scal &SC, int nsys, bool first, int iter, const float *z, float *p,
-              hipStream_t st) {
-    const int nb = (int)((G.nxtot + 255) / 256 < 240 ? (G.nxtot + 255) / 256 : 240);
-    if (first) hipLaunchKernelGGL(k_p<true>, dim3(nb, 1, nsys), dim3(256), 0, st, G, SC, iter, z, p);
-    else hipLaunchKernelGGL(k_p<false>, dim3(nb, 1, nsys), dim3(256), 0, st, G, SC, iter, z, p);
+void launch_irow(const Geom &G, const Scal &SC, int nsys, int mode, int slot, const float2 *spec, float *x,
+                 float *out, hipStream_t st) {
+    const dim3 grid(G.nrowblk, 1, nsys);
+    const size_t lds = row_lds(G, 0);
+    switch (mode) {
+        case IROW_Z: hipLaunchKernelGGL(k_irow<IROW_Z>, grid, dim3(256), lds, st, G, SC, slot, spec, x, out); break;
+        case IROW_FIRST:
+            hipLaunchKernelGGL(k_irow<IROW_FIRST>, grid, dim3(256), lds, st, G, SC, slot, spec, x, out);
+            break;
+        default: hipLaunchKernelGGL(k_irow<IROW_UPDATE>, grid, dim3(256), lds, st, G, SC, slot, spec, x, out); break;
+    }
 }
 void launch_final(const Geom &G, const Scal &SC, int nsys, int npart_rr, const int *stop, hipStream_t st) {
     hipLaunchKernelGGL(k_final, dim3(1, 1, nsys), dim3(256), 0, st, G, SC, npart_rr, stop);

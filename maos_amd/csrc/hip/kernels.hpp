@@ -92,12 +92,14 @@ struct Scal {
     double *part_rz;    // [nsys][maxpart]
     double *part_rr;    // [nsys][maxpart]
     double *part_bb;    // [nsys][maxpart]
-    double *rz;         // [nsys][maxit+2]
+    double *rz;         // [nsys][maxit+2] r.z history: rz[k] is the numerator of alpha_k
+    double *alpha;      // [nsys] step of the current iteration (0 once stopped)
     double *res;        // [nsys]
     float *tt;          // [nsys][nwfs][2]
     int maxpart, maxit;
     int npart_layer;    // partial count written by the layer kernel
-    int npart_row;      // partial count written by the inverse row kernel
+    int npart_row;      // partial count written by the row kernel (r.r)
+    int npart_col;      // partial count written by the column kernel (r.z)
 };
 
 }  // namespace mtk

@@ -17,11 +17,11 @@ size_t row_lds(const Geom &G, int f64);
 void launch_row(const Geom &G, const Scal &SC, int nsys, int mode, int rt64, int iter, float *x, void *r,
                 const void *b, const float *p, const void *Ap, void *spec, int *stop, hipStream_t st);
 size_t col_lds(const Geom &G, int f64);
-void launch_col(const Geom &G, int nsys, int f64, const void *specf, float2 *speci, hipStream_t st);
-void launch_irow(const Geom &G, const Scal &SC, int nsys, int rt64, const float2 *spec, const void *r, float *z,
-                 hipStream_t st);
-void launch_p(const Geom &G, const Scal &SC, int nsys, bool first, int iter, const float *z, float *p,
-              hipStream_t st);
+void launch_col(const Geom &G, const Scal &SC, int nsys, int f64, const void *specf, float2 *speci,
+                hipStream_t st);
+enum { IROW_MODE_Z = 0, IROW_MODE_FIRST = 1, IROW_MODE_UPDATE = 2 };
+void launch_irow(const Geom &G, const Scal &SC, int nsys, int mode, int slot, const float2 *spec, float *x,
+                 float *out, hipStream_t st);
 void launch_final(const Geom &G, const Scal &SC, int nsys, int npart_rr, const int *stop, hipStream_t st);
 void launch_copy(const float *a, float *b, long n, hipStream_t st);
 void launch_zero_i(int *a, int n, hipStream_t st);

@@ -71,8 +71,14 @@ struct DevBuf {
 };
 
 // Kernel classes timed by the profiler.
-enum KClass { KC_TTR, KC_WFS, KC_LAYER, KC_ROW, KC_COL, KC_IROW, KC_P, KC_FINAL, KC_N };
-const char *kc_names[KC_N] = {"k_ttr", "k_wfs", "k_layer", "k_row", "k_col", "k_irow", "k_p", "k_final"};
+// Kernel classes timed by the profiler: the iteration kernels (k_wfs ..
+// k_irow, the UPDATE-iteration forms) and the once-per-solve forms apart.
+enum KClass {
+    KC_TTR, KC_WFS, KC_LAYER, KC_ROW, KC_COL, KC_IROW, KC_FINAL,
+    KC_WFS_RHS, KC_LAYER_RHS, KC_ROW_START, KC_ROW_LAST, KC_IROW_START, KC_N
+};
+const char *kc_names[KC_N] = {"k_ttr", "k_wfs", "k_layer", "k_row", "k_col", "k_irow", "k_final",
+                              "k_wfs_rhs", "k_layer_rhs", "k_row_start", "k_row_last", "k_irow_start"};
 
 }  // namespace
 
@@ -106,7 +112,7 @@ struct gpu_tomo {
     int rt64 = 1;             // 1: r, Ap, b in FP64 (default); 0: FP32
     DevBuf<double2> specf;  // forward spectrum of r (double2, or float2 in the first half when !rt64)
     DevBuf<float2> speci;   // block-multiplied spectrum, inverse transform input
-    DevBuf<double> part_pap, part_rz, part_rr, part_bb, rz, res;
+    DevBuf<double> part_pap, part_rz, part_rr, part_bb, rz, alpha, res;
     DevBuf<int> stop;
     // host staging
     std::vector<float> hstage;
@@ -661,12 +667,13 @@ void alloc_state(gpu_tomo *g) {
     g->tt.alloc(ns * G.nwfs * 2);
     g->specf.alloc(ns * std::max<long>(G.nspec, 1));
     g->speci.alloc(ns * std::max<long>(G.nspec, 1));
-    const int maxpart = std::max({layer_tiles(G) * G.npsr, G.nrowblk, 1});
+    const int maxpart = std::max({layer_tiles(G) * G.npsr, G.nrowblk, G.ngroup, 1});
     g->part_pap.alloc(ns * maxpart);
     g->part_rz.alloc(ns * maxpart);
     g->part_rr.alloc(ns * maxpart);
     g->part_bb.alloc(ns * maxpart);
     g->rz.alloc(ns * (g->maxit + 2));
+    g->alpha.alloc(ns);
     g->res.alloc(ns);
     g->stop.alloc(ns);
     HIPCHK(hipMemset(g->x.p, 0, sizeof(float) * ns * G.nxtot));
@@ -681,12 +688,14 @@ void alloc_state(gpu_tomo *g) {
     SC.part_rr = g->part_rr.p;
     SC.part_bb = g->part_bb.p;
     SC.rz = g->rz.p;
+    SC.alpha = g->alpha.p;
     SC.res = g->res.p;
     SC.tt = g->tt.p;
     SC.maxpart = maxpart;
     SC.maxit = g->maxit;
     SC.npart_layer = layer_tiles(G) * G.npsr;
     SC.npart_row = G.nrowblk;
+    SC.npart_col = G.ngroup;
 }
 
 // ---------------------------------------------------------------------------
@@ -725,16 +734,26 @@ void enqueue_R(gpu_tomo *g, int nsys, const float *grad, void *bout, Rec &rec) {
     bool anyttr = false;
     for (int k = 0; k < G.nwfs; k++) anyttr |= G.ttr[k] != 0;
     if (anyttr) rec(KC_TTR, [&] { launch_ttr(G, nsys, grad, g->tt.p, rec.st); });
-    rec(KC_WFS, [&] { launch_wfs(G, nsys, true, g->rt64, nullptr, grad, g->tt.p, g->q.p, rec.st); });
-    rec(KC_LAYER, [&] { launch_layer(G, nsys, false, g->rt64, g->q.p, nullptr, bout, g->SC.part_bb, g->SC.maxpart, rec.st); });
+    rec(KC_WFS_RHS, [&] { launch_wfs(G, nsys, true, g->rt64, nullptr, grad, g->tt.p, g->q.p, rec.st); });
+    rec(KC_LAYER_RHS, [&] { launch_layer(G, nsys, false, g->rt64, g->q.p, nullptr, bout, g->SC.part_bb, g->SC.maxpart, rec.st); });
 }
-// Preconditioner on the spectrum: r rows -> z rows, partial r.z.
-void enqueue_P_tail(gpu_tomo *g, int nsys, Rec &rec) {
+// Preconditioner on the row-transformed residual (specf): column pass with
+// the spectral r.z partials, then the inverse row pass in the given mode.
+void enqueue_P_tail(gpu_tomo *g, int nsys, int irow_mode, int slot, float *out, Rec &rec) {
     const Geom &G = g->G;
-    rec(KC_COL, [&] { launch_col(G, nsys, g->rt64, g->specf.p, g->speci.p, rec.st); });
-    rec(KC_IROW, [&] { launch_irow(G, g->SC, nsys, g->rt64, g->speci.p, g->r.p, g->z.p, rec.st); });
+    rec(KC_COL, [&] { launch_col(G, g->SC, nsys, g->rt64, g->specf.p, g->speci.p, rec.st); });
+    rec(irow_mode == IROW_MODE_UPDATE ? KC_IROW : KC_IROW_START,
+        [&] { launch_irow(G, g->SC, nsys, irow_mode, slot, g->speci.p, g->x.p, out, rec.st); });
 }
 
+// The solve as lib/pcg.c:28-113 runs it (restart every 100 iterations):
+//   start:  r = b - A x (r = b cold, x = 0); z = M r; p = z; rz[0] = r.z
+//   iter k: Ap; alpha = rz[k] / p.Ap; r -= alpha Ap;
+//           last: x += alpha p, |r| -> residual; else z = M r,
+//           beta = r.z / rz[k], x += alpha p, p = z + beta p
+// Kernels per iteration: wfs, layer (Ap, p.Ap partials), row (alpha, r,
+// forward row FFT), col (column FFTs, blocks, r.z partials), irow (inverse
+// row FFT, beta, x and p).
 void enqueue_solve(gpu_tomo *g, int nsys, int warm, Rec &rec) {
     const Geom &G = g->G;
     Scal &SC = g->SC;
@@ -742,30 +761,32 @@ void enqueue_solve(gpu_tomo *g, int nsys, int warm, Rec &rec) {
     if (!G.precond) fail(MT_EINVAL, "the GPU solve requires the FDPCG preconditioner (tomo.precond=1)");
     launch_zero_i(g->stop.p, nsys, rec.st);
     enqueue_R(g, nsys, g->g.p, g->b.p, rec);
-    if (warm) {
-        enqueue_L(g, nsys, g->x.p, g->Ap.p, nullptr, rec);
-        rec(KC_ROW, [&] {
-            launch_row(G, SC, nsys, ROW_MODE_INIT_WARM, g->rt64, 0, g->x.p, g->r.p, g->b.p, g->p.p, g->Ap.p, g->specf.p,
-                       g->stop.p, rec.st);
-        });
-    } else {
-        rec(KC_ROW, [&] {
-            launch_row(G, SC, nsys, ROW_MODE_INIT_COLD, g->rt64, 0, g->x.p, g->r.p, g->b.p, g->p.p, g->Ap.p, g->specf.p,
-                       g->stop.p, rec.st);
-        });
-    }
-    enqueue_P_tail(g, nsys, rec);
-    rec(KC_P, [&] { launch_p(G, SC, nsys, true, 0, g->z.p, g->p.p, rec.st); });
+    auto start = [&](int k, bool cold) {
+        if (cold) {
+            rec(KC_ROW_START, [&] {
+                launch_row(G, SC, nsys, ROW_MODE_INIT_COLD, g->rt64, k, g->x.p, g->r.p, g->b.p, g->p.p, g->Ap.p,
+                           g->specf.p, g->stop.p, rec.st);
+            });
+        } else {
+            enqueue_L(g, nsys, g->x.p, g->Ap.p, nullptr, rec);
+            rec(KC_ROW_START, [&] {
+                launch_row(G, SC, nsys, ROW_MODE_INIT_WARM, g->rt64, k, g->x.p, g->r.p, g->b.p, g->p.p, g->Ap.p,
+                           g->specf.p, g->stop.p, rec.st);
+            });
+        }
+        enqueue_P_tail(g, nsys, IROW_MODE_FIRST, k, g->p.p, rec);
+    };
+    start(0, !warm);
     for (int k = 0; k < maxit; k++) {
+        if (k > 0 && k % 100 == 0) start(k, false);
         enqueue_L(g, nsys, g->p.p, g->Ap.p, SC.part_pap, rec);
         const bool last = k + 1 == maxit;
-        rec(KC_ROW, [&] {
+        rec(last ? KC_ROW_LAST : KC_ROW, [&] {
             launch_row(G, SC, nsys, last ? ROW_MODE_LAST : ROW_MODE_UPDATE, g->rt64, k, g->x.p, g->r.p, g->b.p, g->p.p,
                        g->Ap.p, g->specf.p, g->stop.p, rec.st);
         });
         if (last) break;
-        enqueue_P_tail(g, nsys, rec);
-        rec(KC_P, [&] { launch_p(G, SC, nsys, false, k, g->z.p, g->p.p, rec.st); });
+        enqueue_P_tail(g, nsys, IROW_MODE_UPDATE, k, g->p.p, rec);
     }
     rec(KC_FINAL, [&] { launch_final(G, SC, nsys, G.nrowblk, g->stop.p, rec.st); });
 }
@@ -957,6 +978,25 @@ int gpu_tomo_solve(gpu_tomo_t *g, int isys, const double *grad, double *x, doubl
     return rc;
 }
 
+int gpu_tomo_solve_batch(gpu_tomo_t *g, int nsys, const double *grad, double *x, double *cgres) {
+    if (!g || !grad || !x || nsys <= 0 || nsys > g->nbatch) return MT_EINVAL;
+    std::vector<double> res(nsys, NAN);
+    int rc = guarded([&] {
+        HIPCHK(hipSetDevice(g->device));
+        for (int s = 0; s < nsys; s++) to_dev_grad(g, s, grad + (long)s * g->ngtot_all);
+        Rec rec{g, g->stream, false};
+        enqueue_solve(g, nsys, g->warm, rec);
+        for (int s = 0; s < nsys; s++) from_dev_x(g, x + (long)s * g->nxtot, g->x.p + (long)s * g->G.nxtot);
+        HIPCHK(hipMemcpy(res.data(), g->res.p, sizeof(double) * nsys, hipMemcpyDeviceToHost));
+    });
+    if (cgres)
+        for (int s = 0; s < nsys; s++) cgres[s] = res[s];
+    if (rc == MT_OK)
+        for (int s = 0; s < nsys; s++)
+            if (!(res[s] <= 1.0)) rc = MT_NOTCONV;
+    return rc;
+}
+
 int gpu_tomo_solve_dev(gpu_tomo_t *g, const float *d_grad, float *d_x, int nbatch, int mode, void *stream) {
     if (!g || nbatch <= 0 || nbatch > g->nbatch) return MT_EINVAL;
     return guarded([&] {
@@ -1047,7 +1087,7 @@ int gpu_tomo_P(gpu_tomo_t *g, const double *xin, double *xout) {
         launch_zero_i(g->stop.p, 1, g->stream);
         launch_row(g->G, g->SC, 1, ROW_MODE_INIT_COLD, g->rt64, 0, g->p.p, g->r.p, g->b.p, g->p.p, g->Ap.p,
                    g->specf.p, g->stop.p, g->stream);
-        enqueue_P_tail(g, 1, rec);
+        enqueue_P_tail(g, 1, IROW_MODE_Z, 0, g->z.p, rec);
         from_dev_x(g, xout, g->z.p);
     });
 }

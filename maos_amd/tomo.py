@@ -179,6 +179,19 @@ class GpuTomo:
         _abi.check(rc, "gpu_tomo_solve")
         return x, res.value, rc
 
+    def tomo_batch(self, grads):
+        """gpu_tomo_solve_batch: solve the stacked systems grads (nsys x ng)
+        in one batched call.  Returns x (nsys x nx); residuals via
+        residuals()."""
+        g = np.ascontiguousarray(grads, dtype=np.float64)
+        if g.ndim != 2 or g.shape[1] != self.ng or g.shape[0] > self.nbatch:
+            raise ValueError("gradients must be (nsys <= %d) x %d" % (self.nbatch, self.ng))
+        x = np.zeros((g.shape[0], self.nx))
+        res = np.zeros(g.shape[0])
+        _abi.check(_abi.lib().gpu_tomo_solve_batch(self._h, g.shape[0], _dp(g), _dp(x), _dp(res)),
+                   "gpu_tomo_solve_batch")
+        return x
+
     def reset(self):
         _abi.check(_abi.lib().gpu_tomo_reset(self._h), "gpu_tomo_reset")
 

@@ -63,3 +63,54 @@ def test_solve(sys_):
     assert rc == 0
     assert e <= 1e-4
     assert abs(resg - reso) <= 1e-3 * reso + 1e-6
+
+
+def test_batched_equals_single(sys_):
+    """Systems of one batched call match the same systems solved alone
+    (a system's result does not depend on its batch neighbours)."""
+    cfg, r, g1, o = sys_
+    grads = [workload.make_inputs(cfg, r, seed) for seed in (11, 12, 13)]
+    singles = [g1.tomo(gr)[0] for gr in grads]
+    gb = tomo.GpuTomo(r, nbatch=3, maxit=cfg["maxit"])
+    xb = gb.tomo_batch(np.stack(grads))
+    for i in range(3):
+        assert np.array_equal(xb[i], singles[i]), "system %d differs" % i
+
+
+def test_warm_start():
+    """Warm start (x0 = previous solution, resident on the device) against
+    the oracle started from its own previous solution (lib/pcg.c warm=1)."""
+    cfg = configs.small8(maxit=10)
+    r = tomo.Recon(cfg)
+    o = Oracle(r, nthread=8)
+    g = tomo.GpuTomo(r, maxit=10, warm=1)
+    xo = None
+    for step, seed in enumerate((21, 22, 23)):
+        grad = workload.make_inputs(cfg, r, seed)
+        xg, resg, rc = g.tomo(grad)
+        xo, reso = o.solve(grad, 10, x0=xo)
+        e = rel(xg, xo)
+        print("warm step %d: rel L2 %.3e residual gpu %.4e oracle %.4e" % (step, e, resg, reso))
+        assert e <= 1e-4
+
+
+def test_restart_past_100():
+    """maxit > 100 crosses the restart of lib/pcg.c:55 (r = b - A x, p = z).
+    Past ~60 iterations the FP32 operators let the iterates drift from the
+    FP64 ones smoothly (small8: 1e-5 at 60, 3.4e-4 at 99); the restart must
+    not add to that drift, and the residual must follow the oracle's."""
+    cfg = configs.small8()
+    r = tomo.Recon(cfg)
+    o = Oracle(r, nthread=8)
+    grad = workload.make_inputs(cfg, r, 1)
+    err = {}
+    for it in (99, 104):
+        g = tomo.GpuTomo(r, maxit=it)
+        xg, resg, rc = g.tomo(grad)
+        xo, reso = o.solve(grad, it)
+        err[it] = rel(xg, xo)
+        print("%d iterations: rel L2 %.3e residual gpu %.5e oracle %.5e" % (it, err[it], resg, reso))
+        assert abs(resg - reso) <= 0.05 * reso
+        g.close()
+    assert err[104] <= 1e-3
+    assert err[104] <= 1.5 * err[99] + 1e-5
