@@ -93,6 +93,8 @@ def main():
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", type=int, default=None,
+                    help="solve precision mode (0, 1, 2; default: the library default)")
     ap.add_argument("--modes", default="plain,graph,batched",
                     help="comma list of modes to time (profiling runs use one)")
     args = ap.parse_args()
@@ -136,6 +138,9 @@ def main():
     modes = set(args.modes.split(","))
     plain = tomo.GpuTomo(r, nbatch=1, maxit=maxit)
     batched = tomo.GpuTomo(r, nbatch=B, maxit=maxit)
+    if args.precision is not None:
+        plain.precision(args.precision)
+        batched.precision(args.precision)
     empty = dict(solves_per_s=float("nan"), p50_ms=float("nan"), p99_ms=float("nan"), ms_per_step=float("nan"))
     res_plain = run_mode(plain, 1, 0, args.steps, args.warmup) if "plain" in modes else empty
     res_graph = run_mode(plain, 1, 1, args.steps, args.warmup) if "graph" in modes else empty

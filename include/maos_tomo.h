@@ -248,10 +248,14 @@ int  gpu_tomo_L(gpu_tomo_t *g, const double *xin, double *xout,
 int  gpu_tomo_R(gpu_tomo_t *g, const double *grad, double *xout, double alpha);
 int  gpu_tomo_P(gpu_tomo_t *g, const double *xin, double *xout);
 
-/* Precision of the residual recurrence: mode 1 (default) keeps r, A*p and
- * the right-hand side in FP64 (operators, FFTs, x, p, z stay FP32); mode 0
- * keeps them in FP32.  Dot products and alpha/beta are FP64 in both.  A
- * 30-iteration FDPCG solve of the 30 m system drifts ~2e-3 (relative L2)
+/* Precision of the solve.  Operators (HX, GP, W, L2), x, p, z and the
+ * inverse FDPCG transform are FP32 and dot products / alpha / beta FP64 in
+ * every mode; the modes differ in the residual recurrence (r, A*p, b) and
+ * the forward FDPCG spectrum:
+ *   0  r, A*p, b FP32; forward spectrum FP32
+ *   1  r, A*p, b FP64; forward spectrum FP64   (default)
+ *   2  r, A*p, b FP64; forward spectrum FP32
+ * A 30-iteration FDPCG solve of the 30 m system drifts ~2e-3 (relative L2)
  * from the FP64 solve with mode 0 and stays ~1e-5 with mode 1. */
 int  gpu_tomo_precision(gpu_tomo_t *g, int mode);
 

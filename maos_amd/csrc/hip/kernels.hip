@@ -68,8 +68,28 @@ __device__ double reduce_partials(const double *part, int n, double *sh) {
 // contiguously; radices 2,3,4,5,7 (sizes that nextfftsize produces).
 // tw[k] = exp(-2 pi i k/n).  SIGN=-1 forward, +1 inverse, unnormalized.
 // ---------------------------------------------------------------------------
+// cos/sin(2 pi k / R), k = 0..R-1, for the odd radices (the R-th roots of
+// unity used inside one radix-R butterfly).
+template <int R> struct Roots;
+template <> struct Roots<3> {
+    static constexpr double c[3] = {1.0, -0.5, -0.5};
+    static constexpr double s[3] = {0.0, 0.86602540378443865, -0.86602540378443865};
+};
+template <> struct Roots<5> {
+    static constexpr double c[5] = {1.0, 0.30901699437494742, -0.80901699437494742, -0.80901699437494742,
+                                    0.30901699437494742};
+    static constexpr double s[5] = {0.0, 0.95105651629515357, 0.58778525229247313, -0.58778525229247313,
+                                    -0.95105651629515357};
+};
+template <> struct Roots<7> {
+    static constexpr double c[7] = {1.0, 0.62348980185873353, -0.22252093395631440, -0.90096886790241913,
+                                    -0.90096886790241913, -0.22252093395631440, 0.62348980185873353};
+    static constexpr double s[7] = {0.0, 0.78183148246802981, 0.97492791218182361, 0.43388373911755812,
+                                    -0.43388373911755812, -0.97492791218182361, -0.78183148246802981};
+};
+
 template <int R, int SIGN, typename C>
-__device__ __forceinline__ void small_dft(C *v, const C *tw, int n) {
+__device__ __forceinline__ void small_dft(C *v) {
     if constexpr (R == 2) {
         C a = v[0], b = v[1];
         v[0] = cadd(a, b);
@@ -84,15 +104,15 @@ __device__ __forceinline__ void small_dft(C *v, const C *tw, int n) {
         v[1] = cadd(a1, a3);
         v[3] = csub(a1, a3);
     } else {
+        typedef typename CT<C>::R F;
         C o[R];
-        const int step = n / R;
 #pragma unroll
         for (int k = 0; k < R; k++) {
             C acc = v[0];
 #pragma unroll
             for (int r = 1; r < R; r++) {
-                C w = tw[((k * r) % R) * step];
-                if (SIGN > 0) w.y = -w.y;
+                // exp(SIGN * 2 pi i k r / R)
+                const C w = CT<C>::make((F)Roots<R>::c[(k * r) % R], (F)(SIGN * Roots<R>::s[(k * r) % R]));
                 acc = cadd(acc, cmul(v[r], w));
             }
             o[k] = acc;
@@ -123,15 +143,17 @@ __device__ void fft_stage(const C *in, C *out, int n, int nf, int Ns, const C *t
                 v[r] = cmul(v[r], w);
             }
         }
-        small_dft<R, SIGN, C>(v, tw, n);
+        small_dft<R, SIGN, C>(v);
         const int idxD = (j / Ns) * Ns * R + jm;
 #pragma unroll
         for (int r = 0; r < R; r++) dst[idxD + r * Ns] = v[r];
     }
 }
 
-// Returns the buffer holding the result (a or b).
-template <int SIGN, typename C>
+// Returns the buffer holding the result (a or b).  tw: the n twiddles, in LDS.
+// RM: largest radix compiled in (3: radices 2, 3, 4; 7: also 5 and 7, whose
+// butterflies need about twice the registers).
+template <int SIGN, typename C, int RM>
 __device__ C *fft_run(C *a, C *b, int n, int nf, int nst, const int *rad, const C *tw) {
     int Ns = 1;
     C *in = a, *out = b;
@@ -141,8 +163,12 @@ __device__ C *fft_run(C *a, C *b, int n, int nf, int nst, const int *rad, const 
             case 2: fft_stage<2, SIGN, C>(in, out, n, nf, Ns, tw); break;
             case 3: fft_stage<3, SIGN, C>(in, out, n, nf, Ns, tw); break;
             case 4: fft_stage<4, SIGN, C>(in, out, n, nf, Ns, tw); break;
-            case 5: fft_stage<5, SIGN, C>(in, out, n, nf, Ns, tw); break;
-            case 7: fft_stage<7, SIGN, C>(in, out, n, nf, Ns, tw); break;
+            case 5:
+                if constexpr (RM >= 5) fft_stage<5, SIGN, C>(in, out, n, nf, Ns, tw);
+                break;
+            case 7:
+                if constexpr (RM >= 7) fft_stage<7, SIGN, C>(in, out, n, nf, Ns, tw);
+                break;
             default: break;
         }
         __syncthreads();
@@ -152,6 +178,12 @@ __device__ C *fft_run(C *a, C *b, int n, int nf, int nst, const int *rad, const 
         Ns *= R;
     }
     return in;
+}
+
+// Copy of the n twiddles into LDS (before the caller's first barrier).
+template <typename C>
+__device__ __forceinline__ void load_tw(C *dst, const C *src, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
 template <typename C> __device__ __forceinline__ const C *tw_of(const Geom &G, bool y);
@@ -377,78 +409,128 @@ __global__ __launch_bounds__(256) void k_layer(Geom G, const RT *q, const float 
 // ---------------------------------------------------------------------------
 template <int POS, typename RT>
 __global__ __launch_bounds__(256) void k_wfs_m(Geom G, const float *x, RT *q) {
-    constexpr int H = TILE + 2 * POS;
-    constexpr int H1 = H + 1;
-    constexpr int S = TILE + POS;
-    constexpr int NP = POS + 1;
+    constexpr int H = TILE + 2 * POS;   // opd window edge
+    constexpr int H1 = H + 1;           // layer window edge
+    constexpr int H12 = H1 * H1;
+    constexpr int S = TILE + POS;       // subaperture-corner window edge
+    constexpr int S2 = S * S;
+    constexpr int NP = POS + 1, NP2 = NP * NP;
+    constexpr int UG = (S2 * NP2 + 255) / 256, UN = (S2 + 255) / 256;
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    float *xs = reinterpret_cast<float *>(lds_raw);  // [npsr][H1*H1]
+    float2 *gps = reinterpret_cast<float2 *>(lds_raw);    // [S2][NP2] GP coefficients of the window
+    float *xs = reinterpret_cast<float *>(gps + S2 * NP2);  // [npsr][H12] layer windows
     __shared__ RT opd[H * H];
-    __shared__ RT wgx[S * S], wgy[S * S];
-    __shared__ int sidx[S * S];
+    __shared__ RT wgx[S2], wgy[S2];
     const int w = blockIdx.y, s = blockIdx.z;
     const int pnx = G.pnx, pny = G.pny;
     const int ntx = (pnx + TILE - 1) / TILE;
     const int tx0 = (blockIdx.x % ntx) * TILE, ty0 = (blockIdx.x / ntx) * TILE;
-    const float *xsys = x + (long)s * G.nxtot;
-    for (int l = 0; l < G.npsr; l++) {
-        const MRay m = G.mray[w * MAXL + l];
-        const int nxl = G.xnx[l], nyl = G.xny[l];
-        const float *xl = xsys + G.xoff[l];
-        const int gx0 = tx0 - POS + m.cx, gy0 = ty0 - POS + m.cy;
-        for (int i = threadIdx.x; i < H1 * H1; i += blockDim.x) {
-            int gx = gx0 + i % H1, gy = gy0 + i / H1;
-            if (gx == nxl && m.wx) gx = 0;  // edge ray: second node wraps
-            if (gy == nyl && m.wy) gy = 0;
-            xs[l * H1 * H1 + i] = (gx >= 0 && gx < nxl && gy >= 0 && gy < nyl) ? xl[gy * nxl + gx] : 0.f;
+    const float2 *gpw = G.gpd + (long)G.gcls[w] * G.nploc * NP2;
+    const float4 *nw4 = G.nead + (long)G.gcls[w] * G.nploc;
+    // geometry of the window (independent of the rays): issued first
+    float2 gv[UG];
+    float4 nv[UN];
+#pragma unroll
+    for (int u = 0; u < UG; u++) {
+        const int i = threadIdx.x + 256 * u;
+        if (i < S2 * NP2) {
+            const int c = i / NP2, z = i - c * NP2;
+            const int cx = tx0 - POS + c % S, cy = ty0 - POS + c / S;
+            gv[u] = (cx >= 0 && cx < pnx && cy >= 0 && cy < pny) ? gpw[(long)(cy * pnx + cx) * NP2 + z]
+                                                               : make_float2(0.f, 0.f);
         }
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < H * H; i += blockDim.x) {
-        const int a = i % H, b = i / H;
-        const int px = tx0 - POS + a, py = ty0 - POS + b;
-        RT v = 0;
-        if (px >= 0 && px < pnx && py >= 0 && py < pny) {
-            for (int l = 0; l < G.npsr; l++) {
+#pragma unroll
+    for (int u = 0; u < UN; u++) {
+        const int i = threadIdx.x + 256 * u;
+        const int cx = tx0 - POS + i % S, cy = ty0 - POS + i / S;
+        nv[u] = (i < S2 && cx >= 0 && cx < pnx && cy >= 0 && cy < pny) ? nw4[cy * pnx + cx]
+                                                                       : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    // layer windows: window (a, b) of layer l holds node (tx0-POS+c+a, ty0-POS+c+b);
+    // four layers' loads in flight at a time (ray offsets are uniform: scalar)
+    constexpr int UH = (H12 + 255) / 256;
+    const float *xsys = x + (long)s * G.nxtot;
+    for (int l0 = 0; l0 < G.npsr; l0 += 4) {
+        float xv[4][UH];
+#pragma unroll
+        for (int dl = 0; dl < 4; dl++) {
+            const int l = l0 + dl;
+            if (l < G.npsr) {
                 const MRay m = G.mray[w * MAXL + l];
-                if (px >= m.ax0 && px < m.ax1 && py >= m.ay0 && py < m.ay1) {
-                    const float *t = xs + l * H1 * H1 + b * H1 + a;
-                    const RT fx = m.fx, fy = m.fy;
-                    v += (RT(1) - fy) * ((RT(1) - fx) * (RT)t[0] + fx * (RT)t[1]) +
-                         fy * ((RT(1) - fx) * (RT)t[H1] + fx * (RT)t[H1 + 1]);
+                const int nxl = G.xnx[l], nyl = G.xny[l];
+                const float *xl = xsys + G.xoff[l];
+#pragma unroll
+                for (int u = 0; u < UH; u++) {
+                    const int j = threadIdx.x + 256 * u;
+                    int gx = tx0 - POS + m.cx + j % H1, gy = ty0 - POS + m.cy + j / H1;
+                    gx = (gx == nxl && m.wx) ? 0 : gx;  // edge ray: second node wraps
+                    gy = (gy == nyl && m.wy) ? 0 : gy;
+                    const bool ok = j < H12 && gx >= 0 && gx < nxl && gy >= 0 && gy < nyl;
+                    xv[dl][u] = ok ? xl[gy * nxl + gx] : 0.f;
                 }
             }
         }
-        opd[i] = v;
+#pragma unroll
+        for (int dl = 0; dl < 4; dl++)
+            if (l0 + dl < G.npsr)
+#pragma unroll
+                for (int u = 0; u < UH; u++) {
+                    const int j = threadIdx.x + 256 * u;
+                    if (j < H12) xs[(l0 + dl) * H12 + j] = xv[dl][u];
+                }
+    }
+#pragma unroll
+    for (int u = 0; u < UG; u++) {
+        const int i = threadIdx.x + 256 * u;
+        if (i < S2 * NP2) gps[i] = gv[u];
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < S * S; i += blockDim.x) {
-        const int a = i % S, b = i / S;
-        const int cx = tx0 - POS + a, cy = ty0 - POS + b;
-        int sa = -1;
-        if (cx >= 0 && cx < pnx && cy >= 0 && cy < pny) sa = G.saidx[(long)w * G.nploc + cy * pnx + cx];
-        sidx[i] = sa;
-        RT ox = 0, oy = 0;
-        if (sa >= 0) {
-            const long so = G.saoff[w] + sa;
+    // HX on the opd window
+    constexpr int UO = (H * H + 255) / 256;
+#pragma unroll
+    for (int u = 0; u < UO; u++) {
+        const int i = threadIdx.x + 256 * u;
+        const int a = i % H, b = i / H;
+        const int px = tx0 - POS + a, py = ty0 - POS + b;
+        if (i < H * H) {
+            RT v = 0;
+            for (int l = 0; l < G.npsr; l++) {
+                const MRay m = G.mray[w * MAXL + l];
+                const bool ok = px >= m.ax0 && px < m.ax1 && py >= m.ay0 && py < m.ay1;
+                const float *t = xs + l * H12 + b * H1 + a;
+                const RT fx = m.fx, fy = m.fy;
+                const RT h = (RT(1) - fy) * ((RT(1) - fx) * (RT)t[0] + fx * (RT)t[1]) +
+                             fy * ((RT(1) - fx) * (RT)t[H1] + fx * (RT)t[H1 + 1]);
+                v += ok ? h : RT(0);
+            }
+            opd[i] = v;
+        }
+    }
+    __syncthreads();
+    // W GP per subaperture corner of the window (zero coefficients where none)
+#pragma unroll
+    for (int u = 0; u < UN; u++) {
+        const int i = threadIdx.x + 256 * u;
+        if (i < S2) {
+            const int a = i % S, b = i / S;
             RT gx = 0, gy = 0;
-            const float *cxp = G.gpx + so * NP * NP, *cyp = G.gpy + so * NP * NP;
 #pragma unroll
             for (int zy = 0; zy < NP; zy++)
 #pragma unroll
                 for (int zx = 0; zx < NP; zx++) {
+                    const float2 c = gps[i * NP2 + zy * NP + zx];
                     const RT o = opd[(b + zy) * H + a + zx];
-                    gx += (RT)cxp[zy * NP + zx] * o;
-                    gy += (RT)cyp[zy * NP + zx] * o;
+                    gx += (RT)c.x * o;
+                    gy += (RT)c.y * o;
                 }
-            const float4 ne = G.neai[so];
-            ox = (RT)ne.x * gx + (RT)ne.z * gy;
-            oy = (RT)ne.z * gx + (RT)ne.y * gy;
+            const float4 ne = nv[u];
+            wgx[i] = (RT)ne.x * gx + (RT)ne.z * gy;
+            wgy[i] = (RT)ne.z * gx + (RT)ne.y * gy;
         }
-        wgx[i] = ox;
-        wgy[i] = oy;
     }
     __syncthreads();
+    // GP' gather onto the tile
     const int ox = threadIdx.x % TILE, oy = threadIdx.x / TILE;
     const int px = tx0 + ox, py = ty0 + oy;
     if (threadIdx.x < TILE * TILE && px < pnx && py < pny) {
@@ -458,12 +540,8 @@ __global__ __launch_bounds__(256) void k_wfs_m(Geom G, const float *x, RT *q) {
 #pragma unroll
             for (int zx = 0; zx < NP; zx++) {
                 const int si = (oy + POS - zy) * S + (ox + POS - zx);
-                const int sa = sidx[si];
-                if (sa >= 0) {
-                    const long so = G.saoff[w] + sa;
-                    acc += (RT)G.gpx[so * NP * NP + zy * NP + zx] * wgx[si] +
-                           (RT)G.gpy[so * NP * NP + zy * NP + zx] * wgy[si];
-                }
+                const float2 c = gps[si * NP2 + zy * NP + zx];
+                acc += (RT)c.x * wgx[si] + (RT)c.y * wgy[si];
             }
         q[((long)s * G.nwfs + w) * G.nploc + py * pnx + px] = acc;
     }
@@ -473,39 +551,67 @@ template <bool LHS, typename RT>
 __global__ __launch_bounds__(256) void k_layer_m(Geom G, const RT *q, const float *p, RT *out,
                                                  double *part, int partstride) {
     constexpr int Q = TILE + 1;      // WFS-field window edge
+    constexpr int Q2 = Q * Q;
     constexpr int PH = TILE + 4;     // p window edge (L2'L2 reach is 2)
+    constexpr int PH2 = PH * PH;
+    constexpr int UQ = (Q2 + 255) / 256, UP = (PH2 + 255) / 256;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     RT *qs = reinterpret_cast<RT *>(lds_raw);                       // [nwfs][Q*Q]
-    float *ps = reinterpret_cast<float *>(qs + G.nwfs * Q * Q);     // [PH*PH]
+    float *ps = reinterpret_cast<float *>(qs + G.nwfs * Q2);        // [PH*PH]
     __shared__ double sh[8];
     const int l = blockIdx.y, s = blockIdx.z;
     const int nxl = G.xnx[l], nyl = G.xny[l];
     const int ntx = (nxl + TILE - 1) / TILE, nty = (nyl + TILE - 1) / TILE;
     const bool active = (int)blockIdx.x < ntx * nty;
     const int jx0 = (blockIdx.x % ntx) * TILE, jy0 = (blockIdx.x / ntx) * TILE;
+    // p window and four WFS windows at a time are loaded with every load of
+    // the group in flight; ray offsets are uniform (scalar loads).
     if (active) {
-        for (int w = 0; w < G.nwfs; w++) {
-            const MRay m = G.mray[w * MAXL + l];
-            const RT *qw = q + ((long)s * G.nwfs + w) * G.nploc;
-            const int ix0 = jx0 - m.cx - 1, iy0 = jy0 - m.cy - 1;
-            for (int i = threadIdx.x; i < Q * Q; i += blockDim.x) {
-                int ix = ix0 + i % Q, iy = iy0 + i / Q;
-                // node 0 is the wrapped second node of the edge ray a1 - 1;
-                // slot i = -c - 1 (node 0's second-node ray) is otherwise empty
-                if (m.wx && ix == -m.cx - 1) ix = m.ax1 - 1;
-                if (m.wy && iy == -m.cy - 1) iy = m.ay1 - 1;
-                RT v = 0;
-                if (ix >= m.ax0 && ix < m.ax1 && iy >= m.ay0 && iy < m.ay1) v = qw[iy * G.pnx + ix];
-                qs[w * Q * Q + i] = v;
-            }
-        }
         if (LHS) {
             const float *pl = p + (long)s * G.nxtot + G.xoff[l];
-            for (int i = threadIdx.x; i < PH * PH; i += blockDim.x) {
+            float pv[UP];
+#pragma unroll
+            for (int u = 0; u < UP; u++) {
+                const int i = threadIdx.x + 256 * u;
                 // periodic; the last tile's window may run past a small layer
                 const int xx = (jx0 - 2 + i % PH + nxl) % nxl, yy = (jy0 - 2 + i / PH + nyl) % nyl;
-                ps[i] = pl[yy * nxl + xx];
+                pv[u] = i < PH2 ? pl[yy * nxl + xx] : 0.f;
             }
+#pragma unroll
+            for (int u = 0; u < UP; u++) {
+                const int i = threadIdx.x + 256 * u;
+                if (i < PH2) ps[i] = pv[u];
+            }
+        }
+        for (int w0 = 0; w0 < G.nwfs; w0 += 4) {
+            RT qv[4][UQ];
+#pragma unroll
+            for (int dw = 0; dw < 4; dw++) {
+                const int w = w0 + dw;
+                if (w < G.nwfs) {
+                    const MRay m = G.mray[w * MAXL + l];
+                    const RT *qw = q + ((long)s * G.nwfs + w) * G.nploc;
+#pragma unroll
+                    for (int u = 0; u < UQ; u++) {
+                        const int j = threadIdx.x + 256 * u;
+                        int ix = jx0 - m.cx - 1 + j % Q, iy = jy0 - m.cy - 1 + j / Q;
+                        // node 0 is the wrapped second node of the edge ray a1 - 1;
+                        // slot i = -c - 1 (node 0's second-node ray) is otherwise empty
+                        ix = (m.wx && ix == -m.cx - 1) ? m.ax1 - 1 : ix;
+                        iy = (m.wy && iy == -m.cy - 1) ? m.ay1 - 1 : iy;
+                        const bool ok = j < Q2 && ix >= m.ax0 && ix < m.ax1 && iy >= m.ay0 && iy < m.ay1;
+                        qv[dw][u] = ok ? qw[iy * G.pnx + ix] : RT(0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int dw = 0; dw < 4; dw++)
+                if (w0 + dw < G.nwfs)
+#pragma unroll
+                    for (int u = 0; u < UQ; u++) {
+                        const int j = threadIdx.x + 256 * u;
+                        if (j < Q2) qs[(w0 + dw) * Q2 + j] = qv[dw][u];
+                    }
         }
     }
     __syncthreads();
@@ -517,7 +623,7 @@ __global__ __launch_bounds__(256) void k_layer_m(Geom G, const RT *q, const floa
         for (int w = 0; w < G.nwfs; w++) {
             const MRay m = G.mray[w * MAXL + l];
             const RT fx = m.fx, fy = m.fy;
-            const RT *t = qs + w * Q * Q + ty * Q + tx;
+            const RT *t = qs + w * Q2 + ty * Q + tx;
             acc += fy * ((RT(1) - fx) * t[1] + fx * t[0]) + (RT(1) - fy) * ((RT(1) - fx) * t[Q + 1] + fx * t[Q]);
         }
         const long o = (long)s * G.nxtot + G.xoff[l];
@@ -544,7 +650,7 @@ __global__ __launch_bounds__(256) void k_layer_m(Geom G, const RT *q, const floa
 // ---------------------------------------------------------------------------
 enum { ROW_INIT_COLD = 0, ROW_INIT_WARM = 1, ROW_UPDATE = 2, ROW_LAST = 3 };
 
-template <int MODE, typename RT, typename FT>
+template <int MODE, typename RT, typename FT, int RM>
 __global__ __launch_bounds__(256) void k_row(Geom G, Scal SC, int iter, float *x, RT *r,
                                              const RT *b, const float *p, const RT *Ap,
                                              FT *spec, int *stop) {
@@ -558,6 +664,38 @@ __global__ __launch_bounds__(256) void k_row(Geom G, Scal SC, int iter, float *x
     const int n = G.fnx, nyl = G.xny[l];
     const int nrows = min(ROWS, nyl - row0);
     const int nf = nrows / 2;
+    FT *twl = lds + ROWS * n;
+    if (MODE != ROW_LAST) load_tw(twl, tw_of<FT>(G, false), n);
+    const long base = (long)s * G.nxtot + G.xoff[l] + (long)row0 * n;
+    const int tot = nrows * n;
+    // First chunk of the vector loads, issued before alpha is reduced (they
+    // do not depend on it); further chunks (rows longer than 96) in the loop.
+    constexpr int U = 6;
+    RT v0[U], v1[U];
+    float pv[U], xv[U];
+    auto load = [&](int c0) {
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = c0 + threadIdx.x + 256 * u;
+            if (i < tot) {
+                const long idx = base + i;
+                if (MODE == ROW_INIT_COLD) {
+                    v0[u] = b[idx];
+                } else if (MODE == ROW_INIT_WARM) {
+                    v0[u] = b[idx];
+                    v1[u] = Ap[idx];
+                } else {
+                    v0[u] = r[idx];
+                    v1[u] = Ap[idx];
+                    if (MODE == ROW_LAST) {
+                        pv[u] = p[idx];
+                        xv[u] = x[idx];
+                    }
+                }
+            }
+        }
+    };
+    load(0);
     double alpha = 0.;
     if (MODE >= ROW_UPDATE) {
         const double pap = reduce_partials(SC.part_pap + (long)s * SC.maxpart, SC.npart_layer, sh);
@@ -575,29 +713,30 @@ __global__ __launch_bounds__(256) void k_row(Geom G, Scal SC, int iter, float *x
     // x += alpha p of an UPDATE iteration is applied by the inverse row
     // kernel, which reads p anyway for p = z + beta p.
     if (MODE == ROW_UPDATE && blockIdx.x == 0 && threadIdx.x == 0) SC.alpha[s] = alpha;
-    const long base = (long)s * G.nxtot + G.xoff[l] + (long)row0 * n;
     FR *bufA = reinterpret_cast<FR *>(lds);
     double dot = 0;
-    for (int i = threadIdx.x; i < nrows * n; i += blockDim.x) {
-        const int row = i / n, col = i - row * n;
-        const long idx = base + (long)row * n + col;
-        RT rv;
-        if (MODE == ROW_INIT_COLD) {
-            rv = b[idx];
-            r[idx] = rv;
-            x[idx] = 0.f;
-        } else if (MODE == ROW_INIT_WARM) {
-            rv = b[idx] - Ap[idx];
-            r[idx] = rv;
-        } else {
-            if (MODE == ROW_LAST) x[idx] += (float)alpha * p[idx];
-            rv = r[idx] - (RT)alpha * Ap[idx];
-            r[idx] = rv;
-        }
-        if (MODE == ROW_LAST) {
-            dot += (double)rv * (double)rv;
-        } else {
-            bufA[2 * ((row >> 1) * n + col) + (row & 1)] = (FR)rv;
+    for (int c0 = 0; c0 < tot; c0 += 256 * U) {
+        if (c0 > 0) load(c0);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = c0 + threadIdx.x + 256 * u;
+            if (i < tot) {
+                const int row = i / n, col = i - row * n;
+                const long idx = base + i;
+                RT rv;
+                if (MODE == ROW_INIT_COLD) {
+                    rv = v0[u];
+                    x[idx] = 0.f;
+                } else if (MODE == ROW_INIT_WARM) {
+                    rv = v0[u] - v1[u];
+                } else {
+                    if (MODE == ROW_LAST) x[idx] = xv[u] + (float)alpha * pv[u];
+                    rv = v0[u] - (RT)alpha * v1[u];
+                }
+                r[idx] = rv;
+                if (MODE == ROW_LAST) dot += (double)rv * (double)rv;
+                else bufA[2 * ((row >> 1) * n + col) + (row & 1)] = (FR)rv;
+            }
         }
     }
     if (MODE == ROW_LAST) {
@@ -607,7 +746,7 @@ __global__ __launch_bounds__(256) void k_row(Geom G, Scal SC, int iter, float *x
     }
     __syncthreads();
     FT *A = lds, *B = lds + ROWS / 2 * n;
-    FT *res = fft_run<-1, FT>(A, B, n, nf, G.nstx, G.radx, tw_of<FT>(G, false));
+    FT *res = fft_run<-1, FT, RM>(A, B, n, nf, G.nstx, G.radx, twl);
     const int nh = n / 2 + 1;
     FT *S = spec + (long)s * G.nspec + G.soff[l];
     for (int i = threadIdx.x; i < nf * nh; i += blockDim.x) {
@@ -623,32 +762,52 @@ __global__ __launch_bounds__(256) void k_row(Geom G, Scal SC, int iter, float *x
 // ---------------------------------------------------------------------------
 // Column kernel: forward column FFTs of one column group (all layers),
 // per-frequency bs x bs block multiply (half spectrum, conjugate flags), then
-// inverse column FFTs.  Everything in LDS.
+// inverse column FFTs, for CS systems per workgroup (each block row of Minv
+// and its positions are loaded once and applied to all CS systems).
+// Everything in LDS: two buffers of CS * ncol * npsr * ny entries; the
+// multiply writes the buffer the forward FFT did not end in, and the inverse
+// FFT ping-pongs between it and the (then free) forward result.
 // ---------------------------------------------------------------------------
-template <typename FT>
-__global__ __launch_bounds__(256) void k_col(Geom G, Scal SC, const FT *specf, float2 *speci) {
+template <typename FT, int CS, int BS, int RM>
+__global__ __launch_bounds__(256) void k_col(Geom G, Scal SC, int nsys, const FT *specf, float2 *speci) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double sh[8];
     typedef typename CT<FT>::R FR;
-    const int g = blockIdx.x, s = blockIdx.z;
-    const int ny = G.fny;
+    const int g = blockIdx.x;
+    const int s0 = blockIdx.z * CS;
+    const int ns = min(CS, nsys - s0);
+    const int ny = G.fny, npsr = G.npsr;
     const int c0 = G.grp_col[g], c1 = G.grp_col[g + 1];
     const int ncg = c1 - c0;
-    const int nf = ncg * G.npsr;
-    const int tot = nf * ny;
-    const int cap = G.maxcol * G.npsr * ny;
+    const int per = ncg * npsr * ny;  // entries of one system
+    const int tot = ns * per;
+    const int cap = CS * G.maxcol * npsr * ny;
     FT *A = reinterpret_cast<FT *>(lds_raw), *B = A + cap;
-    float2 *C = reinterpret_cast<float2 *>(B + cap), *D = C + cap;
-    const FT *Sf = specf + (long)s * G.nspec;
-    float2 *Si = speci + (long)s * G.nspec;
-    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-        const int f = i / ny, ky = i - f * ny;
-        const int ci = f / G.npsr, l = f - ci * G.npsr;
-        const int kx = G.colkx[c0 + ci];
-        A[i] = Sf[G.soff[l] + (long)kx * ny + ky];
+    FT *twf = B + cap;
+    float2 *twi = reinterpret_cast<float2 *>(twf + ny);
+    load_tw(twf, tw_of<FT>(G, true), ny);
+    load_tw(twi, G.twy, ny);
+    constexpr int U = 8;  // loads in flight per thread
+    for (int b0 = 0; b0 < tot; b0 += 256 * U) {
+        FT v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = b0 + threadIdx.x + 256 * u;
+            if (i < tot) {
+                const int si = i / per, j = i - si * per;
+                const int f = j / ny, ky = j - f * ny;
+                const int ci = f / npsr, l = f - ci * npsr;
+                v[u] = specf[(long)(s0 + si) * G.nspec + G.soff[l] + (long)G.colkx[c0 + ci] * ny + ky];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = b0 + threadIdx.x + 256 * u;
+            if (i < tot) A[i] = v[u];
+        }
     }
     __syncthreads();
-    FT *res = fft_run<-1, FT>(A, B, ny, nf, G.nsty, G.rady, tw_of<FT>(G, true));
+    FT *res = fft_run<-1, FT, RM>(A, B, ny, ns * ncg * npsr, G.nsty, G.rady, twf);
     if (G.scale) {
         for (int i = threadIdx.x; i < tot; i += blockDim.x) {
             res[i].x *= (FR)G.fscale;
@@ -666,7 +825,10 @@ __global__ __launch_bounds__(256) void k_col(Geom G, Scal SC, const FT *specf, f
     if (G.zero_dc) {
         for (int ci = 0; ci < ncg; ci++) {
             if (G.colkx[c0 + ci] != 0) continue;
-            for (int l = threadIdx.x; l < G.npsr; l += blockDim.x) res[(ci * G.npsr + l) * ny] = CT<FT>::make(0, 0);
+            for (int t = threadIdx.x; t < ns * npsr; t += blockDim.x) {
+                const int si = t / npsr, l = t - si * npsr;
+                res[si * per + (ci * npsr + l) * ny] = CT<FT>::make(0, 0);
+            }
         }
         __syncthreads();
     }
@@ -674,40 +836,67 @@ __global__ __launch_bounds__(256) void k_col(Geom G, Scal SC, const FT *specf, f
     // sum_xy r z = sc * sum_k conj(R_k) Z_k over the full spectrum (Parseval;
     // sc = fscale when scaled, and R here already carries that fscale), i.e.
     // the half-spectrum sum with weight 2 off the kx = 0 and kx = n/2 columns.
-    const int bs = G.bs;
+    float2 *C = reinterpret_cast<float2 *>(res == A ? B : A);
+    float2 *D = reinterpret_cast<float2 *>(res);
+    // BS > 0: block size known at compile time (loads of a block row issued
+    // together); BS = 0: runtime G.bs.  Blocks are stored in group order.
+    const int bs = BS > 0 ? BS : G.bs;
     const int b0 = G.grp_blk[g], b1 = G.grp_blk[g + 1];
-    double dot = 0;
+    double dot[CS];
+#pragma unroll
+    for (int si = 0; si < CS; si++) dot[si] = 0;
     for (int t = threadIdx.x; t < (b1 - b0) * bs; t += blockDim.x) {
-        const int bi = G.blkid[b0 + t / bs];
-        const int m = t % bs;
+        const int bi = b0 + t / bs;
+        const int m = t - (t / bs) * bs;
         const float2 *M = G.Minv + (long)bi * bs * bs;
         const int *bp = G.blkpos + (long)bi * bs;
-        FT y = CT<FT>::make(0, 0);
+        FT y[CS];
+#pragma unroll
+        for (int si = 0; si < CS; si++) y[si] = CT<FT>::make(0, 0);
+#pragma unroll
         for (int k = 0; k < bs; k++) {
             const int pe = bp[k];
-            FT v = res[pe & 0x1fffffff];
-            if ((pe >> 30) & 1) v.y = -v.y;
+            const int pos = pe & 0x1fffffff;
+            const FR cj = ((pe >> 30) & 1) ? FR(-1) : FR(1);
             const float2 mk = M[m + k * bs];
-            y = cadd(y, cmul(CT<FT>::make(mk.x, mk.y), v));
+            const FT mm = CT<FT>::make(mk.x, mk.y);
+#pragma unroll
+            for (int si = 0; si < CS; si++) {
+                if (si < ns) {
+                    FT v = res[si * per + pos];
+                    v.y *= cj;
+                    y[si] = cadd(y[si], cmul(mm, v));
+                }
+            }
         }
         const int pm = bp[m];
-        if ((pm >> 30) & 1) y.y = -y.y;
         if (!((pm >> 29) & 1)) {
             const int pos = pm & 0x1fffffff;
-            C[pos] = make_float2((float)y.x, (float)y.y);
-            const int kx = G.colkx[c0 + pos / ny / G.npsr];
+            const FR cj = ((pm >> 30) & 1) ? FR(-1) : FR(1);
+            const int kx = G.colkx[c0 + pos / ny / npsr];
             const double wgt = (kx == 0 || 2 * kx == G.fnx) ? 1.0 : 2.0;
-            dot += wgt * ((double)res[pos].x * (double)y.x + (double)res[pos].y * (double)y.y);
+#pragma unroll
+            for (int si = 0; si < CS; si++) {
+                if (si < ns) {
+                    const FT yy = CT<FT>::make(y[si].x, cj * y[si].y);
+                    C[si * per + pos] = make_float2((float)yy.x, (float)yy.y);
+                    const FT rv = res[si * per + pos];
+                    dot[si] += wgt * ((double)rv.x * (double)yy.x + (double)rv.y * (double)yy.y);
+                }
+            }
         }
     }
-    dot = block_sum(dot, sh);  // contains the barrier before the inverse FFT
-    if (threadIdx.x == 0) SC.part_rz[(long)s * SC.maxpart + g] = dot;
-    float2 *res2 = fft_run<1, float2>(C, D, ny, nf, G.nsty, G.rady, G.twy);
+#pragma unroll
+    for (int si = 0; si < CS; si++) {
+        const double d = block_sum(dot[si], sh);  // its barriers also order C before the inverse FFT
+        if (threadIdx.x == 0 && si < ns) SC.part_rz[(long)(s0 + si) * SC.maxpart + g] = d;
+    }
+    float2 *res2 = fft_run<1, float2, RM>(C, D, ny, ns * ncg * npsr, G.nsty, G.rady, twi);
     for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-        const int f = i / ny, ky = i - f * ny;
-        const int ci = f / G.npsr, l = f - ci * G.npsr;
-        const int kx = G.colkx[c0 + ci];
-        Si[G.soff[l] + (long)kx * ny + ky] = res2[i];
+        const int si = i / per, j = i - si * per;
+        const int f = j / ny, ky = j - f * ny;
+        const int ci = f / npsr, l = f - ci * npsr;
+        speci[(long)(s0 + si) * G.nspec + G.soff[l] + (long)G.colkx[c0 + ci] * ny + ky] = res2[i];
     }
 }
 
@@ -723,13 +912,75 @@ __global__ __launch_bounds__(256) void k_col(Geom G, Scal SC, const FT *specf, f
 // ---------------------------------------------------------------------------
 enum { IROW_Z = 0, IROW_FIRST = 1, IROW_UPDATE = 2 };
 
-template <int MODE>
+template <int MODE, int RM>
 __global__ __launch_bounds__(256) void k_irow(Geom G, Scal SC, int slot, const float2 *spec, float *x,
                                               float *out) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     float2 *lds = reinterpret_cast<float2 *>(lds_raw);
     __shared__ double sh[8];
     const int s = blockIdx.z;
+    const int2 rb = G.rowblk[blockIdx.x];
+    const int l = rb.x, row0 = rb.y;
+    const int n = G.fnx, nyl = G.xny[l];
+    const int nrows = min(ROWS, nyl - row0);
+    const int nf = nrows / 2;
+    const float2 *S = spec + (long)s * G.nspec + G.soff[l];
+    float2 *A = lds, *B = lds + ROWS / 2 * n;
+    float2 *twl = lds + ROWS * n;
+    load_tw(twl, G.twx, n);
+    // Loads first (spectrum, and x, p for the update), then the r.z
+    // reduction while they are in flight.
+    const int tots = nf * n;
+    constexpr int US = 4;
+    float2 va[US], vb[US];
+    auto load_spec = [&](int c0) {
+#pragma unroll
+        for (int u = 0; u < US; u++) {
+            const int i = c0 + threadIdx.x + 256 * u;
+            if (i < tots) {
+                const int f = i % nf, k = i / nf;
+                const int ya = row0 + 2 * f;
+                const long o = (long)(k <= n / 2 ? k : n - k) * nyl + ya;
+                va[u] = S[o];
+                vb[u] = S[o + 1];
+            }
+        }
+    };
+    auto store_spec = [&](int c0) {
+#pragma unroll
+        for (int u = 0; u < US; u++) {
+            const int i = c0 + threadIdx.x + 256 * u;
+            if (i < tots) {
+                const int f = i % nf, k = i / nf;
+                float2 a = va[u], b = vb[u];
+                if (k > n / 2) {
+                    a = cconj(a);
+                    b = cconj(b);
+                } else if (k == 0 || k == n / 2) {
+                    a.y = 0.f;
+                    b.y = 0.f;
+                }
+                A[f * n + k] = make_float2(a.x - b.y, a.y + b.x);
+            }
+        }
+    };
+    const long base = (long)s * G.nxtot + G.xoff[l] + (long)row0 * n;
+    const int tot = nrows * n;
+    constexpr int U = 6;
+    float pv[U], xv[U];
+    auto load_xp = [&](int c0) {
+        if (MODE != IROW_UPDATE) return;
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = c0 + threadIdx.x + 256 * u;
+            if (i < tot) {
+                pv[u] = out[base + i];
+                xv[u] = x[base + i];
+            }
+        }
+    };
+    load_spec(0);
+    load_xp(0);
     double beta = 0, alpha = 0;
     if (MODE != IROW_Z) {
         const double rz = reduce_partials(SC.part_rz + (long)s * SC.maxpart, SC.npart_col, sh);
@@ -741,46 +992,30 @@ __global__ __launch_bounds__(256) void k_irow(Geom G, Scal SC, int slot, const f
         }
         if (blockIdx.x == 0 && threadIdx.x == 0) hist[MODE == IROW_FIRST ? slot : slot + 1] = rz;
     }
-    const int2 rb = G.rowblk[blockIdx.x];
-    const int l = rb.x, row0 = rb.y;
-    const int n = G.fnx, nyl = G.xny[l];
-    const int nrows = min(ROWS, nyl - row0);
-    const int nf = nrows / 2;
-    const float2 *S = spec + (long)s * G.nspec + G.soff[l];
-    float2 *A = lds, *B = lds + ROWS / 2 * n;
-    for (int i = threadIdx.x; i < nf * n; i += blockDim.x) {
-        const int f = i % nf, k = i / nf;
-        const int ya = row0 + 2 * f;
-        float2 a, b;
-        if (k <= n / 2) {
-            a = S[(long)k * nyl + ya];
-            b = S[(long)k * nyl + ya + 1];
-            if (k == 0 || k == n / 2) {
-                a.y = 0.f;
-                b.y = 0.f;
-            }
-        } else {
-            a = cconj(S[(long)(n - k) * nyl + ya]);
-            b = cconj(S[(long)(n - k) * nyl + ya + 1]);
-        }
-        A[f * n + k] = make_float2(a.x - b.y, a.y + b.x);
+    for (int c0 = 0; c0 < tots; c0 += 256 * US) {
+        if (c0 > 0) load_spec(c0);
+        store_spec(c0);
     }
     __syncthreads();
-    float2 *res = fft_run<1, float2>(A, B, n, nf, G.nstx, G.radx, G.twx);
+    float2 *res = fft_run<1, float2, RM>(A, B, n, nf, G.nstx, G.radx, twl);
     const float sc = G.scale ? G.fscale : 1.f;
     const float fa = (float)alpha, fb = (float)beta;
-    const long base = (long)s * G.nxtot + G.xoff[l] + (long)row0 * n;
-    for (int i = threadIdx.x; i < nrows * n; i += blockDim.x) {
-        const int row = i / n, col = i - row * n;
-        const float2 v = res[(row >> 1) * n + col];
-        const float zv = ((row & 1) ? v.y : v.x) * sc;
-        const long idx = base + (long)row * n + col;
-        if (MODE == IROW_UPDATE) {
-            const float pv = out[idx];
-            x[idx] += fa * pv;
-            out[idx] = zv + fb * pv;
-        } else {
-            out[idx] = zv;
+    for (int c0 = 0; c0 < tot; c0 += 256 * U) {
+        if (c0 > 0) load_xp(c0);
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = c0 + threadIdx.x + 256 * u;
+            if (i < tot) {
+                const int row = i / n, col = i - row * n;
+                const float2 v = res[(row >> 1) * n + col];
+                const float zv = ((row & 1) ? v.y : v.x) * sc;
+                if (MODE == IROW_UPDATE) {
+                    x[base + i] = xv[u] + fa * pv[u];
+                    out[base + i] = zv + fb * pv[u];
+                } else {
+                    out[base + i] = zv;
+                }
+            }
         }
     }
 }
@@ -825,8 +1060,8 @@ template <typename RT>
 static void launch_wfs_m(const Geom &G, int nsys, const float *x, RT *q, hipStream_t st) {
     const int nt = ((G.pnx + TILE - 1) / TILE) * ((G.pny + TILE - 1) / TILE);
     dim3 grid(nt, G.nwfs, nsys);
-    const int H1 = TILE + 2 * G.pos + 1;
-    const size_t lds = sizeof(float) * G.npsr * H1 * H1;
+    const int H1 = TILE + 2 * G.pos + 1, S = TILE + G.pos, NP = G.pos + 1;
+    const size_t lds = sizeof(float2) * S * S * NP * NP + sizeof(float) * G.npsr * H1 * H1;
     if (G.pos == 1) hipLaunchKernelGGL((k_wfs_m<1, RT>), grid, dim3(256), lds, st, G, x, q);
     else hipLaunchKernelGGL((k_wfs_m<2, RT>), grid, dim3(256), lds, st, G, x, q);
 }
@@ -880,59 +1115,114 @@ void launch_layer(const Geom &G, int nsys, bool lhs, int rt64, const void *q, co
         else hipLaunchKernelGGL((k_layer<false, float>), grid, dim3(256), 0, st, G, qq, p, o, part, partstride);
     }
 }
-size_t row_lds(const Geom &G, int f64) { return (f64 ? sizeof(double2) : sizeof(float2)) * ROWS * G.fnx; }
-template <typename RT, typename FT>
-static void launch_row_t(const Geom &G, const Scal &SC, int nsys, int mode, int iter, float *x, RT *r,
+// LDS of the row kernels: ROWS/2 complex rows twice (ping-pong) + twiddles.
+size_t row_lds(const Geom &G, int f64) { return (f64 ? sizeof(double2) : sizeof(float2)) * (ROWS + 1) * G.fnx; }
+template <typename RT, typename FT, int RM>
+static void launch_row_r(const Geom &G, const Scal &SC, int nsys, int mode, int iter, float *x, RT *r,
                          const RT *b, const float *p, const RT *Ap, FT *spec, int *stop, hipStream_t st) {
     dim3 grid(G.nrowblk, 1, nsys);
     const size_t lds = row_lds(G, sizeof(FT) == sizeof(double2));
     switch (mode) {
         case ROW_INIT_COLD:
-            hipLaunchKernelGGL((k_row<ROW_INIT_COLD, RT, FT>), grid, dim3(256), lds, st, G, SC, iter, x, r, b, p, Ap, spec, stop);
+            hipLaunchKernelGGL((k_row<ROW_INIT_COLD, RT, FT, RM>), grid, dim3(256), lds, st, G, SC, iter, x, r, b, p,
+                               Ap, spec, stop);
             break;
         case ROW_INIT_WARM:
-            hipLaunchKernelGGL((k_row<ROW_INIT_WARM, RT, FT>), grid, dim3(256), lds, st, G, SC, iter, x, r, b, p, Ap, spec, stop);
+            hipLaunchKernelGGL((k_row<ROW_INIT_WARM, RT, FT, RM>), grid, dim3(256), lds, st, G, SC, iter, x, r, b, p,
+                               Ap, spec, stop);
             break;
         case ROW_UPDATE:
-            hipLaunchKernelGGL((k_row<ROW_UPDATE, RT, FT>), grid, dim3(256), lds, st, G, SC, iter, x, r, b, p, Ap, spec, stop);
+            hipLaunchKernelGGL((k_row<ROW_UPDATE, RT, FT, RM>), grid, dim3(256), lds, st, G, SC, iter, x, r, b, p, Ap,
+                               spec, stop);
             break;
         default:
-            hipLaunchKernelGGL((k_row<ROW_LAST, RT, FT>), grid, dim3(256), lds, st, G, SC, iter, x, r, b, p, Ap, spec, stop);
+            hipLaunchKernelGGL((k_row<ROW_LAST, RT, FT, RM>), grid, dim3(256), lds, st, G, SC, iter, x, r, b, p, Ap,
+                               spec, stop);
             break;
     }
 }
-void launch_row(const Geom &G, const Scal &SC, int nsys, int mode, int rt64, int iter, float *x, void *r,
-                const void *b, const float *p, const void *Ap, void *spec, int *stop, hipStream_t st) {
-    if (rt64)
+template <typename RT, typename FT>
+static void launch_row_t(const Geom &G, const Scal &SC, int nsys, int mode, int iter, float *x, RT *r,
+                         const RT *b, const float *p, const RT *Ap, FT *spec, int *stop, hipStream_t st) {
+    if (G.rmax > 4) launch_row_r<RT, FT, 7>(G, SC, nsys, mode, iter, x, r, b, p, Ap, spec, stop, st);
+    else launch_row_r<RT, FT, 3>(G, SC, nsys, mode, iter, x, r, b, p, Ap, spec, stop, st);
+}
+void launch_row(const Geom &G, const Scal &SC, int nsys, int mode, int rt64, int spec64, int iter, float *x,
+                void *r, const void *b, const float *p, const void *Ap, void *spec, int *stop, hipStream_t st) {
+    if (rt64 && spec64)
         launch_row_t<double, double2>(G, SC, nsys, mode, iter, x, (double *)r, (const double *)b, p,
                                       (const double *)Ap, (double2 *)spec, stop, st);
+    else if (rt64)
+        launch_row_t<double, float2>(G, SC, nsys, mode, iter, x, (double *)r, (const double *)b, p,
+                                     (const double *)Ap, (float2 *)spec, stop, st);
     else
         launch_row_t<float, float2>(G, SC, nsys, mode, iter, x, (float *)r, (const float *)b, p, (const float *)Ap,
                                     (float2 *)spec, stop, st);
 }
-size_t col_lds(const Geom &G, int f64) {
-    return ((f64 ? sizeof(double2) : sizeof(float2)) * 2 + sizeof(float2) * 2) * (size_t)G.maxcol * G.npsr * G.fny;
+// LDS of the column kernel: two buffers of cs systems' column groups.
+size_t col_lds(const Geom &G, int f64, int cs) {
+    return (f64 ? sizeof(double2) : sizeof(float2)) * (2 * (size_t)cs * G.maxcol * G.npsr * G.fny + G.fny) +
+           sizeof(float2) * G.fny;
+}
+// Systems per column workgroup: FP64 spectra 2 or 1, FP32 spectra 4 or 1;
+// the larger when it still gives >= 1.5 workgroups per CU and fits 64 KiB of
+// LDS.
+int col_systems(const Geom &G, int nsys, int f64) {
+    const int cs = f64 ? 2 : 4;
+    if ((long)G.ngroup * ((nsys + cs - 1) / cs) >= 384 && col_lds(G, f64, cs) <= 64 * 1024) return cs;
+    return 1;
+}
+template <typename FT, int CS, int RM>
+static void launch_col_cs(const Geom &G, const Scal &SC, int nsys, const FT *specf, float2 *speci, hipStream_t st) {
+    const dim3 grid(G.ngroup, 1, (nsys + CS - 1) / CS);
+    const size_t lds = col_lds(G, sizeof(FT) == sizeof(double2), CS);
+    switch (G.bs) {
+        case 2: hipLaunchKernelGGL((k_col<FT, CS, 2, RM>), grid, dim3(256), lds, st, G, SC, nsys, specf, speci); break;
+        case 3: hipLaunchKernelGGL((k_col<FT, CS, 3, RM>), grid, dim3(256), lds, st, G, SC, nsys, specf, speci); break;
+        case 4: hipLaunchKernelGGL((k_col<FT, CS, 4, RM>), grid, dim3(256), lds, st, G, SC, nsys, specf, speci); break;
+        case 6: hipLaunchKernelGGL((k_col<FT, CS, 6, RM>), grid, dim3(256), lds, st, G, SC, nsys, specf, speci); break;
+        case 8: hipLaunchKernelGGL((k_col<FT, CS, 8, RM>), grid, dim3(256), lds, st, G, SC, nsys, specf, speci); break;
+        default: hipLaunchKernelGGL((k_col<FT, CS, 0, RM>), grid, dim3(256), lds, st, G, SC, nsys, specf, speci); break;
+    }
+}
+template <typename FT, int CSMAX>
+static void launch_col_t(const Geom &G, const Scal &SC, int nsys, int cs, const FT *specf, float2 *speci,
+                         hipStream_t st) {
+    if (G.rmax > 4) {
+        if (cs > 1) launch_col_cs<FT, CSMAX, 7>(G, SC, nsys, specf, speci, st);
+        else launch_col_cs<FT, 1, 7>(G, SC, nsys, specf, speci, st);
+    } else {
+        if (cs > 1) launch_col_cs<FT, CSMAX, 3>(G, SC, nsys, specf, speci, st);
+        else launch_col_cs<FT, 1, 3>(G, SC, nsys, specf, speci, st);
+    }
 }
 void launch_col(const Geom &G, const Scal &SC, int nsys, int f64, const void *specf, float2 *speci,
                 hipStream_t st) {
-    if (f64)
-        hipLaunchKernelGGL(k_col<double2>, dim3(G.ngroup, 1, nsys), dim3(256), col_lds(G, 1), st, G, SC,
-                           (const double2 *)specf, speci);
-    else
-        hipLaunchKernelGGL(k_col<float2>, dim3(G.ngroup, 1, nsys), dim3(256), col_lds(G, 0), st, G, SC,
-                           (const float2 *)specf, speci);
+    const int cs = col_systems(G, nsys, f64);
+    if (f64) launch_col_t<double2, 2>(G, SC, nsys, cs, (const double2 *)specf, speci, st);
+    else launch_col_t<float2, 4>(G, SC, nsys, cs, (const float2 *)specf, speci, st);
 }
-void launch_irow(const Geom &G, const Scal &SC, int nsys, int mode, int slot, const float2 *spec, float *x,
-                 float *out, hipStream_t st) {
+template <int RM>
+static void launch_irow_r(const Geom &G, const Scal &SC, int nsys, int mode, int slot, const float2 *spec, float *x,
+                          float *out, hipStream_t st) {
     const dim3 grid(G.nrowblk, 1, nsys);
     const size_t lds = row_lds(G, 0);
     switch (mode) {
-        case IROW_Z: hipLaunchKernelGGL(k_irow<IROW_Z>, grid, dim3(256), lds, st, G, SC, slot, spec, x, out); break;
-        case IROW_FIRST:
-            hipLaunchKernelGGL(k_irow<IROW_FIRST>, grid, dim3(256), lds, st, G, SC, slot, spec, x, out);
+        case IROW_Z:
+            hipLaunchKernelGGL((k_irow<IROW_Z, RM>), grid, dim3(256), lds, st, G, SC, slot, spec, x, out);
             break;
-        default: hipLaunchKernelGGL(k_irow<IROW_UPDATE>, grid, dim3(256), lds, st, G, SC, slot, spec, x, out); break;
+        case IROW_FIRST:
+            hipLaunchKernelGGL((k_irow<IROW_FIRST, RM>), grid, dim3(256), lds, st, G, SC, slot, spec, x, out);
+            break;
+        default:
+            hipLaunchKernelGGL((k_irow<IROW_UPDATE, RM>), grid, dim3(256), lds, st, G, SC, slot, spec, x, out);
+            break;
     }
+}
+void launch_irow(const Geom &G, const Scal &SC, int nsys, int mode, int slot, const float2 *spec, float *x,
+                 float *out, hipStream_t st) {
+    if (G.rmax > 4) launch_irow_r<7>(G, SC, nsys, mode, slot, spec, x, out, st);
+    else launch_irow_r<3>(G, SC, nsys, mode, slot, spec, x, out, st);
 }
 void launch_final(const Geom &G, const Scal &SC, int nsys, int npart_rr, const int *stop, hipStream_t st) {
     hipLaunchKernelGGL(k_final, dim3(1, 1, nsys), dim3(256), 0, st, G, SC, npart_rr, stop);

@@ -60,6 +60,12 @@ struct Geom {
     const float4 *neai;           // per WFS per subap (xx, yy, xy, 0)
     const float *ptt;             // per WFS 2 x 2nsa (column-major), ttr only
     long saoff[MAXW + 1];         // subap offset of WFS w in gpx/neai (units of subaps)
+    // the same GP / NEA laid out per ploc point (lower-left subaperture corner;
+    // zero where no subaperture), one copy per class of WFS with identical
+    // subapertures, GP and NEA (the WFS of one powfs)
+    int gcls[MAXW];               // class of WFS w
+    const float2 *gpd;            // [class][nploc][np*np] (x coef, y coef)
+    const float4 *nead;           // [class][nploc] (xx, yy, xy, 0)
     // FDPCG
     int precond, bs, scale;
     int zero_dc;                  // zero the DC bin of the residual spectrum (see k_col)
@@ -69,15 +75,16 @@ struct Geom {
     float fscale;                 // 1/sqrt(nx*ny) when scale
     int nstx, nsty;               // FFT stages
     int radx[MAXST], rady[MAXST];
+    int rmax;                     // largest radix of either axis
     const float2 *twx, *twy;      // exp(-2 pi i k/n)
     const double2 *twx64, *twy64; // same in FP64 (forward transform of the FP64 residual)
-    const float2 *Minv;           // nbhalf blocks bs*bs, column-major
+    const float2 *Minv;           // nbhalf blocks bs*bs, column-major, group order
     int ngroup;                   // column groups of the block apply
     const int *grp_col;           // per group: first column index into colkx
     const int *colkx;             // column kx list (per group, contiguous)
-    const int *grp_blk;           // per group: first block index into blkid (ngroup+1)
-    const int *blkid;             // block ids ordered by group
-    const int *blkpos;            // per block (nbhalf*bs): LDS position | (conj << 30) | (nowrite << 29)
+    const int *grp_blk;           // per group: first block (group order), ngroup+1 entries
+    const int *blkpos;            // per block in group order (nbhalf*bs):
+                                  //   LDS position | (conj << 30) | (nowrite << 29)
     int maxcol;                   // max columns in one group
     // row blocks of the row FFT kernels
     int nrowblk;

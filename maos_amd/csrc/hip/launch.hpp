@@ -14,9 +14,11 @@ int layer_tiles(const Geom &G);
 void launch_layer(const Geom &G, int nsys, bool lhs, int rt64, const void *q, const float *p, void *out,
                   double *part, int partstride, hipStream_t st);
 size_t row_lds(const Geom &G, int f64);
-void launch_row(const Geom &G, const Scal &SC, int nsys, int mode, int rt64, int iter, float *x, void *r,
-                const void *b, const float *p, const void *Ap, void *spec, int *stop, hipStream_t st);
-size_t col_lds(const Geom &G, int f64);
+// spec64: the forward row/column spectrum is double2 (requires rt64).
+void launch_row(const Geom &G, const Scal &SC, int nsys, int mode, int rt64, int spec64, int iter, float *x,
+                void *r, const void *b, const float *p, const void *Ap, void *spec, int *stop, hipStream_t st);
+size_t col_lds(const Geom &G, int f64, int cs);
+int col_systems(const Geom &G, int nsys, int f64);
 void launch_col(const Geom &G, const Scal &SC, int nsys, int f64, const void *specf, float2 *speci,
                 hipStream_t st);
 enum { IROW_MODE_Z = 0, IROW_MODE_FIRST = 1, IROW_MODE_UPDATE = 2 };

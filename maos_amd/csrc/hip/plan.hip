@@ -101,15 +101,18 @@ struct gpu_tomo {
     DevBuf<MRay> mray;
     DevBuf<int> saidx;
     DevBuf<float> gpx, gpy, ptt;
-    DevBuf<float4> neai;
+    DevBuf<float4> neai, nead;
+    DevBuf<float2> gpd;
     DevBuf<float2> twx, twy, Minv;
     DevBuf<double2> twx64, twy64;
-    DevBuf<int> grp_col, colkx, grp_blk, blkid, blkpos;
+    DevBuf<int> grp_col, colkx, grp_blk, blkpos;
+    std::vector<int> blk_order;  // half-spectrum block id of each block in group order
     DevBuf<int2> rowblk;
     // state
     DevBuf<float> x, z, p, g, tt;
     DevBuf<double> r, Ap, b, q;  // residual side: double, or float in the first half when !rt64
     int rt64 = 1;             // 1: r, Ap, b in FP64 (default); 0: FP32
+    int spec64 = 1;           // 1: forward FDPCG spectrum in FP64; 0: FP32
     DevBuf<double2> specf;  // forward spectrum of r (double2, or float2 in the first half when !rt64)
     DevBuf<float2> speci;   // block-multiplied spectrum, inverse transform input
     DevBuf<double> part_pap, part_rz, part_rr, part_bb, rz, alpha, res;
@@ -256,6 +259,15 @@ std::vector<float2> twiddles(int n) {
 // with b the block holding the conjugate frequencies (P: member matching),
 // reproduces the CPU apply exactly and makes duplicated half-spectrum writes
 // agree.
+std::vector<float2> half_blocks(const mt_recon_t *R, long nx0, long ny0);
+// Blocks of M (bs*bs each) permuted into the given order.
+std::vector<float2> group_order(const std::vector<float2> &M, const std::vector<int> &order, long bs) {
+    std::vector<float2> out(M.size());
+    for (size_t j = 0; j < order.size(); j++)
+        std::copy(M.begin() + (long)order[j] * bs * bs, M.begin() + ((long)order[j] + 1) * bs * bs,
+                  out.begin() + (long)j * bs * bs);
+    return out;
+}
 std::vector<float2> half_blocks(const mt_recon_t *R, long nx0, long ny0) {
     const long bs = R->fd_bs, nb = R->fd_nb, nbh = R->fd_nbhalf;
     const long nfull = nx0 * ny0;
@@ -505,6 +517,39 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
             for (long i = 0; i < 4 * nsa; i++) hptt[(size_t)G.saoff[k] * 4 + i] = (float)P[i];
         }
     }
+    {
+        // dense per-point copies, deduplicated over WFS
+        const size_t nl = (size_t)G.nploc;
+        std::vector<float2> dgp;
+        std::vector<float4> dne;
+        int ncls = 0;
+        for (int k = 0; k < nw; k++) {
+            std::vector<float2> cg(nl * NP * NP, make_float2(0.f, 0.f));
+            std::vector<float4> cn(nl, make_float4(0.f, 0.f, 0.f, 0.f));
+            for (size_t c = 0; c < nl; c++) {
+                const int sa = hsaidx[(size_t)k * nl + c];
+                if (sa < 0) continue;
+                const size_t so = (size_t)G.saoff[k] + sa;
+                for (int z = 0; z < NP * NP; z++) cg[c * NP * NP + z] = make_float2(hgpx[so * NP * NP + z], hgpy[so * NP * NP + z]);
+                cn[c] = hneai[so];
+            }
+            int found = -1;
+            for (int c = 0; c < ncls && found < 0; c++)
+                if (!std::memcmp(cg.data(), dgp.data() + (size_t)c * cg.size(), sizeof(float2) * cg.size()) &&
+                    !std::memcmp(cn.data(), dne.data() + (size_t)c * nl, sizeof(float4) * nl))
+                    found = c;
+            if (found < 0) {
+                found = ncls++;
+                dgp.insert(dgp.end(), cg.begin(), cg.end());
+                dne.insert(dne.end(), cn.begin(), cn.end());
+            }
+            G.gcls[k] = found;
+        }
+        g->gpd.upload(dgp);
+        g->nead.upload(dne);
+        G.gpd = g->gpd.p;
+        G.nead = g->nead.p;
+    }
     g->saidx.upload(hsaidx);
     g->gpx.upload(hgpx);
     g->gpy.upload(hgpy);
@@ -550,6 +595,9 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
         G.nsty = (int)ry.size();
         for (size_t i = 0; i < rx.size(); i++) G.radx[i] = rx[i];
         for (size_t i = 0; i < ry.size(); i++) G.rady[i] = ry[i];
+        G.rmax = 2;
+        for (int r : rx) G.rmax = std::max(G.rmax, r);
+        for (int r : ry) G.rmax = std::max(G.rmax, r);
         g->twx.upload(twiddles(nx0));
         g->twy.upload(twiddles(ny0));
         G.twx = g->twx.p;
@@ -638,17 +686,21 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
         }
         g->grp_col.upload(hgc);
         g->colkx.upload(hkx);
+        // blocks (positions and Minv) are stored in group order, so a column
+        // workgroup reads one contiguous range
+        g->blk_order = hbid;
+        std::vector<int> hbpos_g(nbh * bs);
+        for (long j = 0; j < nbh; j++)
+            for (long m = 0; m < bs; m++) hbpos_g[j * bs + m] = hbpos[(long)hbid[j] * bs + m];
         g->grp_blk.upload(hgb);
-        g->blkid.upload(hbid);
-        g->blkpos.upload(hbpos);
+        g->blkpos.upload(hbpos_g);
         G.grp_col = g->grp_col.p;
         G.colkx = g->colkx.p;
         G.grp_blk = g->grp_blk.p;
-        G.blkid = g->blkid.p;
         G.blkpos = g->blkpos.p;
-        g->Minv.upload(half_blocks(R, nx0, ny0));
+        g->Minv.upload(group_order(half_blocks(R, nx0, ny0), g->blk_order, bs));
         G.Minv = g->Minv.p;
-        if (col_lds(G, 1) > 64 * 1024) fail(MT_EINVAL, "FDPCG column group too large for LDS");
+        if (col_lds(G, 1, 1) > 64 * 1024) fail(MT_EINVAL, "FDPCG column group too large for LDS");
     }
     if (row_lds(G, 1) > 64 * 1024) fail(MT_EINVAL, "layer grid too large for the row FFT kernel");
 }
@@ -741,7 +793,7 @@ void enqueue_R(gpu_tomo *g, int nsys, const float *grad, void *bout, Rec &rec) {
 // the spectral r.z partials, then the inverse row pass in the given mode.
 void enqueue_P_tail(gpu_tomo *g, int nsys, int irow_mode, int slot, float *out, Rec &rec) {
     const Geom &G = g->G;
-    rec(KC_COL, [&] { launch_col(G, g->SC, nsys, g->rt64, g->specf.p, g->speci.p, rec.st); });
+    rec(KC_COL, [&] { launch_col(G, g->SC, nsys, g->spec64, g->specf.p, g->speci.p, rec.st); });
     rec(irow_mode == IROW_MODE_UPDATE ? KC_IROW : KC_IROW_START,
         [&] { launch_irow(G, g->SC, nsys, irow_mode, slot, g->speci.p, g->x.p, out, rec.st); });
 }
@@ -764,13 +816,13 @@ void enqueue_solve(gpu_tomo *g, int nsys, int warm, Rec &rec) {
     auto start = [&](int k, bool cold) {
         if (cold) {
             rec(KC_ROW_START, [&] {
-                launch_row(G, SC, nsys, ROW_MODE_INIT_COLD, g->rt64, k, g->x.p, g->r.p, g->b.p, g->p.p, g->Ap.p,
+                launch_row(G, SC, nsys, ROW_MODE_INIT_COLD, g->rt64, g->spec64, k, g->x.p, g->r.p, g->b.p, g->p.p, g->Ap.p,
                            g->specf.p, g->stop.p, rec.st);
             });
         } else {
             enqueue_L(g, nsys, g->x.p, g->Ap.p, nullptr, rec);
             rec(KC_ROW_START, [&] {
-                launch_row(G, SC, nsys, ROW_MODE_INIT_WARM, g->rt64, k, g->x.p, g->r.p, g->b.p, g->p.p, g->Ap.p,
+                launch_row(G, SC, nsys, ROW_MODE_INIT_WARM, g->rt64, g->spec64, k, g->x.p, g->r.p, g->b.p, g->p.p, g->Ap.p,
                            g->specf.p, g->stop.p, rec.st);
             });
         }
@@ -782,7 +834,7 @@ void enqueue_solve(gpu_tomo *g, int nsys, int warm, Rec &rec) {
         enqueue_L(g, nsys, g->p.p, g->Ap.p, SC.part_pap, rec);
         const bool last = k + 1 == maxit;
         rec(last ? KC_ROW_LAST : KC_ROW, [&] {
-            launch_row(G, SC, nsys, last ? ROW_MODE_LAST : ROW_MODE_UPDATE, g->rt64, k, g->x.p, g->r.p, g->b.p, g->p.p,
+            launch_row(G, SC, nsys, last ? ROW_MODE_LAST : ROW_MODE_UPDATE, g->rt64, g->spec64, k, g->x.p, g->r.p, g->b.p, g->p.p,
                        g->Ap.p, g->specf.p, g->stop.p, rec.st);
         });
         if (last) break;
@@ -920,7 +972,7 @@ int gpu_tomo_update_cn2(gpu_tomo_t *g, const mt_recon_t *recon) {
         if (g->G.precond) {
             const long nbh = recon->fd_nbhalf, bs = recon->fd_bs;
             if (bs != g->G.bs || (long)g->Minv.n != nbh * bs * bs) fail(MT_EINVAL, "FDPCG layout changed");
-            std::vector<float2> hM = half_blocks(recon, g->G.fnx, g->G.fny);
+            std::vector<float2> hM = group_order(half_blocks(recon, g->G.fnx, g->G.fny), g->blk_order, bs);
             HIPCHK(hipMemcpy(g->Minv.p, hM.data(), sizeof(float2) * hM.size(), hipMemcpyHostToDevice));
         }
         // kernel arguments changed: drop the captured graph
@@ -1085,7 +1137,7 @@ int gpu_tomo_P(gpu_tomo_t *g, const double *xin, double *xout) {
         to_dev_rt(g, g->b.p, xin);
         Rec rec{g, g->stream, false};
         launch_zero_i(g->stop.p, 1, g->stream);
-        launch_row(g->G, g->SC, 1, ROW_MODE_INIT_COLD, g->rt64, 0, g->p.p, g->r.p, g->b.p, g->p.p, g->Ap.p,
+        launch_row(g->G, g->SC, 1, ROW_MODE_INIT_COLD, g->rt64, g->spec64, 0, g->p.p, g->r.p, g->b.p, g->p.p, g->Ap.p,
                    g->specf.p, g->stop.p, g->stream);
         enqueue_P_tail(g, 1, IROW_MODE_Z, 0, g->z.p, rec);
         from_dev_x(g, xout, g->z.p);
@@ -1093,11 +1145,12 @@ int gpu_tomo_P(gpu_tomo_t *g, const double *xin, double *xout) {
 }
 
 int gpu_tomo_precision(gpu_tomo_t *g, int mode) {
-    if (!g || (mode != 0 && mode != 1)) return MT_EINVAL;
+    if (!g || mode < 0 || mode > 2) return MT_EINVAL;
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
         HIPCHK(hipStreamSynchronize(g->stream));
-        g->rt64 = mode;
+        g->rt64 = mode >= 1;
+        g->spec64 = mode == 1;
         if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
         if (g->graph) (void)hipGraphDestroy(g->graph);
         g->gexec = nullptr;

@@ -154,7 +154,8 @@ class GpuTomo:
         self.nx, self.ng = nx.value, ng.value
 
     def precision(self, mode):
-        """1: FP64 residual recurrence (default); 0: FP32 residual."""
+        """0: FP32 residual and spectrum; 1: FP64 residual recurrence and
+        forward spectrum (default); 2: FP64 residual, FP32 spectrum."""
         _abi.check(_abi.lib().gpu_tomo_precision(self._h, int(mode)), "gpu_tomo_precision")
 
     def close(self):

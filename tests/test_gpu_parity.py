@@ -114,3 +114,18 @@ def test_restart_past_100():
         g.close()
     assert err[104] <= 1e-3
     assert err[104] <= 1.5 * err[99] + 1e-5
+
+
+@pytest.mark.parametrize("mode,tol", [(0, 5e-3), (1, 1e-4), (2, 1e-4)])
+def test_precision_modes(large30, mode, tol):
+    """The 30 m solve in each precision mode against the FP64 oracle."""
+    cfg, r = large30
+    o = Oracle(r, nthread=8)
+    g = tomo.GpuTomo(r, maxit=cfg["maxit"])
+    g.precision(mode)
+    grad = workload.make_inputs(cfg, r, 1)
+    xg, resg, rc = g.tomo(grad)
+    xo, reso = o.solve(grad, cfg["maxit"])
+    e = rel(xg, xo)
+    print("precision mode %d: rel L2 %.3e residual gpu %.5e oracle %.5e" % (mode, e, resg, reso))
+    assert e <= tol
