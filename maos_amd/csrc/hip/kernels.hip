@@ -122,19 +122,20 @@ __device__ __forceinline__ void small_dft(C *v) {
     }
 }
 
+__device__ __forceinline__ unsigned fdiv(unsigned t, FDiv d) { return d.m ? __umulhi(t, d.m) : t; }
+
 template <int R, int SIGN, typename C>
-__device__ void fft_stage(const C *in, C *out, int n, int nf, int Ns, const C *tw) {
-    const int nb = n / R;
+__device__ void fft_stage(const C *in, C *out, int n, int nf, const FStage &st, const C *tw) {
+    const int nb = st.nb, Ns = st.Ns, tstep = st.tstep;
     const int tot = nf * nb;
-    const int tstep = n / (Ns * R);
     for (int t = threadIdx.x; t < tot; t += blockDim.x) {
-        const int f = t / nb, j = t - f * nb;
+        const int f = fdiv(t, st.dnb), j = t - f * nb;
         const C *src = in + f * n;
         C *dst = out + f * n;
         C v[R];
 #pragma unroll
         for (int r = 0; r < R; r++) v[r] = src[j + r * nb];
-        const int jm = j % Ns;
+        const int jm = j - fdiv(j, st.dNs) * Ns;
         if (Ns > 1) {
 #pragma unroll
             for (int r = 1; r < R; r++) {
@@ -144,7 +145,7 @@ __device__ void fft_stage(const C *in, C *out, int n, int nf, int Ns, const C *t
             }
         }
         small_dft<R, SIGN, C>(v);
-        const int idxD = (j / Ns) * Ns * R + jm;
+        const int idxD = (j - jm) * R + jm;
 #pragma unroll
         for (int r = 0; r < R; r++) dst[idxD + r * Ns] = v[r];
     }
@@ -154,20 +155,19 @@ __device__ void fft_stage(const C *in, C *out, int n, int nf, int Ns, const C *t
 // RM: largest radix compiled in (3: radices 2, 3, 4; 7: also 5 and 7, whose
 // butterflies need about twice the registers).
 template <int SIGN, typename C, int RM>
-__device__ C *fft_run(C *a, C *b, int n, int nf, int nst, const int *rad, const C *tw) {
-    int Ns = 1;
+__device__ C *fft_run(C *a, C *b, int n, int nf, int nst, const FStage *stages, const C *tw) {
     C *in = a, *out = b;
     for (int s = 0; s < nst; s++) {
-        const int R = rad[s];
-        switch (R) {
-            case 2: fft_stage<2, SIGN, C>(in, out, n, nf, Ns, tw); break;
-            case 3: fft_stage<3, SIGN, C>(in, out, n, nf, Ns, tw); break;
-            case 4: fft_stage<4, SIGN, C>(in, out, n, nf, Ns, tw); break;
+        const FStage &st = stages[s];
+        switch (st.R) {
+            case 2: fft_stage<2, SIGN, C>(in, out, n, nf, st, tw); break;
+            case 3: fft_stage<3, SIGN, C>(in, out, n, nf, st, tw); break;
+            case 4: fft_stage<4, SIGN, C>(in, out, n, nf, st, tw); break;
             case 5:
-                if constexpr (RM >= 5) fft_stage<5, SIGN, C>(in, out, n, nf, Ns, tw);
+                if constexpr (RM >= 5) fft_stage<5, SIGN, C>(in, out, n, nf, st, tw);
                 break;
             case 7:
-                if constexpr (RM >= 7) fft_stage<7, SIGN, C>(in, out, n, nf, Ns, tw);
+                if constexpr (RM >= 7) fft_stage<7, SIGN, C>(in, out, n, nf, st, tw);
                 break;
             default: break;
         }
@@ -175,7 +175,6 @@ __device__ C *fft_run(C *a, C *b, int n, int nf, int nst, const int *rad, const 
         C *t = in;
         in = out;
         out = t;
-        Ns *= R;
     }
     return in;
 }
@@ -407,27 +406,23 @@ __global__ __launch_bounds__(256) void k_layer(Geom G, const RT *q, const float 
 // mirror.  Layer windows (k_wfs_m) and WFS-field windows (k_layer_m) are
 // staged in LDS with coalesced loads.
 // ---------------------------------------------------------------------------
-template <int POS, typename RT>
-__global__ __launch_bounds__(256) void k_wfs_m(Geom G, const float *x, RT *q) {
+template <int POS, typename RT, int CS>
+__global__ __launch_bounds__(256) void k_wfs_m(Geom G, int nsys, const float *x, RT *q) {
     constexpr int H = TILE + 2 * POS;   // opd window edge
-    constexpr int H1 = H + 1;           // layer window edge
-    constexpr int H12 = H1 * H1;
     constexpr int S = TILE + POS;       // subaperture-corner window edge
     constexpr int S2 = S * S;
     constexpr int NP = POS + 1, NP2 = NP * NP;
-    constexpr int UG = (S2 * NP2 + 255) / 256, UN = (S2 + 255) / 256;
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    float2 *gps = reinterpret_cast<float2 *>(lds_raw);    // [S2][NP2] GP coefficients of the window
-    float *xs = reinterpret_cast<float *>(gps + S2 * NP2);  // [npsr][H12] layer windows
-    __shared__ RT opd[H * H];
-    __shared__ RT wgx[S2], wgy[S2];
-    const int w = blockIdx.y, s = blockIdx.z;
+    constexpr int UG = (S2 * NP2 + 255) / 256, UN = (S2 + 255) / 256, UO = (H * H + 255) / 256;
+    __shared__ float2 gps[S2 * NP2];  // GP coefficients of the window's corners
+    __shared__ RT opd[CS][H * H];
+    __shared__ RT wgx[CS][S2], wgy[CS][S2];
+    const int w = blockIdx.y, s0 = blockIdx.z * CS, ns = min(CS, nsys - s0);
     const int pnx = G.pnx, pny = G.pny;
     const int ntx = (pnx + TILE - 1) / TILE;
     const int tx0 = (blockIdx.x % ntx) * TILE, ty0 = (blockIdx.x / ntx) * TILE;
     const float2 *gpw = G.gpd + (long)G.gcls[w] * G.nploc * NP2;
     const float4 *nw4 = G.nead + (long)G.gcls[w] * G.nploc;
-    // geometry of the window (independent of the rays): issued first
+    // geometry of the window (shared by the CS systems)
     float2 gv[UG];
     float4 nv[UN];
 #pragma unroll
@@ -447,65 +442,69 @@ __global__ __launch_bounds__(256) void k_wfs_m(Geom G, const float *x, RT *q) {
         nv[u] = (i < S2 && cx >= 0 && cx < pnx && cy >= 0 && cy < pny) ? nw4[cy * pnx + cx]
                                                                        : make_float4(0.f, 0.f, 0.f, 0.f);
     }
-    // layer windows: window (a, b) of layer l holds node (tx0-POS+c+a, ty0-POS+c+b);
-    // four layers' loads in flight at a time (ray offsets are uniform: scalar)
-    constexpr int UH = (H12 + 255) / 256;
-    const float *xsys = x + (long)s * G.nxtot;
-    for (int l0 = 0; l0 < G.npsr; l0 += 4) {
-        float xv[4][UH];
-#pragma unroll
-        for (int dl = 0; dl < 4; dl++) {
-            const int l = l0 + dl;
-            if (l < G.npsr) {
-                const MRay m = G.mray[w * MAXL + l];
-                const int nxl = G.xnx[l], nyl = G.xny[l];
-                const float *xl = xsys + G.xoff[l];
-#pragma unroll
-                for (int u = 0; u < UH; u++) {
-                    const int j = threadIdx.x + 256 * u;
-                    int gx = tx0 - POS + m.cx + j % H1, gy = ty0 - POS + m.cy + j / H1;
-                    gx = (gx == nxl && m.wx) ? 0 : gx;  // edge ray: second node wraps
-                    gy = (gy == nyl && m.wy) ? 0 : gy;
-                    const bool ok = j < H12 && gx >= 0 && gx < nxl && gy >= 0 && gy < nyl;
-                    xv[dl][u] = ok ? xl[gy * nxl + gx] : 0.f;
-                }
-            }
-        }
-#pragma unroll
-        for (int dl = 0; dl < 4; dl++)
-            if (l0 + dl < G.npsr)
-#pragma unroll
-                for (int u = 0; u < UH; u++) {
-                    const int j = threadIdx.x + 256 * u;
-                    if (j < H12) xs[(l0 + dl) * H12 + j] = xv[dl][u];
-                }
-    }
 #pragma unroll
     for (int u = 0; u < UG; u++) {
         const int i = threadIdx.x + 256 * u;
         if (i < S2 * NP2) gps[i] = gv[u];
     }
-    __syncthreads();
-    // HX on the opd window
-    constexpr int UO = (H * H + 255) / 256;
+    // HX on the opd window, gathered straight from the layers (L1/L2 hits):
+    // ploc point i reads nodes i+c and i+c+1 (the latter wrapping to 0 at the
+    // last node, the reference's edge ray).  Per layer the window is either
+    // wholly inside the ray interval (plain loads) or not (indices clamped
+    // into the grid, points outside the interval contribute zero).  Index
+    // work is shared by the CS systems.
+    RT v[UO][CS];
+#pragma unroll
+    for (int u = 0; u < UO; u++)
+#pragma unroll
+        for (int si = 0; si < CS; si++) v[u][si] = 0;
+    for (int l = 0; l < G.npsr; l++) {
+        const MRay m = G.mray[w * MAXL + l];
+        const int nxl = G.xnx[l], nyl = G.xny[l];
+        const float *xl = x + (long)s0 * G.nxtot + G.xoff[l];
+        const RT fx = m.fx, fy = m.fy;
+        const bool fast = tx0 - POS >= m.ax0 && tx0 + TILE + POS <= (m.wx ? m.ax1 - 1 : m.ax1) &&
+                          ty0 - POS >= m.ay0 && ty0 + TILE + POS <= (m.wy ? m.ay1 - 1 : m.ay1);
+#pragma unroll
+        for (int u = 0; u < UO; u++) {
+            const int i = threadIdx.x + 256 * u;
+            if (i >= H * H) continue;
+            const int px = tx0 - POS + i % H, py = ty0 - POS + i / H;
+            if (fast) {
+                const int o = (py + m.cy) * nxl + (px + m.cx);
+#pragma unroll
+                for (int si = 0; si < CS; si++) {
+                    if (si < ns) {
+                        const float *t = xl + (long)si * G.nxtot + o;
+                        const RT a = t[0], b = t[1], c = t[nxl], d = t[nxl + 1];
+                        v[u][si] += (RT(1) - fy) * ((RT(1) - fx) * a + fx * b) + fy * ((RT(1) - fx) * c + fx * d);
+                    }
+                }
+            } else {
+                const bool ok = px >= m.ax0 && px < m.ax1 && py >= m.ay0 && py < m.ay1;
+                const int n0x = min(max(px + m.cx, 0), nxl - 1), n0y = min(max(py + m.cy, 0), nyl - 1);
+                const int n1x = n0x + 1 == nxl ? 0 : n0x + 1;
+                const int n1y = n0y + 1 == nyl ? 0 : n0y + 1;
+                const int o00 = n0y * nxl + n0x, o10 = n0y * nxl + n1x, o01 = n1y * nxl + n0x,
+                          o11 = n1y * nxl + n1x;
+#pragma unroll
+                for (int si = 0; si < CS; si++) {
+                    if (si < ns) {
+                        const float *t = xl + (long)si * G.nxtot;
+                        const RT a = t[o00], b = t[o10], c = t[o01], d = t[o11];
+                        const RT h = (RT(1) - fy) * ((RT(1) - fx) * a + fx * b) + fy * ((RT(1) - fx) * c + fx * d);
+                        v[u][si] += ok ? h : RT(0);
+                    }
+                }
+            }
+        }
+    }
 #pragma unroll
     for (int u = 0; u < UO; u++) {
         const int i = threadIdx.x + 256 * u;
-        const int a = i % H, b = i / H;
-        const int px = tx0 - POS + a, py = ty0 - POS + b;
-        if (i < H * H) {
-            RT v = 0;
-            for (int l = 0; l < G.npsr; l++) {
-                const MRay m = G.mray[w * MAXL + l];
-                const bool ok = px >= m.ax0 && px < m.ax1 && py >= m.ay0 && py < m.ay1;
-                const float *t = xs + l * H12 + b * H1 + a;
-                const RT fx = m.fx, fy = m.fy;
-                const RT h = (RT(1) - fy) * ((RT(1) - fx) * (RT)t[0] + fx * (RT)t[1]) +
-                             fy * ((RT(1) - fx) * (RT)t[H1] + fx * (RT)t[H1 + 1]);
-                v += ok ? h : RT(0);
-            }
-            opd[i] = v;
-        }
+#pragma unroll
+        for (int si = 0; si < CS; si++)
+            if (i < H * H && si < ns) opd[si][i] = v[u][si];
     }
     __syncthreads();
     // W GP per subaperture corner of the window (zero coefficients where none)
@@ -514,19 +513,26 @@ __global__ __launch_bounds__(256) void k_wfs_m(Geom G, const float *x, RT *q) {
         const int i = threadIdx.x + 256 * u;
         if (i < S2) {
             const int a = i % S, b = i / S;
-            RT gx = 0, gy = 0;
+            float2 c[NP2];
 #pragma unroll
-            for (int zy = 0; zy < NP; zy++)
-#pragma unroll
-                for (int zx = 0; zx < NP; zx++) {
-                    const float2 c = gps[i * NP2 + zy * NP + zx];
-                    const RT o = opd[(b + zy) * H + a + zx];
-                    gx += (RT)c.x * o;
-                    gy += (RT)c.y * o;
-                }
+            for (int z = 0; z < NP2; z++) c[z] = gps[i * NP2 + z];
             const float4 ne = nv[u];
-            wgx[i] = (RT)ne.x * gx + (RT)ne.z * gy;
-            wgy[i] = (RT)ne.z * gx + (RT)ne.y * gy;
+#pragma unroll
+            for (int si = 0; si < CS; si++) {
+                if (si < ns) {
+                    RT gx = 0, gy = 0;
+#pragma unroll
+                    for (int zy = 0; zy < NP; zy++)
+#pragma unroll
+                        for (int zx = 0; zx < NP; zx++) {
+                            const RT o = opd[si][(b + zy) * H + a + zx];
+                            gx += (RT)c[zy * NP + zx].x * o;
+                            gy += (RT)c[zy * NP + zx].y * o;
+                        }
+                    wgx[si][i] = (RT)ne.x * gx + (RT)ne.z * gy;
+                    wgy[si][i] = (RT)ne.z * gx + (RT)ne.y * gy;
+                }
+            }
         }
     }
     __syncthreads();
@@ -534,114 +540,158 @@ __global__ __launch_bounds__(256) void k_wfs_m(Geom G, const float *x, RT *q) {
     const int ox = threadIdx.x % TILE, oy = threadIdx.x / TILE;
     const int px = tx0 + ox, py = ty0 + oy;
     if (threadIdx.x < TILE * TILE && px < pnx && py < pny) {
-        RT acc = 0;
+        float2 c[NP2];
+        int sidx[NP2];
 #pragma unroll
         for (int zy = 0; zy < NP; zy++)
 #pragma unroll
             for (int zx = 0; zx < NP; zx++) {
                 const int si = (oy + POS - zy) * S + (ox + POS - zx);
-                const float2 c = gps[si * NP2 + zy * NP + zx];
-                acc += (RT)c.x * wgx[si] + (RT)c.y * wgy[si];
+                sidx[zy * NP + zx] = si;
+                c[zy * NP + zx] = gps[si * NP2 + zy * NP + zx];
             }
-        q[((long)s * G.nwfs + w) * G.nploc + py * pnx + px] = acc;
+#pragma unroll
+        for (int si = 0; si < CS; si++) {
+            if (si < ns) {
+                RT acc = 0;
+#pragma unroll
+                for (int z = 0; z < NP2; z++) acc += (RT)c[z].x * wgx[si][sidx[z]] + (RT)c[z].y * wgy[si][sidx[z]];
+                q[((long)(s0 + si) * G.nwfs + w) * G.nploc + py * pnx + px] = acc;
+            }
+        }
     }
 }
 
-template <bool LHS, typename RT>
-__global__ __launch_bounds__(256) void k_layer_m(Geom G, const RT *q, const float *p, RT *out,
+template <bool LHS, typename RT, int CS>
+__global__ __launch_bounds__(256) void k_layer_m(Geom G, int nsys, const RT *q, const float *p, RT *out,
                                                  double *part, int partstride) {
-    constexpr int Q = TILE + 1;      // WFS-field window edge
-    constexpr int Q2 = Q * Q;
     constexpr int PH = TILE + 4;     // p window edge (L2'L2 reach is 2)
     constexpr int PH2 = PH * PH;
-    constexpr int UQ = (Q2 + 255) / 256, UP = (PH2 + 255) / 256;
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    RT *qs = reinterpret_cast<RT *>(lds_raw);                       // [nwfs][Q*Q]
-    float *ps = reinterpret_cast<float *>(qs + G.nwfs * Q2);        // [PH*PH]
+    constexpr int UP = (PH2 + 255) / 256;
+    __shared__ float ps[CS][LHS ? PH2 : 1];
     __shared__ double sh[8];
-    const int l = blockIdx.y, s = blockIdx.z;
+    const int l = blockIdx.y, s0 = blockIdx.z * CS, ns = min(CS, nsys - s0);
     const int nxl = G.xnx[l], nyl = G.xny[l];
     const int ntx = (nxl + TILE - 1) / TILE, nty = (nyl + TILE - 1) / TILE;
     const bool active = (int)blockIdx.x < ntx * nty;
     const int jx0 = (blockIdx.x % ntx) * TILE, jy0 = (blockIdx.x / ntx) * TILE;
-    // p window and four WFS windows at a time are loaded with every load of
-    // the group in flight; ray offsets are uniform (scalar loads).
-    if (active) {
-        if (LHS) {
-            const float *pl = p + (long)s * G.nxtot + G.xoff[l];
-            float pv[UP];
+    const int jx = jx0 + (int)threadIdx.x % TILE, jy = jy0 + (int)threadIdx.x / TILE;
+    const bool mine = active && jx < nxl && jy < nyl;
+    if (LHS && active) {
+        // p windows, periodic (layers are >= TILE + 2, so one wrap suffices)
+        int off[UP];
+#pragma unroll
+        for (int u = 0; u < UP; u++) {
+            const int i = threadIdx.x + 256 * u;
+            int xx = jx0 - 2 + i % PH, yy = jy0 - 2 + i / PH;
+            xx += xx < 0 ? nxl : 0;
+            xx -= xx >= nxl ? nxl : 0;
+            yy += yy < 0 ? nyl : 0;
+            yy -= yy >= nyl ? nyl : 0;
+            off[u] = yy * nxl + xx;
+        }
+        float pv[CS][UP];
+#pragma unroll
+        for (int si = 0; si < CS; si++) {
+            const float *pl = p + (long)(s0 + si) * G.nxtot + G.xoff[l];
 #pragma unroll
             for (int u = 0; u < UP; u++) {
                 const int i = threadIdx.x + 256 * u;
-                // periodic; the last tile's window may run past a small layer
-                const int xx = (jx0 - 2 + i % PH + nxl) % nxl, yy = (jy0 - 2 + i / PH + nyl) % nyl;
-                pv[u] = i < PH2 ? pl[yy * nxl + xx] : 0.f;
+                pv[si][u] = (si < ns && i < PH2) ? pl[off[u]] : 0.f;
             }
+        }
+#pragma unroll
+        for (int si = 0; si < CS; si++)
 #pragma unroll
             for (int u = 0; u < UP; u++) {
                 const int i = threadIdx.x + 256 * u;
-                if (i < PH2) ps[i] = pv[u];
+                if (i < PH2) ps[si][i] = pv[si][u];
             }
-        }
-        for (int w0 = 0; w0 < G.nwfs; w0 += 4) {
-            RT qv[4][UQ];
-#pragma unroll
-            for (int dw = 0; dw < 4; dw++) {
-                const int w = w0 + dw;
-                if (w < G.nwfs) {
-                    const MRay m = G.mray[w * MAXL + l];
-                    const RT *qw = q + ((long)s * G.nwfs + w) * G.nploc;
-#pragma unroll
-                    for (int u = 0; u < UQ; u++) {
-                        const int j = threadIdx.x + 256 * u;
-                        int ix = jx0 - m.cx - 1 + j % Q, iy = jy0 - m.cy - 1 + j / Q;
-                        // node 0 is the wrapped second node of the edge ray a1 - 1;
-                        // slot i = -c - 1 (node 0's second-node ray) is otherwise empty
-                        ix = (m.wx && ix == -m.cx - 1) ? m.ax1 - 1 : ix;
-                        iy = (m.wy && iy == -m.cy - 1) ? m.ay1 - 1 : iy;
-                        const bool ok = j < Q2 && ix >= m.ax0 && ix < m.ax1 && iy >= m.ay0 && iy < m.ay1;
-                        qv[dw][u] = ok ? qw[iy * G.pnx + ix] : RT(0);
-                    }
-                }
-            }
-#pragma unroll
-            for (int dw = 0; dw < 4; dw++)
-                if (w0 + dw < G.nwfs)
-#pragma unroll
-                    for (int u = 0; u < UQ; u++) {
-                        const int j = threadIdx.x + 256 * u;
-                        if (j < Q2) qs[(w0 + dw) * Q2 + j] = qv[dw][u];
-                    }
-        }
     }
-    __syncthreads();
-    double dot = 0;
-    const int tx = threadIdx.x % TILE, ty = threadIdx.x / TILE;
-    const int jx = jx0 + tx, jy = jy0 + ty;
-    if (active && jx < nxl && jy < nyl) {
-        RT acc = 0;
+    // HX' gather straight from the WFS fields (L1/L2 hits): node j takes the
+    // ray whose second node it is (ploc index j-c-1, weight f) and the ray
+    // whose first node it is (j-c, weight 1-f) on each axis.  Per WFS the tile
+    // is classified uniformly: no ray lands on it (skip), every node's four
+    // rays are valid (plain loads), or mixed (indices clamped into the grid,
+    // rays outside the interval weighted zero).  Index work is shared by the
+    // CS systems.
+    RT acc[CS];
+#pragma unroll
+    for (int si = 0; si < CS; si++) acc[si] = 0;
+    const int tw = min(TILE, nxl - jx0), th = min(TILE, nyl - jy0);
+    if (mine) {
         for (int w = 0; w < G.nwfs; w++) {
             const MRay m = G.mray[w * MAXL + l];
+            const bool edgex = m.wx && jx0 == 0, edgey = m.wy && jy0 == 0;
+            if (!edgex && (jx0 + tw - 1 - m.cx < m.ax0 || jx0 - m.cx - 1 >= m.ax1)) continue;
+            if (!edgey && (jy0 + th - 1 - m.cy < m.ay0 || jy0 - m.cy - 1 >= m.ay1)) continue;
+            const RT *qw = q + ((long)s0 * G.nwfs + w) * G.nploc;
+            const long qstride = (long)G.nwfs * G.nploc;
             const RT fx = m.fx, fy = m.fy;
-            const RT *t = qs + w * Q2 + ty * Q + tx;
-            acc += fy * ((RT(1) - fx) * t[1] + fx * t[0]) + (RT(1) - fy) * ((RT(1) - fx) * t[Q + 1] + fx * t[Q]);
+            if (!edgex && !edgey && jx0 - m.cx - 1 >= m.ax0 && jx0 + tw - m.cx <= m.ax1 &&
+                jy0 - m.cy - 1 >= m.ay0 && jy0 + th - m.cy <= m.ay1) {
+                const int o = (jy - m.cy - 1) * G.pnx + (jx - m.cx - 1);
+#pragma unroll
+                for (int si = 0; si < CS; si++) {
+                    if (si < ns) {
+                        const RT *t = qw + si * qstride + o;
+                        const RT q00 = t[0], q10 = t[1], q01 = t[G.pnx], q11 = t[G.pnx + 1];
+                        acc[si] += fy * ((RT(1) - fx) * q10 + fx * q00) + (RT(1) - fy) * ((RT(1) - fx) * q11 + fx * q01);
+                    }
+                }
+                continue;
+            }
+            int ix0 = jx - m.cx - 1, iy0 = jy - m.cy - 1;
+            // node 0 is the wrapped second node of the edge ray a1 - 1
+            if (m.wx && jx == 0) ix0 = m.ax1 - 1;
+            if (m.wy && jy == 0) iy0 = m.ay1 - 1;
+            const int ix1 = jx - m.cx, iy1 = jy - m.cy;
+            const RT wx0 = (ix0 >= m.ax0 && ix0 < m.ax1) ? fx : RT(0);
+            const RT wx1 = (ix1 >= m.ax0 && ix1 < m.ax1) ? RT(1) - fx : RT(0);
+            const RT wy0 = (iy0 >= m.ay0 && iy0 < m.ay1) ? fy : RT(0);
+            const RT wy1 = (iy1 >= m.ay0 && iy1 < m.ay1) ? RT(1) - fy : RT(0);
+            const int cx0 = min(max(ix0, 0), G.pnx - 1), cx1 = min(max(ix1, 0), G.pnx - 1);
+            const int cy0 = min(max(iy0, 0), G.pny - 1), cy1 = min(max(iy1, 0), G.pny - 1);
+            const int o00 = cy0 * G.pnx + cx0, o10 = cy0 * G.pnx + cx1, o01 = cy1 * G.pnx + cx0,
+                      o11 = cy1 * G.pnx + cx1;
+#pragma unroll
+            for (int si = 0; si < CS; si++) {
+                if (si < ns) {
+                    const RT *t = qw + si * qstride;
+                    acc[si] += wy0 * (wx1 * t[o10] + wx0 * t[o00]) + wy1 * (wx1 * t[o11] + wx0 * t[o01]);
+                }
+            }
         }
-        const long o = (long)s * G.nxtot + G.xoff[l];
-        if (LHS) {
-            auto P = [&](int dx, int dy) { return (RT)ps[(ty + 2 + dy) * PH + tx + 2 + dx]; };
-            const RT c = P(0, 0);
-            const RT s4 = P(-1, 0) + P(1, 0) + P(0, -1) + P(0, 1);
-            const RT sd = P(-1, -1) + P(1, -1) + P(-1, 1) + P(1, 1);
-            const RT s2 = P(-2, 0) + P(2, 0) + P(0, -2) + P(0, 2);
-            acc += (RT)G.l2c[l] * (RT(20) * c - RT(8) * s4 + RT(2) * sd + s2);
-            dot = (double)c * (double)acc;
-        } else {
-            dot = (double)acc * (double)acc;
-        }
-        out[o + jy * nxl + jx] = acc;
     }
-    dot = block_sum(dot, sh);
-    if (threadIdx.x == 0 && part) part[(long)s * partstride + blockIdx.y * gridDim.x + blockIdx.x] = dot;
+    if (LHS) __syncthreads();
+    double dot[CS];
+#pragma unroll
+    for (int si = 0; si < CS; si++) {
+        dot[si] = 0;
+        if (mine && si < ns) {
+            RT a = acc[si];
+            if (LHS) {
+                const int tx = threadIdx.x % TILE, ty = threadIdx.x / TILE;
+                const float *P0 = ps[si] + (ty + 2) * PH + tx + 2;
+                auto P = [&](int dx, int dy) { return (RT)P0[dy * PH + dx]; };
+                const RT c = P(0, 0);
+                const RT s4 = P(-1, 0) + P(1, 0) + P(0, -1) + P(0, 1);
+                const RT sd = P(-1, -1) + P(1, -1) + P(-1, 1) + P(1, 1);
+                const RT s2 = P(-2, 0) + P(2, 0) + P(0, -2) + P(0, 2);
+                a += (RT)G.l2c[l] * (RT(20) * c - RT(8) * s4 + RT(2) * sd + s2);
+                dot[si] = (double)c * (double)a;
+            } else {
+                dot[si] = (double)a * (double)a;
+            }
+            out[(long)(s0 + si) * G.nxtot + G.xoff[l] + jy * nxl + jx] = a;
+        }
+    }
+#pragma unroll
+    for (int si = 0; si < CS; si++) {
+        const double d = block_sum(dot[si], sh);
+        if (threadIdx.x == 0 && part && si < ns)
+            part[(long)(s0 + si) * partstride + blockIdx.y * gridDim.x + blockIdx.x] = d;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -721,7 +771,7 @@ __global__ __launch_bounds__(256) void k_row(Geom G, Scal SC, int iter, float *x
         for (int u = 0; u < U; u++) {
             const int i = c0 + threadIdx.x + 256 * u;
             if (i < tot) {
-                const int row = i / n, col = i - row * n;
+                const int row = fdiv(i, G.dfnx), col = i - row * n;
                 const long idx = base + i;
                 RT rv;
                 if (MODE == ROW_INIT_COLD) {
@@ -746,13 +796,14 @@ __global__ __launch_bounds__(256) void k_row(Geom G, Scal SC, int iter, float *x
     }
     __syncthreads();
     FT *A = lds, *B = lds + ROWS / 2 * n;
-    FT *res = fft_run<-1, FT, RM>(A, B, n, nf, G.nstx, G.radx, twl);
+    FT *res = fft_run<-1, FT, RM>(A, B, n, nf, G.nstx, G.fsx, twl);
     const int nh = n / 2 + 1;
     FT *S = spec + (long)s * G.nspec + G.soff[l];
+    const FDiv dnf = G.dsmall[nf];
     for (int i = threadIdx.x; i < nf * nh; i += blockDim.x) {
-        const int f = i % nf, k = i / nf;
+        const int k = fdiv(i, dnf), f = i - k * nf;
         const FT Z = res[f * n + k];
-        const FT Zc = cconj(res[f * n + (n - k) % n]);
+        const FT Zc = cconj(res[f * n + (k == 0 ? 0 : n - k)]);
         const FT sum = cadd(Z, Zc), dif = csub(Z, Zc);
         S[(long)k * nyl + row0 + 2 * f] = CT<FT>::make(FR(0.5) * sum.x, FR(0.5) * sum.y);
         S[(long)k * nyl + row0 + 2 * f + 1] = CT<FT>::make(FR(0.5) * dif.y, FR(-0.5) * dif.x);
@@ -794,9 +845,10 @@ __global__ __launch_bounds__(256) void k_col(Geom G, Scal SC, int nsys, const FT
         for (int u = 0; u < U; u++) {
             const int i = b0 + threadIdx.x + 256 * u;
             if (i < tot) {
-                const int si = i / per, j = i - si * per;
-                const int f = j / ny, ky = j - f * ny;
-                const int ci = f / npsr, l = f - ci * npsr;
+                const int si = CS == 1 ? 0 : (i >= per) + (i >= 2 * per) + (i >= 3 * per);
+                const int j = i - si * per;
+                const int f = fdiv(j, G.dfny), ky = j - f * ny;
+                const int ci = fdiv(f, G.dnpsr), l = f - ci * npsr;
                 v[u] = specf[(long)(s0 + si) * G.nspec + G.soff[l] + (long)G.colkx[c0 + ci] * ny + ky];
             }
         }
@@ -807,7 +859,7 @@ __global__ __launch_bounds__(256) void k_col(Geom G, Scal SC, int nsys, const FT
         }
     }
     __syncthreads();
-    FT *res = fft_run<-1, FT, RM>(A, B, ny, ns * ncg * npsr, G.nsty, G.rady, twf);
+    FT *res = fft_run<-1, FT, RM>(A, B, ny, ns * ncg * npsr, G.nsty, G.fsy, twf);
     if (G.scale) {
         for (int i = threadIdx.x; i < tot; i += blockDim.x) {
             res[i].x *= (FR)G.fscale;
@@ -873,7 +925,7 @@ __global__ __launch_bounds__(256) void k_col(Geom G, Scal SC, int nsys, const FT
         if (!((pm >> 29) & 1)) {
             const int pos = pm & 0x1fffffff;
             const FR cj = ((pm >> 30) & 1) ? FR(-1) : FR(1);
-            const int kx = G.colkx[c0 + pos / ny / npsr];
+            const int kx = G.colkx[c0 + fdiv(fdiv(pos, G.dfny), G.dnpsr)];
             const double wgt = (kx == 0 || 2 * kx == G.fnx) ? 1.0 : 2.0;
 #pragma unroll
             for (int si = 0; si < CS; si++) {
@@ -891,11 +943,12 @@ __global__ __launch_bounds__(256) void k_col(Geom G, Scal SC, int nsys, const FT
         const double d = block_sum(dot[si], sh);  // its barriers also order C before the inverse FFT
         if (threadIdx.x == 0 && si < ns) SC.part_rz[(long)(s0 + si) * SC.maxpart + g] = d;
     }
-    float2 *res2 = fft_run<1, float2, RM>(C, D, ny, ns * ncg * npsr, G.nsty, G.rady, twi);
+    float2 *res2 = fft_run<1, float2, RM>(C, D, ny, ns * ncg * npsr, G.nsty, G.fsy, twi);
     for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-        const int si = i / per, j = i - si * per;
-        const int f = j / ny, ky = j - f * ny;
-        const int ci = f / npsr, l = f - ci * npsr;
+        const int si = CS == 1 ? 0 : (i >= per) + (i >= 2 * per) + (i >= 3 * per);
+        const int j = i - si * per;
+        const int f = fdiv(j, G.dfny), ky = j - f * ny;
+        const int ci = fdiv(f, G.dnpsr), l = f - ci * npsr;
         speci[(long)(s0 + si) * G.nspec + G.soff[l] + (long)G.colkx[c0 + ci] * ny + ky] = res2[i];
     }
 }
@@ -931,6 +984,7 @@ __global__ __launch_bounds__(256) void k_irow(Geom G, Scal SC, int slot, const f
     // Loads first (spectrum, and x, p for the update), then the r.z
     // reduction while they are in flight.
     const int tots = nf * n;
+    const FDiv dnf = G.dsmall[nf];
     constexpr int US = 4;
     float2 va[US], vb[US];
     auto load_spec = [&](int c0) {
@@ -938,7 +992,7 @@ __global__ __launch_bounds__(256) void k_irow(Geom G, Scal SC, int slot, const f
         for (int u = 0; u < US; u++) {
             const int i = c0 + threadIdx.x + 256 * u;
             if (i < tots) {
-                const int f = i % nf, k = i / nf;
+                const int k = fdiv(i, dnf), f = i - k * nf;
                 const int ya = row0 + 2 * f;
                 const long o = (long)(k <= n / 2 ? k : n - k) * nyl + ya;
                 va[u] = S[o];
@@ -951,7 +1005,7 @@ __global__ __launch_bounds__(256) void k_irow(Geom G, Scal SC, int slot, const f
         for (int u = 0; u < US; u++) {
             const int i = c0 + threadIdx.x + 256 * u;
             if (i < tots) {
-                const int f = i % nf, k = i / nf;
+                const int k = fdiv(i, dnf), f = i - k * nf;
                 float2 a = va[u], b = vb[u];
                 if (k > n / 2) {
                     a = cconj(a);
@@ -997,7 +1051,7 @@ __global__ __launch_bounds__(256) void k_irow(Geom G, Scal SC, int slot, const f
         store_spec(c0);
     }
     __syncthreads();
-    float2 *res = fft_run<1, float2, RM>(A, B, n, nf, G.nstx, G.radx, twl);
+    float2 *res = fft_run<1, float2, RM>(A, B, n, nf, G.nstx, G.fsx, twl);
     const float sc = G.scale ? G.fscale : 1.f;
     const float fa = (float)alpha, fb = (float)beta;
     for (int c0 = 0; c0 < tot; c0 += 256 * U) {
@@ -1006,7 +1060,7 @@ __global__ __launch_bounds__(256) void k_irow(Geom G, Scal SC, int slot, const f
         for (int u = 0; u < U; u++) {
             const int i = c0 + threadIdx.x + 256 * u;
             if (i < tot) {
-                const int row = i / n, col = i - row * n;
+                const int row = fdiv(i, G.dfnx), col = i - row * n;
                 const float2 v = res[(row >> 1) * n + col];
                 const float zv = ((row & 1) ? v.y : v.x) * sc;
                 if (MODE == IROW_UPDATE) {
@@ -1056,14 +1110,39 @@ static void launch_wfs_t(const Geom &G, int nsys, bool rhs, const float *x, cons
         else hipLaunchKernelGGL((k_wfs<2, false, RT>), grid, dim3(256), 0, st, G, x, grad, tt, q);
     }
 }
+// Systems per workgroup of the matched operator kernels: the most of 4, 2,
+// 1 that still gives >= 2 workgroups per CU (pos 2: at most 2, for LDS).
+static int op_systems(long wg_per_sys, int nsys, int maxcs) {
+    for (int cs = maxcs; cs > 1; cs >>= 1)
+        if (wg_per_sys * ((nsys + cs - 1) / cs) >= 512) return cs;
+    return 1;
+}
+template <typename RT, int POS>
+static void launch_wfs_m_p(const Geom &G, int nsys, const float *x, RT *q, hipStream_t st) {
+    const int nt = ((G.pnx + TILE - 1) / TILE) * ((G.pny + TILE - 1) / TILE);
+    const int cs = op_systems((long)nt * G.nwfs, nsys, POS == 1 ? 4 : 2);
+    dim3 grid(nt, G.nwfs, (nsys + cs - 1) / cs);
+    if (cs == 4) hipLaunchKernelGGL((k_wfs_m<POS, RT, POS == 1 ? 4 : 2>), grid, dim3(256), 0, st, G, nsys, x, q);
+    else if (cs == 2) hipLaunchKernelGGL((k_wfs_m<POS, RT, 2>), grid, dim3(256), 0, st, G, nsys, x, q);
+    else hipLaunchKernelGGL((k_wfs_m<POS, RT, 1>), grid, dim3(256), 0, st, G, nsys, x, q);
+}
 template <typename RT>
 static void launch_wfs_m(const Geom &G, int nsys, const float *x, RT *q, hipStream_t st) {
-    const int nt = ((G.pnx + TILE - 1) / TILE) * ((G.pny + TILE - 1) / TILE);
-    dim3 grid(nt, G.nwfs, nsys);
-    const int H1 = TILE + 2 * G.pos + 1, S = TILE + G.pos, NP = G.pos + 1;
-    const size_t lds = sizeof(float2) * S * S * NP * NP + sizeof(float) * G.npsr * H1 * H1;
-    if (G.pos == 1) hipLaunchKernelGGL((k_wfs_m<1, RT>), grid, dim3(256), lds, st, G, x, q);
-    else hipLaunchKernelGGL((k_wfs_m<2, RT>), grid, dim3(256), lds, st, G, x, q);
+    if (G.pos == 1) launch_wfs_m_p<RT, 1>(G, nsys, x, q, st);
+    else launch_wfs_m_p<RT, 2>(G, nsys, x, q, st);
+}
+template <bool LHS, typename RT>
+static void launch_layer_m(const Geom &G, int nsys, const RT *q, const float *p, RT *out, double *part,
+                           int partstride, hipStream_t st) {
+    const int nt = layer_tiles(G);
+    const int cs = op_systems((long)nt * G.npsr, nsys, 4);
+    dim3 grid(nt, G.npsr, (nsys + cs - 1) / cs);
+    if (cs == 4)
+        hipLaunchKernelGGL((k_layer_m<LHS, RT, 4>), grid, dim3(256), 0, st, G, nsys, q, p, out, part, partstride);
+    else if (cs == 2)
+        hipLaunchKernelGGL((k_layer_m<LHS, RT, 2>), grid, dim3(256), 0, st, G, nsys, q, p, out, part, partstride);
+    else
+        hipLaunchKernelGGL((k_layer_m<LHS, RT, 1>), grid, dim3(256), 0, st, G, nsys, q, p, out, part, partstride);
 }
 void launch_wfs(const Geom &G, int nsys, bool rhs, int rt64, const float *x, const float *grad,
                 const float *tt, void *q, hipStream_t st) {
@@ -1087,19 +1166,12 @@ void launch_layer(const Geom &G, int nsys, bool lhs, int rt64, const void *q, co
                   double *part, int partstride, hipStream_t st) {
     dim3 grid(layer_tiles(G), G.npsr, nsys);
     if (G.matched) {
-        const size_t Q2 = (TILE + 1) * (TILE + 1), P2 = (TILE + 4) * (TILE + 4);
         if (rt64) {
-            const size_t lds = sizeof(double) * G.nwfs * Q2 + sizeof(float) * P2;
-            double *o = (double *)out;
-            const double *qq = (const double *)q;
-            if (lhs) hipLaunchKernelGGL((k_layer_m<true, double>), grid, dim3(256), lds, st, G, qq, p, o, part, partstride);
-            else hipLaunchKernelGGL((k_layer_m<false, double>), grid, dim3(256), lds, st, G, qq, p, o, part, partstride);
+            if (lhs) launch_layer_m<true, double>(G, nsys, (const double *)q, p, (double *)out, part, partstride, st);
+            else launch_layer_m<false, double>(G, nsys, (const double *)q, p, (double *)out, part, partstride, st);
         } else {
-            const size_t lds = sizeof(float) * G.nwfs * Q2 + sizeof(float) * P2;
-            float *o = (float *)out;
-            const float *qq = (const float *)q;
-            if (lhs) hipLaunchKernelGGL((k_layer_m<true, float>), grid, dim3(256), lds, st, G, qq, p, o, part, partstride);
-            else hipLaunchKernelGGL((k_layer_m<false, float>), grid, dim3(256), lds, st, G, qq, p, o, part, partstride);
+            if (lhs) launch_layer_m<true, float>(G, nsys, (const float *)q, p, (float *)out, part, partstride, st);
+            else launch_layer_m<false, float>(G, nsys, (const float *)q, p, (float *)out, part, partstride, st);
         }
         return;
     }

@@ -22,9 +22,10 @@ namespace mtk {
 
 constexpr int MAXL = 16;      // layers
 constexpr int MAXW = 16;      // wavefront sensors (active)
-constexpr int MAXST = 16;     // FFT stages
+constexpr int MAXST = 8;      // FFT stages
 constexpr int TILE = 16;      // output tile edge of the WFS / layer passes
 constexpr int TAPS = 4;       // max ploc samples landing on one layer node per axis
+constexpr int ROWS = 16;      // rows per row-FFT workgroup (8 complex FFTs)
 
 // Matched-sampling ray of one (WFS, layer) pair: ploc index i maps to layer
 // nodes i + c and i + c + 1 with a constant fraction f, for i in [a0, a1)
@@ -33,6 +34,18 @@ struct MRay {
     int cx, cy, ax0, ax1, ay0, ay1;
     int wx, wy;
     float fx, fy;
+};
+
+// Division by a runtime constant d >= 1 as a multiply-high: m = ceil(2^32/d)
+// (0 for d = 1); exact for dividends below 2^16.
+struct FDiv {
+    unsigned d, m;
+};
+// One Stockham stage of a length-n FFT: radix R, Ns = product of the
+// previous radices, nb = n/R butterflies per transform, tstep = n/(Ns*R).
+struct FStage {
+    int R, Ns, nb, tstep;
+    FDiv dnb, dNs;
 };
 
 // Geometry shared by all systems of a plan (device copy).
@@ -74,8 +87,10 @@ struct Geom {
     long soff[MAXL + 1];
     float fscale;                 // 1/sqrt(nx*ny) when scale
     int nstx, nsty;               // FFT stages
-    int radx[MAXST], rady[MAXST];
+    FStage fsx[MAXST], fsy[MAXST];
     int rmax;                     // largest radix of either axis
+    FDiv dfnx, dfny, dnpsr;       // division by fnx, fny, npsr
+    FDiv dsmall[ROWS / 2 + 1];    // division by 1 .. ROWS/2 (complex rows per block)
     const float2 *twx, *twy;      // exp(-2 pi i k/n)
     const double2 *twx64, *twy64; // same in FP64 (forward transform of the FP64 residual)
     const float2 *Minv;           // nbhalf blocks bs*bs, column-major, group order
@@ -91,7 +106,6 @@ struct Geom {
     const int2 *rowblk;           // (layer, first row), ROWS rows each
 };
 
-constexpr int ROWS = 16;          // rows per row-FFT workgroup (8 complex FFTs)
 
 // Per-batch scalar state (device, FP64).
 struct Scal {

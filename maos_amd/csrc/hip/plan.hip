@@ -212,6 +212,27 @@ void ray_axes(double mdx, double mox, double moy, long nxin, long nyin, long pnx
     }
 }
 
+FDiv make_fdiv(unsigned d) {
+    FDiv f;
+    f.d = d;
+    f.m = d <= 1 ? 0u : (unsigned)((((unsigned long long)1 << 32) + d - 1) / d);
+    return f;
+}
+// Per-stage constants of the Stockham FFT of length n with radices rad.
+void fft_stages(const std::vector<int> &rad, int n, FStage *out) {
+    int Ns = 1;
+    for (size_t i = 0; i < rad.size(); i++) {
+        FStage &st = out[i];
+        st.R = rad[i];
+        st.Ns = Ns;
+        st.nb = n / st.R;
+        st.tstep = n / (Ns * st.R);
+        st.dnb = make_fdiv((unsigned)st.nb);
+        st.dNs = make_fdiv((unsigned)Ns);
+        Ns *= st.R;
+    }
+}
+
 // Radix plan for the Stockham FFT.
 std::vector<int> fft_radices(int n) {
     std::vector<int> r;
@@ -454,6 +475,9 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
                 !simple_axis(hny, hfy, wl * pny, pny, R->xny[l], m.cy, m.fy, m.ay0, m.ay1, m.wy))
                 matched = 0;
         }
+    // the matched kernels wrap the L2 window once: layers of at least TILE + 2
+    for (int l = 0; l < R->npsr; l++)
+        if (R->xnx[l] < TILE + 2 || R->xny[l] < TILE + 2) matched = 0;
     G.matched = matched;
     g->mray.upload(hm);
     G.mray = g->mray.p;
@@ -593,8 +617,12 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
         auto rx = fft_radices(nx0), ry = fft_radices(ny0);
         G.nstx = (int)rx.size();
         G.nsty = (int)ry.size();
-        for (size_t i = 0; i < rx.size(); i++) G.radx[i] = rx[i];
-        for (size_t i = 0; i < ry.size(); i++) G.rady[i] = ry[i];
+        fft_stages(rx, nx0, G.fsx);
+        fft_stages(ry, ny0, G.fsy);
+        G.dfnx = make_fdiv((unsigned)nx0);
+        G.dfny = make_fdiv((unsigned)ny0);
+        G.dnpsr = make_fdiv((unsigned)R->npsr);
+        for (int d = 0; d <= ROWS / 2; d++) G.dsmall[d] = make_fdiv((unsigned)std::max(d, 1));
         G.rmax = 2;
         for (int r : rx) G.rmax = std::max(G.rmax, r);
         for (int r : ry) G.rmax = std::max(G.rmax, r);
