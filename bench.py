@@ -85,6 +85,151 @@ def cpu_baseline(r, grads, maxit, seconds):
                        % (n, el, 1e3 * np.percentile(lat, 50), 1e3 * np.percentile(lat, 99))), x0
 
 
+def bench_uneven(steps, check=3):
+    """The uneven batch: 64 systems with D in {8,10,20,30} m, 1-6 LGS, 2-6
+    layers (seeded), 30 iterations each.  Identical configurations share a
+    plan; one group launch per kernel per iteration covers all 64 (hipGraph).
+    Compared with the same 64 systems solved one at a time (plain launches)."""
+    import collections
+    from maos_amd import configs, tomo, workload
+    cfgs = configs.batch64()
+    groups = collections.OrderedDict()
+    for i, c in enumerate(cfgs):
+        groups.setdefault(c["name"], []).append(i)
+    plans, members, slots = {}, [], {}
+    for name, idx in groups.items():
+        cfg = cfgs[idx[0]]
+        r = tomo.Recon(cfg)
+        g = tomo.GpuTomo(r, nbatch=len(idx), maxit=cfg["maxit"])
+        plans[name] = (cfg, r, g)
+        members.append((g, 0, len(idx)))
+        for s, i in enumerate(idx):
+            slots[i] = (name, s)
+    grads = {}
+    for i, c in enumerate(cfgs):
+        name, s = slots[i]
+        cfg, r, g = plans[name]
+        grads[i] = workload.make_inputs(cfg, r, 1000 + i)
+        g.set_grad(s, grads[i])
+    grp = tomo.GpuGroup(members)
+    for _ in range(2):
+        grp.solve(mode=1)
+    lat = []
+    for _ in range(steps):
+        t = time.perf_counter()
+        grp.solve(mode=1)
+        lat.append(time.perf_counter() - t)
+    lat = np.array(lat)
+    # one system at a time, plain launches
+    singles = [tomo.GpuGroup([(plans[slots[i][0]][2], slots[i][1], 1)]) for i in range(len(cfgs))]
+    for sg in singles:
+        sg.solve(mode=0)
+    t = time.perf_counter()
+    for sg in singles:
+        sg.solve(mode=0)
+    seq = time.perf_counter() - t
+    # accuracy of a few members (after the group solve above wrote x)
+    grp.solve(mode=1)
+    errs = []
+    from oracle.oracle import Oracle
+    for i in range(check):
+        name, s = slots[i]
+        cfg, r, g = plans[name]
+        x, res = g.get_x(s)
+        xo, _ = Oracle(r, nthread=8).solve(grads[i], cfg["maxit"])
+        errs.append(float(np.linalg.norm(x - xo) / np.linalg.norm(xo)))
+    for sg in singles:
+        sg.close()
+    grp.close()
+    n = len(cfgs)
+    return dict(workload="64 uneven systems: D in {8,10,20,30} m, 1-6 LGS, 2-6 layers, 30 iterations",
+                plans=len(plans), solves_per_s=n * len(lat) / lat.sum(),
+                p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
+                sequential_plain_solves_per_s=n / seq, speedup=float((n * len(lat) / lat.sum()) / (n / seq)),
+                rel_l2_vs_oracle_first=errs)
+
+
+def served_loops():
+    """16 AO loops mixed from the three sizes: 4 x the 8 m NGS system, 2 x the
+    30 m 6-LGS system, 10 x systems of the uneven batch."""
+    from maos_amd import configs
+    cf = [configs.small8() for _ in range(4)] + [configs.large30() for _ in range(2)]
+    cf += configs.batch64()[:10]
+    return cf
+
+
+def bench_served(nsteps=40, cycles=3, rate_hz=800.0, seed=11):
+    """Served stream: 16 loops at rate_hz with Poisson-jittered arrivals,
+    time-correlated (frozen-flow, 10 m/s) gradients, warm start and 10
+    iterations per step, through the request loop (continuous batching over
+    several streams, graphs cached per loop set).  The comparator is the same
+    arrival schedule through one-at-a-time plain solves (one request per
+    launch sequence, one stream, no graph).  Latency = submit to completion."""
+    from maos_amd import tomo, workload
+    from oracle.oracle import Oracle
+    cfgs = served_loops()
+    recons, streams = {}, []
+    for i, c in enumerate(cfgs):
+        if c["name"] not in recons:
+            recons[c["name"]] = tomo.Recon(c)
+        r = recons[c["name"]]
+        streams.append(workload.frozen_flow_stream(c, r, seed + i, nsteps))
+    T = 1.0 / rate_hz
+    rng = np.random.default_rng(seed)
+    events = []
+    for i in range(len(cfgs)):
+        ph = rng.uniform(0, T)
+        for k in range(nsteps * cycles):
+            events.append((k * T + ph + min(rng.exponential(0.1 * T), 0.5 * T), i, k))
+    events.sort()
+
+    def run(max_inflight, max_batch, graph):
+        srv = tomo.Server(mem_budget_gb=32.0, max_inflight=max_inflight, max_batch=max_batch, use_graph=graph)
+        ids = [srv.add_loop(recons[c["name"]], maxit=10) for c in cfgs]
+        # warm-up: one step per loop (graph capture, first touch)
+        for i, lid in enumerate(ids):
+            srv.submit(lid, streams[i][0], tag=-1)
+        srv.drain()
+        srv.poll(4096)
+        done = []
+        t0 = time.perf_counter()
+        for t, i, k in events:
+            while time.perf_counter() - t0 < t:
+                pass
+            srv.submit(ids[i], streams[i][k % nsteps], tag=k)
+        srv.drain()
+        t1 = time.perf_counter()
+        while True:
+            d = srv.poll(4096)
+            if not d:
+                break
+            done.extend(d)
+        lat = np.array([d[3] for d in done]) * 1e-3
+        out = dict(requests=len(done), solves_per_s=len(done) / (t1 - t0), p50_ms=float(np.percentile(lat, 50)),
+                   p99_ms=float(np.percentile(lat, 99)), max_ms=float(lat.max()), wall_s=t1 - t0)
+        return srv, ids, out
+
+    srv, ids, served = run(3, 64, True)
+    # per-step error of the warm 10-iteration solution against a cold
+    # 30-iteration FP64 oracle solve of the same step (three loops, next steps)
+    errs = []
+    for i in (0, 4, 6):
+        r = recons[cfgs[i]["name"]]
+        g = streams[i][(nsteps * cycles) % nsteps]
+        srv.submit(ids[i], g, tag=-2)
+        srv.drain()
+        x, _ = srv.get_x(ids[i])
+        xo, _ = Oracle(r, nthread=8).solve(g, 30)
+        errs.append(float(np.linalg.norm(x - xo) / np.linalg.norm(xo)))
+    srv.close()
+    srv_p, _, plain = run(1, 1, False)
+    srv_p.close()
+    return dict(workload="16 loops at %g Hz (4 x 8 m NGS, 2 x 30 m 6-LGS, 10 x uneven-batch systems), "
+                         "frozen flow 10 m/s, Poisson-jittered arrivals, warm start, 10 iterations" % rate_hz,
+                offered_per_s=len(cfgs) * rate_hz, served=served, one_at_a_time_plain=plain,
+                rel_l2_warm10_vs_cold30_fp64=errs)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -95,7 +240,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", type=int, default=None,
                     help="solve precision mode (0, 1, 2; default: the library default)")
-    ap.add_argument("--modes", default="plain,graph,batched",
+    ap.add_argument("--modes", default="plain,graph,batched,uneven,served",
                     help="comma list of modes to time (profiling runs use one)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -174,6 +319,20 @@ def main():
                     frac=round(kd["GBs"] / HBM_PEAK_GBS, 4), traffic=None, kernel=dom,
                     avg_us=round(kd["avg_us"], 2), alg_bytes_per_launch=int(ab[dom][0] * B + ab[dom][1]))
 
+    uneven = None
+    if "uneven" in modes:
+        try:
+            uneven = bench_uneven(max(5, args.steps // 20))
+        except Exception as e:  # reported, not fatal for the headline line
+            uneven = {"error": repr(e)}
+
+    served = None
+    if "served" in modes:
+        try:
+            served = bench_served()
+        except Exception as e:  # reported, not fatal for the headline line
+            served = {"error": repr(e)}
+
     cpu = None
     err = None
     if not args.no_cpu:
@@ -204,6 +363,8 @@ def main():
         "roofline": roofline,
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "uneven64": uneven,
+        "served16": served,
     }
     print(json.dumps(out))
 

@@ -214,9 +214,10 @@ int  gpu_tomo_update_cn2(gpu_tomo_t *g, const mt_recon_t *recon);
 int  gpu_tomo_reset(gpu_tomo_t *g);
 void gpu_tomo_free(gpu_tomo_t *g);
 
-/* Full solve with host buffers (double, MAOS dcell layout), one system:
- * grad -> RHS -> PCG (maxit, warm start from the resident solution) -> x.
- * cgres receives sqrt(r'r / b'b).  Replaces gpu_tomo (cuda/recon.cu:564). */
+/* Full solve with host buffers (double, MAOS dcell layout), one system in
+ * slot isys of the plan: grad -> RHS -> PCG (maxit, warm start from the
+ * slot's resident solution) -> x.  cgres receives sqrt(r'r / b'b).
+ * Replaces gpu_tomo (cuda/recon.cu:564). */
 int  gpu_tomo_solve(gpu_tomo_t *g, int isys, const double *grad,
                     double *x, double *cgres);
 
@@ -258,6 +259,60 @@ int  gpu_tomo_P(gpu_tomo_t *g, const double *xin, double *xout);
  * A 30-iteration FDPCG solve of the 30 m system drifts ~2e-3 (relative L2)
  * from the FP64 solve with mode 0 and stays ~1e-5 with mode 1. */
 int  gpu_tomo_precision(gpu_tomo_t *g, int mode);
+
+/* Host gradients of system slot isys, and the solution / residual of slot
+ * isys after a solve (for use with groups). */
+int  gpu_tomo_set_grad(gpu_tomo_t *g, int isys, const double *grad);
+int  gpu_tomo_get_x(gpu_tomo_t *g, int isys, double *x, double *cgres);
+
+/* Uneven batch: system ranges [s0[i], s0[i]+nsys[i]) of n plans, of any
+ * geometry (aperture, WFS and layer counts), solved in lockstep with one
+ * launch per kernel per iteration covering all of them; every workgroup
+ * finds its plan and block in a table.  Plans must share the device,
+ * precision mode, maxit and warm flag.  Inputs are the plans' gradient
+ * slots (gpu_tomo_set_grad or gpu_tomo_solve_dev's buffers); results stay in
+ * the plans (gpu_tomo_get_x).  mode: 0 = launches, 1 = replay a captured
+ * hipGraph.  stream 0: the group's own stream, synchronised on return.
+ * No reference counterpart: MAOS solves one system per gpu_tomo call
+ * (cuda/recon.cu:564); this batches many independent loops. */
+typedef struct gpu_tomo_group gpu_tomo_group_t;
+int  gpu_tomo_group_create(int n, gpu_tomo_t *const *plans, const int *s0,
+                           const int *nsys, gpu_tomo_group_t **out);
+int  gpu_tomo_group_solve(gpu_tomo_group_t *grp, int mode, void *stream);
+void gpu_tomo_group_free(gpu_tomo_group_t *grp);
+
+/* Request loop with continuous batching.  Each AO loop (add_loop) owns a
+ * plan with one warm-start slot, admitted only while the plans' device
+ * memory stays within mem_budget_gb (else MT_EFULL; remove_loop frees it).
+ * submit() queues one time step's gradients of a loop and returns at once;
+ * a worker thread launches the pending requests of loops that are not on
+ * the GPU as one batch (one launch per kernel per iteration for all of
+ * them; use_graph: replayed from a hipGraph cached per set of loops) on one
+ * of max_inflight streams, at most max_batch requests per batch, so requests
+ * that arrive while others run are admitted as soon as a stream and their
+ * loop are free, and batches overlap on the device.  Each loop warm-starts
+ * from its own previous solution; all loops of a server use one maxit.
+ * poll() returns completed requests (tag,
+ * residual, submit-to-completion latency); get_x() the loop's latest
+ * solution.  No reference counterpart (MAOS solves one loop per process,
+ * cuda/recon.cu:564); this serves many loops on one GPU. */
+typedef struct gpu_tomo_server gpu_tomo_server_t;
+int  gpu_tomo_server_create(int device, double mem_budget_gb, int max_inflight,
+                            int max_batch, int use_graph,
+                            gpu_tomo_server_t **out);
+int  gpu_tomo_server_add_loop(gpu_tomo_server_t *s, const mt_recon_t *recon,
+                              int maxit, int *loop);
+int  gpu_tomo_server_remove_loop(gpu_tomo_server_t *s, int loop);
+int  gpu_tomo_server_submit(gpu_tomo_server_t *s, int loop, const double *grad,
+                            long tag);
+int  gpu_tomo_server_poll(gpu_tomo_server_t *s, int maxn, int *loop, long *tag,
+                          double *cgres, double *latency_us, int *n);
+int  gpu_tomo_server_drain(gpu_tomo_server_t *s);
+int  gpu_tomo_server_get_x(gpu_tomo_server_t *s, int loop, double *x,
+                           double *cgres);
+int  gpu_tomo_server_usage(gpu_tomo_server_t *s, double *used_gb,
+                           double *budget_gb);
+void gpu_tomo_server_free(gpu_tomo_server_t *s);
 
 /* Sizes of one system's vectors. */
 int  gpu_tomo_sizes(const gpu_tomo_t *g, long *nx, long *ng);

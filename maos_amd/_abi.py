@@ -102,6 +102,22 @@ def lib():
     L.gpu_tomo_free.restype = None
     L.gpu_tomo_solve.argtypes = [vp, C.c_int, dp, dp, dp]
     L.gpu_tomo_solve_batch.argtypes = [vp, C.c_int, dp, dp, dp]
+    L.gpu_tomo_set_grad.argtypes = [vp, C.c_int, dp]
+    L.gpu_tomo_get_x.argtypes = [vp, C.c_int, dp, dp]
+    L.gpu_tomo_group_create.argtypes = [C.c_int, C.POINTER(vp), ip, ip, C.POINTER(vp)]
+    L.gpu_tomo_group_solve.argtypes = [vp, C.c_int, vp]
+    L.gpu_tomo_group_free.argtypes = [vp]
+    L.gpu_tomo_group_free.restype = None
+    L.gpu_tomo_server_create.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
+    L.gpu_tomo_server_add_loop.argtypes = [vp, C.POINTER(MtRecon), C.c_int, ip]
+    L.gpu_tomo_server_remove_loop.argtypes = [vp, C.c_int]
+    L.gpu_tomo_server_submit.argtypes = [vp, C.c_int, dp, C.c_long]
+    L.gpu_tomo_server_poll.argtypes = [vp, C.c_int, ip, lp, dp, dp, ip]
+    L.gpu_tomo_server_drain.argtypes = [vp]
+    L.gpu_tomo_server_get_x.argtypes = [vp, C.c_int, dp, dp]
+    L.gpu_tomo_server_usage.argtypes = [vp, dp, dp]
+    L.gpu_tomo_server_free.argtypes = [vp]
+    L.gpu_tomo_server_free.restype = None
     L.gpu_tomo_solve_dev.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp]
     L.gpu_tomo_residuals.argtypes = [vp, dp, C.c_int]
     L.gpu_tomo_L.argtypes = [vp, dp, dp, C.c_double, C.c_double]
@@ -121,7 +137,11 @@ EXPORTS = [
     "mt_setup_free", "mt_setup_fdpcg_mhat", "mt_setup_sizes", "mt_fdpcg_prepare",
     "mt_fdpcg_free", "gpu_tomo_setup",
     "gpu_tomo_update_cn2", "gpu_tomo_reset", "gpu_tomo_free", "gpu_tomo_solve",
-    "gpu_tomo_solve_batch", "gpu_tomo_solve_dev", "gpu_tomo_residuals", "gpu_tomo_L", "gpu_tomo_R", "gpu_tomo_P",
+    "gpu_tomo_solve_batch", "gpu_tomo_solve_dev", "gpu_tomo_set_grad", "gpu_tomo_get_x",
+    "gpu_tomo_group_create", "gpu_tomo_group_solve", "gpu_tomo_group_free",
+    "gpu_tomo_server_create", "gpu_tomo_server_add_loop", "gpu_tomo_server_remove_loop",
+    "gpu_tomo_server_submit", "gpu_tomo_server_poll", "gpu_tomo_server_drain", "gpu_tomo_server_get_x",
+    "gpu_tomo_server_usage", "gpu_tomo_server_free", "gpu_tomo_residuals", "gpu_tomo_L", "gpu_tomo_R", "gpu_tomo_P",
     "gpu_tomo_sizes", "gpu_tomo_profile", "gpu_tomo_profile_read", "gpu_tomo_precision",
 ]
 

@@ -36,6 +36,72 @@ template <> struct CT<double2> {
     static __device__ __forceinline__ double2 make(double a, double b) { return make_double2(a, b); }
 };
 
+// Job and block of this workgroup (see Job / WGI in kernels.hpp).
+#define KJOB                                                   \
+    const WGI wi_ = wgi[blockIdx.x];                           \
+    const Job J = jobs[wi_.job];                               \
+    if (!J.on) return;                                         \
+    CPD &PD = *(const CONSTAS PlanDev *)J.pd;                  \
+    CG &G = PD.G;                                              \
+    const int bx = wi_.bx, by = wi_.by, bz = wi_.bz;           \
+    (void)bx;                                                  \
+    (void)by;                                                  \
+    (void)bz
+#define KARGS const Job *__restrict__ jobs, const WGI *__restrict__ wgi
+
+__device__ __forceinline__ float *vsel(CPD &P, int sel) { return sel == BUF_X ? P.x : sel == BUF_P ? P.p : P.z; }
+template <typename RT>
+__device__ __forceinline__ RT *rsel(CPD &P, int sel) {
+    return (RT *)(sel == BUF_AP ? P.Ap : P.b);
+}
+__device__ __forceinline__ double *psel(CPD &P, int sel) {
+    return sel == PART_PAP ? P.SC.part_pap : sel == PART_BB ? P.SC.part_bb : nullptr;
+}
+// Plain copies of small structs read through the constant address space.
+__device__ __forceinline__ MRay mray(CG &G, int i) {
+    const CONSTAS MRay &r = ((const CONSTAS MRay *)G.mray)[i];
+    MRay m;
+    m.cx = r.cx;
+    m.cy = r.cy;
+    m.ax0 = r.ax0;
+    m.ax1 = r.ax1;
+    m.ay0 = r.ay0;
+    m.ay1 = r.ay1;
+    m.wx = r.wx;
+    m.wy = r.wy;
+    m.fx = r.fx;
+    m.fy = r.fy;
+    return m;
+}
+__device__ __forceinline__ FDiv ldf(const CONSTAS FDiv &f) { return FDiv{f.d, f.m}; }
+__device__ __forceinline__ FStage ldst(const CONSTAS FStage &f) {
+    FStage st;
+    st.R = f.R;
+    st.Ns = f.Ns;
+    st.nb = f.nb;
+    st.tstep = f.tstep;
+    st.dnb = ldf(f.dnb);
+    st.dNs = ldf(f.dNs);
+    return st;
+}
+__device__ __forceinline__ Scal ldsc(const CONSTAS Scal &c) {
+    Scal r;
+    r.part_pap = c.part_pap;
+    r.part_rz = c.part_rz;
+    r.part_rr = c.part_rr;
+    r.part_bb = c.part_bb;
+    r.rz = c.rz;
+    r.alpha = c.alpha;
+    r.res = c.res;
+    r.tt = c.tt;
+    r.maxpart = c.maxpart;
+    r.maxit = c.maxit;
+    r.npart_layer = c.npart_layer;
+    r.npart_row = c.npart_row;
+    r.npart_col = c.npart_col;
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // Deterministic FP64 reductions (fixed order: per-lane strided sums, then a
 // 64-lane shuffle tree, then wavefront partials summed by lane 0 in order).
@@ -155,10 +221,10 @@ __device__ void fft_stage(const C *in, C *out, int n, int nf, const FStage &st, 
 // RM: largest radix compiled in (3: radices 2, 3, 4; 7: also 5 and 7, whose
 // butterflies need about twice the registers).
 template <int SIGN, typename C, int RM>
-__device__ C *fft_run(C *a, C *b, int n, int nf, int nst, const FStage *stages, const C *tw) {
+__device__ C *fft_run(C *a, C *b, int n, int nf, int nst, const CONSTAS FStage *stages, const C *tw) {
     C *in = a, *out = b;
     for (int s = 0; s < nst; s++) {
-        const FStage &st = stages[s];
+        const FStage st = ldst(stages[s]);
         switch (st.R) {
             case 2: fft_stage<2, SIGN, C>(in, out, n, nf, st, tw); break;
             case 3: fft_stage<3, SIGN, C>(in, out, n, nf, st, tw); break;
@@ -185,16 +251,19 @@ __device__ __forceinline__ void load_tw(C *dst, const C *src, int n) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) dst[i] = src[i];
 }
 
-template <typename C> __device__ __forceinline__ const C *tw_of(const Geom &G, bool y);
-template <> __device__ __forceinline__ const float2 *tw_of<float2>(const Geom &G, bool y) { return y ? G.twy : G.twx; }
-template <> __device__ __forceinline__ const double2 *tw_of<double2>(const Geom &G, bool y) { return y ? G.twy64 : G.twx64; }
+template <typename C> __device__ __forceinline__ const C *tw_of(CG &G, bool y);
+template <> __device__ __forceinline__ const float2 *tw_of<float2>(CG &G, bool y) { return y ? G.twy : G.twx; }
+template <> __device__ __forceinline__ const double2 *tw_of<double2>(CG &G, bool y) { return y ? G.twy64 : G.twx64; }
 
 // ---------------------------------------------------------------------------
 // TT projection of the RHS gradients: tt = PTT * g per (system, WFS).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_ttr(Geom G, const float *grad, float *tt) {
+__global__ __launch_bounds__(256) void k_ttr(KARGS) {
+    KJOB;
     __shared__ double sh[8];
-    const int w = blockIdx.x, s = blockIdx.z;
+    const int w = bx, s = J.s0 + bz;
+    const float *grad = PD.g;
+    float *tt = PD.tt;
     float *out = tt + ((long)s * G.nwfs + w) * 2;
     if (!G.ttr[w]) {
         if (threadIdx.x == 0) out[0] = out[1] = 0.f;
@@ -223,18 +292,20 @@ __global__ __launch_bounds__(256) void k_ttr(Geom G, const float *grad, float *t
 // point.  RHS: W (g - TT tt) then the same GP' gather.
 // ---------------------------------------------------------------------------
 template <int POS, bool RHS, typename RT>
-__global__ __launch_bounds__(256) void k_wfs(Geom G, const float *x, const float *grad,
-                                             const float *tt, RT *q) {
+__global__ __launch_bounds__(256) void k_wfs(KARGS, int xsel) {
+    KJOB;
+    const float *x = vsel(PD, xsel), *grad = PD.g, *tt = PD.tt;
+    RT *q = (RT *)PD.q;
     constexpr int H = TILE + 2 * POS;
     constexpr int S = TILE + POS;
     constexpr int NP = POS + 1;
     __shared__ RT opd[RHS ? 1 : H * H];
     __shared__ RT wgx[S * S], wgy[S * S];
     __shared__ int sidx[S * S];
-    const int w = blockIdx.y, s = blockIdx.z;
+    const int w = by, s = J.s0 + bz;
     const int pnx = G.pnx, pny = G.pny;
     const int ntx = (pnx + TILE - 1) / TILE;
-    const int tx0 = (blockIdx.x % ntx) * TILE, ty0 = (blockIdx.x / ntx) * TILE;
+    const int tx0 = (bx % ntx) * TILE, ty0 = (bx / ntx) * TILE;
     if (!RHS) {
         const float *xs = x + (long)s * G.nxtot;
         for (int i = threadIdx.x; i < H * H; i += blockDim.x) {
@@ -325,16 +396,21 @@ __global__ __launch_bounds__(256) void k_wfs(Geom G, const float *x, const float
 // covers it.  Also writes the FP64 partial of p.Ap (LHS) or out.out (RHS).
 // ---------------------------------------------------------------------------
 template <bool LHS, typename RT>
-__global__ __launch_bounds__(256) void k_layer(Geom G, const RT *q, const float *p, RT *out,
-                                               double *part, int partstride) {
+__global__ __launch_bounds__(256) void k_layer(KARGS, int psel_, int osel, int partsel) {
+    KJOB;
+    const RT *q = (const RT *)PD.q;
+    const float *p = vsel(PD, psel_);
+    RT *out = rsel<RT>(PD, osel);
+    double *part = psel(PD, partsel);
+    const int partstride = PD.SC.maxpart;
     __shared__ double sh[8];
-    const int l = blockIdx.y, s = blockIdx.z;
+    const int l = by, s = J.s0 + bz;
     const int nxl = G.xnx[l], nyl = G.xny[l];
     const int ntx = (nxl + TILE - 1) / TILE, nty = (nyl + TILE - 1) / TILE;
     double dot = 0;
-    if ((int)blockIdx.x < ntx * nty) {
-        const int jx = (blockIdx.x % ntx) * TILE + threadIdx.x % TILE;
-        const int jy = (blockIdx.x / ntx) * TILE + threadIdx.x / TILE;
+    if (bx < ntx * nty) {
+        const int jx = (bx % ntx) * TILE + threadIdx.x % TILE;
+        const int jy = (bx / ntx) * TILE + threadIdx.x / TILE;
         if (jx < nxl && jy < nyl) {
             RT acc = 0;
             const long xt = G.hxt_xoff[G.npsr];  // per-WFS stride of transposed tables
@@ -395,7 +471,7 @@ __global__ __launch_bounds__(256) void k_layer(Geom G, const RT *q, const float 
         }
     }
     dot = block_sum(dot, sh);
-    if (threadIdx.x == 0 && part) part[(long)s * partstride + blockIdx.y * gridDim.x + blockIdx.x] = dot;
+    if (threadIdx.x == 0 && part) part[(long)s * partstride + by * PD.ltiles + bx] = dot;
 }
 
 // ---------------------------------------------------------------------------
@@ -407,7 +483,10 @@ __global__ __launch_bounds__(256) void k_layer(Geom G, const RT *q, const float 
 // staged in LDS with coalesced loads.
 // ---------------------------------------------------------------------------
 template <int POS, typename RT, int CS>
-__global__ __launch_bounds__(256) void k_wfs_m(Geom G, int nsys, const float *x, RT *q) {
+__global__ __launch_bounds__(256) void k_wfs_m(KARGS, int xsel) {
+    KJOB;
+    const float *x = vsel(PD, xsel);
+    RT *q = (RT *)PD.q;
     constexpr int H = TILE + 2 * POS;   // opd window edge
     constexpr int S = TILE + POS;       // subaperture-corner window edge
     constexpr int S2 = S * S;
@@ -416,10 +495,10 @@ __global__ __launch_bounds__(256) void k_wfs_m(Geom G, int nsys, const float *x,
     __shared__ float2 gps[S2 * NP2];  // GP coefficients of the window's corners
     __shared__ RT opd[CS][H * H];
     __shared__ RT wgx[CS][S2], wgy[CS][S2];
-    const int w = blockIdx.y, s0 = blockIdx.z * CS, ns = min(CS, nsys - s0);
+    const int w = by, s0 = J.s0 + bz * CS, ns = min(CS, J.s0 + J.nsys - s0);
     const int pnx = G.pnx, pny = G.pny;
     const int ntx = (pnx + TILE - 1) / TILE;
-    const int tx0 = (blockIdx.x % ntx) * TILE, ty0 = (blockIdx.x / ntx) * TILE;
+    const int tx0 = (bx % ntx) * TILE, ty0 = (bx / ntx) * TILE;
     const float2 *gpw = G.gpd + (long)G.gcls[w] * G.nploc * NP2;
     const float4 *nw4 = G.nead + (long)G.gcls[w] * G.nploc;
     // geometry of the window (shared by the CS systems)
@@ -459,7 +538,7 @@ __global__ __launch_bounds__(256) void k_wfs_m(Geom G, int nsys, const float *x,
 #pragma unroll
         for (int si = 0; si < CS; si++) v[u][si] = 0;
     for (int l = 0; l < G.npsr; l++) {
-        const MRay m = G.mray[w * MAXL + l];
+        const MRay m = mray(G, w * MAXL + l);
         const int nxl = G.xnx[l], nyl = G.xny[l];
         const float *xl = x + (long)s0 * G.nxtot + G.xoff[l];
         const RT fx = m.fx, fy = m.fy;
@@ -563,18 +642,23 @@ __global__ __launch_bounds__(256) void k_wfs_m(Geom G, int nsys, const float *x,
 }
 
 template <bool LHS, typename RT, int CS>
-__global__ __launch_bounds__(256) void k_layer_m(Geom G, int nsys, const RT *q, const float *p, RT *out,
-                                                 double *part, int partstride) {
+__global__ __launch_bounds__(256) void k_layer_m(KARGS, int psel_, int osel, int partsel) {
+    KJOB;
+    const RT *q = (const RT *)PD.q;
+    const float *p = vsel(PD, psel_);
+    RT *out = rsel<RT>(PD, osel);
+    double *part = psel(PD, partsel);
+    const int partstride = PD.SC.maxpart;
     constexpr int PH = TILE + 4;     // p window edge (L2'L2 reach is 2)
     constexpr int PH2 = PH * PH;
     constexpr int UP = (PH2 + 255) / 256;
     __shared__ float ps[CS][LHS ? PH2 : 1];
     __shared__ double sh[8];
-    const int l = blockIdx.y, s0 = blockIdx.z * CS, ns = min(CS, nsys - s0);
+    const int l = by, s0 = J.s0 + bz * CS, ns = min(CS, J.s0 + J.nsys - s0);
     const int nxl = G.xnx[l], nyl = G.xny[l];
     const int ntx = (nxl + TILE - 1) / TILE, nty = (nyl + TILE - 1) / TILE;
-    const bool active = (int)blockIdx.x < ntx * nty;
-    const int jx0 = (blockIdx.x % ntx) * TILE, jy0 = (blockIdx.x / ntx) * TILE;
+    const bool active = bx < ntx * nty;
+    const int jx0 = (bx % ntx) * TILE, jy0 = (bx / ntx) * TILE;
     const int jx = jx0 + (int)threadIdx.x % TILE, jy = jy0 + (int)threadIdx.x / TILE;
     const bool mine = active && jx < nxl && jy < nyl;
     if (LHS && active) {
@@ -621,7 +705,7 @@ __global__ __launch_bounds__(256) void k_layer_m(Geom G, int nsys, const RT *q, 
     const int tw = min(TILE, nxl - jx0), th = min(TILE, nyl - jy0);
     if (mine) {
         for (int w = 0; w < G.nwfs; w++) {
-            const MRay m = G.mray[w * MAXL + l];
+            const MRay m = mray(G, w * MAXL + l);
             const bool edgex = m.wx && jx0 == 0, edgey = m.wy && jy0 == 0;
             if (!edgex && (jx0 + tw - 1 - m.cx < m.ax0 || jx0 - m.cx - 1 >= m.ax1)) continue;
             if (!edgey && (jy0 + th - 1 - m.cy < m.ay0 || jy0 - m.cy - 1 >= m.ay1)) continue;
@@ -690,7 +774,7 @@ __global__ __launch_bounds__(256) void k_layer_m(Geom G, int nsys, const RT *q, 
     for (int si = 0; si < CS; si++) {
         const double d = block_sum(dot[si], sh);
         if (threadIdx.x == 0 && part && si < ns)
-            part[(long)(s0 + si) * partstride + blockIdx.y * gridDim.x + blockIdx.x] = d;
+            part[(long)(s0 + si) * partstride + by * PD.ltiles + bx] = d;
     }
 }
 
@@ -701,15 +785,21 @@ __global__ __launch_bounds__(256) void k_layer_m(Geom G, int nsys, const RT *q, 
 enum { ROW_INIT_COLD = 0, ROW_INIT_WARM = 1, ROW_UPDATE = 2, ROW_LAST = 3 };
 
 template <int MODE, typename RT, typename FT, int RM>
-__global__ __launch_bounds__(256) void k_row(Geom G, Scal SC, int iter, float *x, RT *r,
-                                             const RT *b, const float *p, const RT *Ap,
-                                             FT *spec, int *stop) {
+__global__ __launch_bounds__(256) void k_row(KARGS, int iter, int xsel) {
+    KJOB;
+    const Scal SC = ldsc(PD.SC);
+    float *x = vsel(PD, xsel);
+    RT *r = (RT *)PD.r;
+    const RT *b = (const RT *)PD.b, *Ap = (const RT *)PD.Ap;
+    const float *p = PD.p;
+    FT *spec = (FT *)PD.specf;
+    int *stop = PD.stop;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     FT *lds = reinterpret_cast<FT *>(lds_raw);
     typedef typename CT<FT>::R FR;
     __shared__ double sh[8];
-    const int s = blockIdx.z;
-    const int2 rb = G.rowblk[blockIdx.x];
+    const int s = J.s0 + bz;
+    const int2 rb = G.rowblk[bx];
     const int l = rb.x, row0 = rb.y;
     const int n = G.fnx, nyl = G.xny[l];
     const int nrows = min(ROWS, nyl - row0);
@@ -756,13 +846,13 @@ __global__ __launch_bounds__(256) void k_row(Geom G, Scal SC, int iter, float *x
             // lib/pcg.c:74-77: a non-finite alpha ends the solve; every
             // workgroup computes the same alpha, so all of them freeze x.
             a = 0;
-            if (!stopped && threadIdx.x == 0 && blockIdx.x == 0) stop[s] = 1;
+            if (!stopped && threadIdx.x == 0 && bx == 0) stop[s] = 1;
         }
         alpha = a;
     }
     // x += alpha p of an UPDATE iteration is applied by the inverse row
     // kernel, which reads p anyway for p = z + beta p.
-    if (MODE == ROW_UPDATE && blockIdx.x == 0 && threadIdx.x == 0) SC.alpha[s] = alpha;
+    if (MODE == ROW_UPDATE && bx == 0 && threadIdx.x == 0) SC.alpha[s] = alpha;
     FR *bufA = reinterpret_cast<FR *>(lds);
     double dot = 0;
     for (int c0 = 0; c0 < tot; c0 += 256 * U) {
@@ -771,7 +861,7 @@ __global__ __launch_bounds__(256) void k_row(Geom G, Scal SC, int iter, float *x
         for (int u = 0; u < U; u++) {
             const int i = c0 + threadIdx.x + 256 * u;
             if (i < tot) {
-                const int row = fdiv(i, G.dfnx), col = i - row * n;
+                const int row = fdiv(i, ldf(G.dfnx)), col = i - row * n;
                 const long idx = base + i;
                 RT rv;
                 if (MODE == ROW_INIT_COLD) {
@@ -791,7 +881,7 @@ __global__ __launch_bounds__(256) void k_row(Geom G, Scal SC, int iter, float *x
     }
     if (MODE == ROW_LAST) {
         dot = block_sum(dot, sh);
-        if (threadIdx.x == 0) SC.part_rr[(long)s * SC.maxpart + blockIdx.x] = dot;
+        if (threadIdx.x == 0) SC.part_rr[(long)s * SC.maxpart + bx] = dot;
         return;
     }
     __syncthreads();
@@ -799,7 +889,7 @@ __global__ __launch_bounds__(256) void k_row(Geom G, Scal SC, int iter, float *x
     FT *res = fft_run<-1, FT, RM>(A, B, n, nf, G.nstx, G.fsx, twl);
     const int nh = n / 2 + 1;
     FT *S = spec + (long)s * G.nspec + G.soff[l];
-    const FDiv dnf = G.dsmall[nf];
+    const FDiv dnf = ldf(G.dsmall[nf]);
     for (int i = threadIdx.x; i < nf * nh; i += blockDim.x) {
         const int k = fdiv(i, dnf), f = i - k * nf;
         const FT Z = res[f * n + k];
@@ -820,13 +910,17 @@ __global__ __launch_bounds__(256) void k_row(Geom G, Scal SC, int iter, float *x
 // FFT ping-pongs between it and the (then free) forward result.
 // ---------------------------------------------------------------------------
 template <typename FT, int CS, int BS, int RM>
-__global__ __launch_bounds__(256) void k_col(Geom G, Scal SC, int nsys, const FT *specf, float2 *speci) {
+__global__ __launch_bounds__(256) void k_col(KARGS) {
+    KJOB;
+    const Scal SC = ldsc(PD.SC);
+    const FT *specf = (const FT *)PD.specf;
+    float2 *speci = PD.speci;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ double sh[8];
     typedef typename CT<FT>::R FR;
-    const int g = blockIdx.x;
-    const int s0 = blockIdx.z * CS;
-    const int ns = min(CS, nsys - s0);
+    const int g = bx;
+    const int s0 = J.s0 + bz * CS;
+    const int ns = min(CS, J.s0 + J.nsys - s0);
     const int ny = G.fny, npsr = G.npsr;
     const int c0 = G.grp_col[g], c1 = G.grp_col[g + 1];
     const int ncg = c1 - c0;
@@ -847,8 +941,8 @@ __global__ __launch_bounds__(256) void k_col(Geom G, Scal SC, int nsys, const FT
             if (i < tot) {
                 const int si = CS == 1 ? 0 : (i >= per) + (i >= 2 * per) + (i >= 3 * per);
                 const int j = i - si * per;
-                const int f = fdiv(j, G.dfny), ky = j - f * ny;
-                const int ci = fdiv(f, G.dnpsr), l = f - ci * npsr;
+                const int f = fdiv(j, ldf(G.dfny)), ky = j - f * ny;
+                const int ci = fdiv(f, ldf(G.dnpsr)), l = f - ci * npsr;
                 v[u] = specf[(long)(s0 + si) * G.nspec + G.soff[l] + (long)G.colkx[c0 + ci] * ny + ky];
             }
         }
@@ -925,7 +1019,7 @@ __global__ __launch_bounds__(256) void k_col(Geom G, Scal SC, int nsys, const FT
         if (!((pm >> 29) & 1)) {
             const int pos = pm & 0x1fffffff;
             const FR cj = ((pm >> 30) & 1) ? FR(-1) : FR(1);
-            const int kx = G.colkx[c0 + fdiv(fdiv(pos, G.dfny), G.dnpsr)];
+            const int kx = G.colkx[c0 + fdiv(fdiv(pos, ldf(G.dfny)), ldf(G.dnpsr))];
             const double wgt = (kx == 0 || 2 * kx == G.fnx) ? 1.0 : 2.0;
 #pragma unroll
             for (int si = 0; si < CS; si++) {
@@ -947,8 +1041,8 @@ __global__ __launch_bounds__(256) void k_col(Geom G, Scal SC, int nsys, const FT
     for (int i = threadIdx.x; i < tot; i += blockDim.x) {
         const int si = CS == 1 ? 0 : (i >= per) + (i >= 2 * per) + (i >= 3 * per);
         const int j = i - si * per;
-        const int f = fdiv(j, G.dfny), ky = j - f * ny;
-        const int ci = fdiv(f, G.dnpsr), l = f - ci * npsr;
+        const int f = fdiv(j, ldf(G.dfny)), ky = j - f * ny;
+        const int ci = fdiv(f, ldf(G.dnpsr)), l = f - ci * npsr;
         speci[(long)(s0 + si) * G.nspec + G.soff[l] + (long)G.colkx[c0 + ci] * ny + ky] = res2[i];
     }
 }
@@ -966,13 +1060,17 @@ __global__ __launch_bounds__(256) void k_col(Geom G, Scal SC, int nsys, const FT
 enum { IROW_Z = 0, IROW_FIRST = 1, IROW_UPDATE = 2 };
 
 template <int MODE, int RM>
-__global__ __launch_bounds__(256) void k_irow(Geom G, Scal SC, int slot, const float2 *spec, float *x,
-                                              float *out) {
+__global__ __launch_bounds__(256) void k_irow(KARGS, int slot, int osel) {
+    KJOB;
+    const Scal SC = ldsc(PD.SC);
+    const float2 *spec = PD.speci;
+    float *x = PD.x;
+    float *out = vsel(PD, osel);
     extern __shared__ __align__(16) unsigned char lds_raw[];
     float2 *lds = reinterpret_cast<float2 *>(lds_raw);
     __shared__ double sh[8];
-    const int s = blockIdx.z;
-    const int2 rb = G.rowblk[blockIdx.x];
+    const int s = J.s0 + bz;
+    const int2 rb = G.rowblk[bx];
     const int l = rb.x, row0 = rb.y;
     const int n = G.fnx, nyl = G.xny[l];
     const int nrows = min(ROWS, nyl - row0);
@@ -984,7 +1082,7 @@ __global__ __launch_bounds__(256) void k_irow(Geom G, Scal SC, int slot, const f
     // Loads first (spectrum, and x, p for the update), then the r.z
     // reduction while they are in flight.
     const int tots = nf * n;
-    const FDiv dnf = G.dsmall[nf];
+    const FDiv dnf = ldf(G.dsmall[nf]);
     constexpr int US = 4;
     float2 va[US], vb[US];
     auto load_spec = [&](int c0) {
@@ -1044,7 +1142,7 @@ __global__ __launch_bounds__(256) void k_irow(Geom G, Scal SC, int slot, const f
             if (!isfinite(beta)) beta = 0;
             alpha = SC.alpha[s];
         }
-        if (blockIdx.x == 0 && threadIdx.x == 0) hist[MODE == IROW_FIRST ? slot : slot + 1] = rz;
+        if (bx == 0 && threadIdx.x == 0) hist[MODE == IROW_FIRST ? slot : slot + 1] = rz;
     }
     for (int c0 = 0; c0 < tots; c0 += 256 * US) {
         if (c0 > 0) load_spec(c0);
@@ -1060,7 +1158,7 @@ __global__ __launch_bounds__(256) void k_irow(Geom G, Scal SC, int slot, const f
         for (int u = 0; u < U; u++) {
             const int i = c0 + threadIdx.x + 256 * u;
             if (i < tot) {
-                const int row = fdiv(i, G.dfnx), col = i - row * n;
+                const int row = fdiv(i, ldf(G.dfnx)), col = i - row * n;
                 const float2 v = res[(row >> 1) * n + col];
                 const float zv = ((row & 1) ? v.y : v.x) * sc;
                 if (MODE == IROW_UPDATE) {
@@ -1075,9 +1173,13 @@ __global__ __launch_bounds__(256) void k_irow(Geom G, Scal SC, int slot, const f
 }
 
 // Residual sqrt(r.r / b.b) per system (NaN when the solve stopped early).
-__global__ __launch_bounds__(256) void k_final(Geom G, Scal SC, int npart_rr, const int *stop) {
+__global__ __launch_bounds__(256) void k_final(KARGS) {
+    KJOB;
+    const Scal SC = ldsc(PD.SC);
+    const int *stop = PD.stop;
+    const int npart_rr = G.nrowblk;
     __shared__ double sh[8];
-    const int s = blockIdx.z;
+    const int s = J.s0 + bz;
     const double rr = reduce_partials(SC.part_rr + (long)s * SC.maxpart, npart_rr, sh);
     const double bb = reduce_partials(SC.part_bb + (long)s * SC.maxpart, SC.npart_layer, sh);
     if (threadIdx.x == 0) SC.res[s] = stop[s] ? NAN : sqrt(rr / bb);
@@ -1087,224 +1189,144 @@ __global__ __launch_bounds__(256) void k_final(Geom G, Scal SC, int npart_rr, co
 __global__ void k_copy(const float *a, float *b, long n) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
 }
-__global__ void k_zero_i(int *a, int n) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) a[i] = 0;
+__global__ void k_zero_stop(KARGS) {
+    KJOB;
+    for (int i = threadIdx.x; i < J.nsys; i += blockDim.x) PD.stop[J.s0 + i] = 0;
 }
 
 // ---------------------------------------------------------------------------
-// launch wrappers
+// launch wrappers: one 1-D grid of t.nwg workgroups, each finding its job and
+// block in t.wgi.  Template choices (precision, radix set, block size,
+// systems per workgroup) are uniform over a launch.
 // ---------------------------------------------------------------------------
-void launch_ttr(const Geom &G, int nsys, const float *grad, float *tt, hipStream_t st) {
-    hipLaunchKernelGGL(k_ttr, dim3(G.nwfs, 1, nsys), dim3(256), 0, st, G, grad, tt);
+#define KT t.jobs, t.wgi
+void launch_ttr(const LaunchTab &t, hipStream_t st) {
+    hipLaunchKernelGGL(k_ttr, dim3(t.nwg), dim3(256), 0, st, KT);
 }
 template <typename RT>
-static void launch_wfs_t(const Geom &G, int nsys, bool rhs, const float *x, const float *grad,
-                         const float *tt, RT *q, hipStream_t st) {
-    const int nt = ((G.pnx + TILE - 1) / TILE) * ((G.pny + TILE - 1) / TILE);
-    dim3 grid(nt, G.nwfs, nsys);
-    if (G.pos == 1) {
-        if (rhs) hipLaunchKernelGGL((k_wfs<1, true, RT>), grid, dim3(256), 0, st, G, x, grad, tt, q);
-        else hipLaunchKernelGGL((k_wfs<1, false, RT>), grid, dim3(256), 0, st, G, x, grad, tt, q);
+static void launch_wfs_rt(const LaunchTab &t, bool rhs, int matched, int pos, int cs, int xsel, hipStream_t st) {
+    const dim3 grid(t.nwg), blk(256);
+    if (rhs) {
+        if (pos == 1) hipLaunchKernelGGL((k_wfs<1, true, RT>), grid, blk, 0, st, KT, xsel);
+        else hipLaunchKernelGGL((k_wfs<2, true, RT>), grid, blk, 0, st, KT, xsel);
+    } else if (!matched) {
+        if (pos == 1) hipLaunchKernelGGL((k_wfs<1, false, RT>), grid, blk, 0, st, KT, xsel);
+        else hipLaunchKernelGGL((k_wfs<2, false, RT>), grid, blk, 0, st, KT, xsel);
+    } else if (pos == 1) {
+        if (cs == 4) hipLaunchKernelGGL((k_wfs_m<1, RT, 4>), grid, blk, 0, st, KT, xsel);
+        else if (cs == 2) hipLaunchKernelGGL((k_wfs_m<1, RT, 2>), grid, blk, 0, st, KT, xsel);
+        else hipLaunchKernelGGL((k_wfs_m<1, RT, 1>), grid, blk, 0, st, KT, xsel);
     } else {
-        if (rhs) hipLaunchKernelGGL((k_wfs<2, true, RT>), grid, dim3(256), 0, st, G, x, grad, tt, q);
-        else hipLaunchKernelGGL((k_wfs<2, false, RT>), grid, dim3(256), 0, st, G, x, grad, tt, q);
+        if (cs >= 2) hipLaunchKernelGGL((k_wfs_m<2, RT, 2>), grid, blk, 0, st, KT, xsel);
+        else hipLaunchKernelGGL((k_wfs_m<2, RT, 1>), grid, blk, 0, st, KT, xsel);
     }
 }
-// Systems per workgroup of the matched operator kernels: the most of 4, 2,
-// 1 that still gives >= 2 workgroups per CU (pos 2: at most 2, for LDS).
-static int op_systems(long wg_per_sys, int nsys, int maxcs) {
-    for (int cs = maxcs; cs > 1; cs >>= 1)
-        if (wg_per_sys * ((nsys + cs - 1) / cs) >= 512) return cs;
-    return 1;
-}
-template <typename RT, int POS>
-static void launch_wfs_m_p(const Geom &G, int nsys, const float *x, RT *q, hipStream_t st) {
-    const int nt = ((G.pnx + TILE - 1) / TILE) * ((G.pny + TILE - 1) / TILE);
-    const int cs = op_systems((long)nt * G.nwfs, nsys, POS == 1 ? 4 : 2);
-    dim3 grid(nt, G.nwfs, (nsys + cs - 1) / cs);
-    if (cs == 4) hipLaunchKernelGGL((k_wfs_m<POS, RT, POS == 1 ? 4 : 2>), grid, dim3(256), 0, st, G, nsys, x, q);
-    else if (cs == 2) hipLaunchKernelGGL((k_wfs_m<POS, RT, 2>), grid, dim3(256), 0, st, G, nsys, x, q);
-    else hipLaunchKernelGGL((k_wfs_m<POS, RT, 1>), grid, dim3(256), 0, st, G, nsys, x, q);
-}
-template <typename RT>
-static void launch_wfs_m(const Geom &G, int nsys, const float *x, RT *q, hipStream_t st) {
-    if (G.pos == 1) launch_wfs_m_p<RT, 1>(G, nsys, x, q, st);
-    else launch_wfs_m_p<RT, 2>(G, nsys, x, q, st);
+void launch_wfs(const LaunchTab &t, bool rhs, int matched, int pos, int cs, int rt64, int xsel, hipStream_t st) {
+    if (rt64) launch_wfs_rt<double>(t, rhs, matched, pos, cs, xsel, st);
+    else launch_wfs_rt<float>(t, rhs, matched, pos, cs, xsel, st);
 }
 template <bool LHS, typename RT>
-static void launch_layer_m(const Geom &G, int nsys, const RT *q, const float *p, RT *out, double *part,
-                           int partstride, hipStream_t st) {
-    const int nt = layer_tiles(G);
-    const int cs = op_systems((long)nt * G.npsr, nsys, 4);
-    dim3 grid(nt, G.npsr, (nsys + cs - 1) / cs);
-    if (cs == 4)
-        hipLaunchKernelGGL((k_layer_m<LHS, RT, 4>), grid, dim3(256), 0, st, G, nsys, q, p, out, part, partstride);
-    else if (cs == 2)
-        hipLaunchKernelGGL((k_layer_m<LHS, RT, 2>), grid, dim3(256), 0, st, G, nsys, q, p, out, part, partstride);
-    else
-        hipLaunchKernelGGL((k_layer_m<LHS, RT, 1>), grid, dim3(256), 0, st, G, nsys, q, p, out, part, partstride);
+static void launch_layer_rt(const LaunchTab &t, int matched, int cs, int psel_, int osel, int partsel,
+                            hipStream_t st) {
+    const dim3 grid(t.nwg), blk(256);
+    if (!matched) hipLaunchKernelGGL((k_layer<LHS, RT>), grid, blk, 0, st, KT, psel_, osel, partsel);
+    else if (cs == 4) hipLaunchKernelGGL((k_layer_m<LHS, RT, 4>), grid, blk, 0, st, KT, psel_, osel, partsel);
+    else if (cs == 2) hipLaunchKernelGGL((k_layer_m<LHS, RT, 2>), grid, blk, 0, st, KT, psel_, osel, partsel);
+    else hipLaunchKernelGGL((k_layer_m<LHS, RT, 1>), grid, blk, 0, st, KT, psel_, osel, partsel);
 }
-void launch_wfs(const Geom &G, int nsys, bool rhs, int rt64, const float *x, const float *grad,
-                const float *tt, void *q, hipStream_t st) {
-    if (!rhs && G.matched) {
-        if (rt64) launch_wfs_m<double>(G, nsys, x, (double *)q, st);
-        else launch_wfs_m<float>(G, nsys, x, (float *)q, st);
-        return;
-    }
-    if (rt64) launch_wfs_t<double>(G, nsys, rhs, x, grad, tt, (double *)q, st);
-    else launch_wfs_t<float>(G, nsys, rhs, x, grad, tt, (float *)q, st);
-}
-int layer_tiles(const Geom &G) {
-    int mx = 0;
-    for (int l = 0; l < G.npsr; l++) {
-        int t = ((G.xnx[l] + TILE - 1) / TILE) * ((G.xny[l] + TILE - 1) / TILE);
-        mx = t > mx ? t : mx;
-    }
-    return mx;
-}
-void launch_layer(const Geom &G, int nsys, bool lhs, int rt64, const void *q, const float *p, void *out,
-                  double *part, int partstride, hipStream_t st) {
-    dim3 grid(layer_tiles(G), G.npsr, nsys);
-    if (G.matched) {
-        if (rt64) {
-            if (lhs) launch_layer_m<true, double>(G, nsys, (const double *)q, p, (double *)out, part, partstride, st);
-            else launch_layer_m<false, double>(G, nsys, (const double *)q, p, (double *)out, part, partstride, st);
-        } else {
-            if (lhs) launch_layer_m<true, float>(G, nsys, (const float *)q, p, (float *)out, part, partstride, st);
-            else launch_layer_m<false, float>(G, nsys, (const float *)q, p, (float *)out, part, partstride, st);
-        }
-        return;
-    }
+void launch_layer(const LaunchTab &t, bool lhs, int matched, int cs, int rt64, int psel_, int osel, int partsel,
+                  hipStream_t st) {
     if (rt64) {
-        double *o = (double *)out;
-        const double *qq = (const double *)q;
-        if (lhs) hipLaunchKernelGGL((k_layer<true, double>), grid, dim3(256), 0, st, G, qq, p, o, part, partstride);
-        else hipLaunchKernelGGL((k_layer<false, double>), grid, dim3(256), 0, st, G, qq, p, o, part, partstride);
+        if (lhs) launch_layer_rt<true, double>(t, matched, cs, psel_, osel, partsel, st);
+        else launch_layer_rt<false, double>(t, matched, cs, psel_, osel, partsel, st);
     } else {
-        float *o = (float *)out;
-        const float *qq = (const float *)q;
-        if (lhs) hipLaunchKernelGGL((k_layer<true, float>), grid, dim3(256), 0, st, G, qq, p, o, part, partstride);
-        else hipLaunchKernelGGL((k_layer<false, float>), grid, dim3(256), 0, st, G, qq, p, o, part, partstride);
+        if (lhs) launch_layer_rt<true, float>(t, matched, cs, psel_, osel, partsel, st);
+        else launch_layer_rt<false, float>(t, matched, cs, psel_, osel, partsel, st);
     }
 }
 // LDS of the row kernels: ROWS/2 complex rows twice (ping-pong) + twiddles.
-size_t row_lds(const Geom &G, int f64) { return (f64 ? sizeof(double2) : sizeof(float2)) * (ROWS + 1) * G.fnx; }
+size_t row_lds(int fnx, int f64) { return (f64 ? sizeof(double2) : sizeof(float2)) * (ROWS + 1) * (size_t)fnx; }
 template <typename RT, typename FT, int RM>
-static void launch_row_r(const Geom &G, const Scal &SC, int nsys, int mode, int iter, float *x, RT *r,
-                         const RT *b, const float *p, const RT *Ap, FT *spec, int *stop, hipStream_t st) {
-    dim3 grid(G.nrowblk, 1, nsys);
-    const size_t lds = row_lds(G, sizeof(FT) == sizeof(double2));
+static void launch_row_r(const LaunchTab &t, int mode, int iter, int xsel, size_t lds, hipStream_t st) {
+    const dim3 grid(t.nwg), blk(256);
     switch (mode) {
         case ROW_INIT_COLD:
-            hipLaunchKernelGGL((k_row<ROW_INIT_COLD, RT, FT, RM>), grid, dim3(256), lds, st, G, SC, iter, x, r, b, p,
-                               Ap, spec, stop);
+            hipLaunchKernelGGL((k_row<ROW_INIT_COLD, RT, FT, RM>), grid, blk, lds, st, KT, iter, xsel);
             break;
         case ROW_INIT_WARM:
-            hipLaunchKernelGGL((k_row<ROW_INIT_WARM, RT, FT, RM>), grid, dim3(256), lds, st, G, SC, iter, x, r, b, p,
-                               Ap, spec, stop);
+            hipLaunchKernelGGL((k_row<ROW_INIT_WARM, RT, FT, RM>), grid, blk, lds, st, KT, iter, xsel);
             break;
         case ROW_UPDATE:
-            hipLaunchKernelGGL((k_row<ROW_UPDATE, RT, FT, RM>), grid, dim3(256), lds, st, G, SC, iter, x, r, b, p, Ap,
-                               spec, stop);
+            hipLaunchKernelGGL((k_row<ROW_UPDATE, RT, FT, RM>), grid, blk, lds, st, KT, iter, xsel);
             break;
-        default:
-            hipLaunchKernelGGL((k_row<ROW_LAST, RT, FT, RM>), grid, dim3(256), lds, st, G, SC, iter, x, r, b, p, Ap,
-                               spec, stop);
-            break;
+        default: hipLaunchKernelGGL((k_row<ROW_LAST, RT, FT, RM>), grid, blk, lds, st, KT, iter, xsel); break;
     }
 }
 template <typename RT, typename FT>
-static void launch_row_t(const Geom &G, const Scal &SC, int nsys, int mode, int iter, float *x, RT *r,
-                         const RT *b, const float *p, const RT *Ap, FT *spec, int *stop, hipStream_t st) {
-    if (G.rmax > 4) launch_row_r<RT, FT, 7>(G, SC, nsys, mode, iter, x, r, b, p, Ap, spec, stop, st);
-    else launch_row_r<RT, FT, 3>(G, SC, nsys, mode, iter, x, r, b, p, Ap, spec, stop, st);
+static void launch_row_t(const LaunchTab &t, int mode, int rm, int iter, int xsel, size_t lds, hipStream_t st) {
+    if (rm > 4) launch_row_r<RT, FT, 7>(t, mode, iter, xsel, lds, st);
+    else launch_row_r<RT, FT, 3>(t, mode, iter, xsel, lds, st);
 }
-void launch_row(const Geom &G, const Scal &SC, int nsys, int mode, int rt64, int spec64, int iter, float *x,
-                void *r, const void *b, const float *p, const void *Ap, void *spec, int *stop, hipStream_t st) {
-    if (rt64 && spec64)
-        launch_row_t<double, double2>(G, SC, nsys, mode, iter, x, (double *)r, (const double *)b, p,
-                                      (const double *)Ap, (double2 *)spec, stop, st);
-    else if (rt64)
-        launch_row_t<double, float2>(G, SC, nsys, mode, iter, x, (double *)r, (const double *)b, p,
-                                     (const double *)Ap, (float2 *)spec, stop, st);
-    else
-        launch_row_t<float, float2>(G, SC, nsys, mode, iter, x, (float *)r, (const float *)b, p, (const float *)Ap,
-                                    (float2 *)spec, stop, st);
+void launch_row(const LaunchTab &t, int mode, int rt64, int spec64, int rm, int iter, int xsel, size_t lds,
+                hipStream_t st) {
+    if (rt64 && spec64) launch_row_t<double, double2>(t, mode, rm, iter, xsel, lds, st);
+    else if (rt64) launch_row_t<double, float2>(t, mode, rm, iter, xsel, lds, st);
+    else launch_row_t<float, float2>(t, mode, rm, iter, xsel, lds, st);
 }
-// LDS of the column kernel: two buffers of cs systems' column groups.
-size_t col_lds(const Geom &G, int f64, int cs) {
-    return (f64 ? sizeof(double2) : sizeof(float2)) * (2 * (size_t)cs * G.maxcol * G.npsr * G.fny + G.fny) +
-           sizeof(float2) * G.fny;
-}
-// Systems per column workgroup: FP64 spectra 2 or 1, FP32 spectra 4 or 1;
-// the larger when it still gives >= 1.5 workgroups per CU and fits 64 KiB of
-// LDS.
-int col_systems(const Geom &G, int nsys, int f64) {
-    const int cs = f64 ? 2 : 4;
-    if ((long)G.ngroup * ((nsys + cs - 1) / cs) >= 384 && col_lds(G, f64, cs) <= 64 * 1024) return cs;
-    return 1;
+// LDS of the column kernel: two buffers of cs systems' column groups + the
+// forward and inverse twiddles.
+size_t col_lds(int maxcol, int npsr, int fny, int f64, int cs) {
+    return (f64 ? sizeof(double2) : sizeof(float2)) * (2 * (size_t)cs * maxcol * npsr * fny + fny) +
+           sizeof(float2) * fny;
 }
 template <typename FT, int CS, int RM>
-static void launch_col_cs(const Geom &G, const Scal &SC, int nsys, const FT *specf, float2 *speci, hipStream_t st) {
-    const dim3 grid(G.ngroup, 1, (nsys + CS - 1) / CS);
-    const size_t lds = col_lds(G, sizeof(FT) == sizeof(double2), CS);
-    switch (G.bs) {
-        case 2: hipLaunchKernelGGL((k_col<FT, CS, 2, RM>), grid, dim3(256), lds, st, G, SC, nsys, specf, speci); break;
-        case 3: hipLaunchKernelGGL((k_col<FT, CS, 3, RM>), grid, dim3(256), lds, st, G, SC, nsys, specf, speci); break;
-        case 4: hipLaunchKernelGGL((k_col<FT, CS, 4, RM>), grid, dim3(256), lds, st, G, SC, nsys, specf, speci); break;
-        case 6: hipLaunchKernelGGL((k_col<FT, CS, 6, RM>), grid, dim3(256), lds, st, G, SC, nsys, specf, speci); break;
-        case 8: hipLaunchKernelGGL((k_col<FT, CS, 8, RM>), grid, dim3(256), lds, st, G, SC, nsys, specf, speci); break;
-        default: hipLaunchKernelGGL((k_col<FT, CS, 0, RM>), grid, dim3(256), lds, st, G, SC, nsys, specf, speci); break;
+static void launch_col_cs(const LaunchTab &t, int bs, size_t lds, hipStream_t st) {
+    const dim3 grid(t.nwg), blk(256);
+    switch (bs) {
+        case 2: hipLaunchKernelGGL((k_col<FT, CS, 2, RM>), grid, blk, lds, st, KT); break;
+        case 3: hipLaunchKernelGGL((k_col<FT, CS, 3, RM>), grid, blk, lds, st, KT); break;
+        case 4: hipLaunchKernelGGL((k_col<FT, CS, 4, RM>), grid, blk, lds, st, KT); break;
+        case 6: hipLaunchKernelGGL((k_col<FT, CS, 6, RM>), grid, blk, lds, st, KT); break;
+        case 8: hipLaunchKernelGGL((k_col<FT, CS, 8, RM>), grid, blk, lds, st, KT); break;
+        default: hipLaunchKernelGGL((k_col<FT, CS, 0, RM>), grid, blk, lds, st, KT); break;
     }
 }
 template <typename FT, int CSMAX>
-static void launch_col_t(const Geom &G, const Scal &SC, int nsys, int cs, const FT *specf, float2 *speci,
-                         hipStream_t st) {
-    if (G.rmax > 4) {
-        if (cs > 1) launch_col_cs<FT, CSMAX, 7>(G, SC, nsys, specf, speci, st);
-        else launch_col_cs<FT, 1, 7>(G, SC, nsys, specf, speci, st);
+static void launch_col_t(const LaunchTab &t, int cs, int bs, int rm, size_t lds, hipStream_t st) {
+    if (rm > 4) {
+        if (cs > 1) launch_col_cs<FT, CSMAX, 7>(t, bs, lds, st);
+        else launch_col_cs<FT, 1, 7>(t, bs, lds, st);
     } else {
-        if (cs > 1) launch_col_cs<FT, CSMAX, 3>(G, SC, nsys, specf, speci, st);
-        else launch_col_cs<FT, 1, 3>(G, SC, nsys, specf, speci, st);
+        if (cs > 1) launch_col_cs<FT, CSMAX, 3>(t, bs, lds, st);
+        else launch_col_cs<FT, 1, 3>(t, bs, lds, st);
     }
 }
-void launch_col(const Geom &G, const Scal &SC, int nsys, int f64, const void *specf, float2 *speci,
-                hipStream_t st) {
-    const int cs = col_systems(G, nsys, f64);
-    if (f64) launch_col_t<double2, 2>(G, SC, nsys, cs, (const double2 *)specf, speci, st);
-    else launch_col_t<float2, 4>(G, SC, nsys, cs, (const float2 *)specf, speci, st);
+void launch_col(const LaunchTab &t, int spec64, int cs, int bs, int rm, size_t lds, hipStream_t st) {
+    if (spec64) launch_col_t<double2, 2>(t, cs, bs, rm, lds, st);
+    else launch_col_t<float2, 4>(t, cs, bs, rm, lds, st);
 }
 template <int RM>
-static void launch_irow_r(const Geom &G, const Scal &SC, int nsys, int mode, int slot, const float2 *spec, float *x,
-                          float *out, hipStream_t st) {
-    const dim3 grid(G.nrowblk, 1, nsys);
-    const size_t lds = row_lds(G, 0);
+static void launch_irow_r(const LaunchTab &t, int mode, int slot, int osel, size_t lds, hipStream_t st) {
+    const dim3 grid(t.nwg), blk(256);
     switch (mode) {
-        case IROW_Z:
-            hipLaunchKernelGGL((k_irow<IROW_Z, RM>), grid, dim3(256), lds, st, G, SC, slot, spec, x, out);
-            break;
-        case IROW_FIRST:
-            hipLaunchKernelGGL((k_irow<IROW_FIRST, RM>), grid, dim3(256), lds, st, G, SC, slot, spec, x, out);
-            break;
-        default:
-            hipLaunchKernelGGL((k_irow<IROW_UPDATE, RM>), grid, dim3(256), lds, st, G, SC, slot, spec, x, out);
-            break;
+        case IROW_Z: hipLaunchKernelGGL((k_irow<IROW_Z, RM>), grid, blk, lds, st, KT, slot, osel); break;
+        case IROW_FIRST: hipLaunchKernelGGL((k_irow<IROW_FIRST, RM>), grid, blk, lds, st, KT, slot, osel); break;
+        default: hipLaunchKernelGGL((k_irow<IROW_UPDATE, RM>), grid, blk, lds, st, KT, slot, osel); break;
     }
 }
-void launch_irow(const Geom &G, const Scal &SC, int nsys, int mode, int slot, const float2 *spec, float *x,
-                 float *out, hipStream_t st) {
-    if (G.rmax > 4) launch_irow_r<7>(G, SC, nsys, mode, slot, spec, x, out, st);
-    else launch_irow_r<3>(G, SC, nsys, mode, slot, spec, x, out, st);
+void launch_irow(const LaunchTab &t, int mode, int rm, int slot, int osel, size_t lds, hipStream_t st) {
+    if (rm > 4) launch_irow_r<7>(t, mode, slot, osel, lds, st);
+    else launch_irow_r<3>(t, mode, slot, osel, lds, st);
 }
-void launch_final(const Geom &G, const Scal &SC, int nsys, int npart_rr, const int *stop, hipStream_t st) {
-    hipLaunchKernelGGL(k_final, dim3(1, 1, nsys), dim3(256), 0, st, G, SC, npart_rr, stop);
+void launch_final(const LaunchTab &t, hipStream_t st) {
+    hipLaunchKernelGGL(k_final, dim3(t.nwg), dim3(256), 0, st, KT);
+}
+void launch_zero_stop(const LaunchTab &t, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_stop, dim3(t.nwg), dim3(256), 0, st, KT);
 }
 void launch_copy(const float *a, float *b, long n, hipStream_t st) {
     const long nb = (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;
     hipLaunchKernelGGL(k_copy, dim3((unsigned)nb), dim3(256), 0, st, a, b, n);
-}
-void launch_zero_i(int *a, int n, hipStream_t st) {
-    hipLaunchKernelGGL(k_zero_i, dim3((n + 255) / 256), dim3(256), 0, st, a, n);
 }
 
 }  // namespace mtk

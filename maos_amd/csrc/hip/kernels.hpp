@@ -107,7 +107,7 @@ struct Geom {
 };
 
 
-// Per-batch scalar state (device, FP64).
+// Per-plan scalar state (device, FP64).
 struct Scal {
     double *part_pap;   // [nsys][maxpart]
     double *part_rz;    // [nsys][maxpart]
@@ -123,4 +123,44 @@ struct Scal {
     int npart_col;      // partial count written by the column kernel (r.z)
 };
 
+// Everything a kernel needs of one plan: geometry, the state buffers of its
+// nbatch systems (residual-side arrays are double, or float when the plan
+// runs precision mode 0) and the scalar state.  One device copy per plan.
+struct PlanDev {
+    Geom G;
+    float *x, *z, *p, *g, *tt;
+    void *r, *Ap, *b, *q;
+    void *specf;        // forward spectrum (double2, or float2)
+    float2 *speci;      // inverse spectrum
+    Scal SC;
+    int *stop;
+    int ltiles;         // layer tiles per layer (layer partial index stride)
+};
+
+// A job is a range of systems of one plan; a launch covers a list of jobs,
+// and every workgroup finds its job and its (x, y, z) block in a table, so
+// one launch serves systems of different geometries (an uneven batch) as
+// well as one plan alone.
+struct Job {
+    const PlanDev *pd;
+    int s0, nsys;
+    int on;             // 0: the job's workgroups exit at once (idle slot)
+    int pad_;
+};
+struct WGI {
+    unsigned short job, bx, by, bz;
+};
+
+// Buffer selectors (which state array of the plan a kernel reads/writes).
+enum { BUF_X = 0, BUF_P = 1, BUF_Z = 2, BUF_AP = 3, BUF_B = 4 };
+enum { PART_NONE = 0, PART_PAP = 1, PART_BB = 2 };
+
+}  // namespace mtk
+
+// Loads through the constant address space are uniform and read-only: the
+// compiler uses scalar loads for them, as it does for kernel arguments.
+#define CONSTAS __attribute__((address_space(4)))
+namespace mtk {
+typedef const CONSTAS Geom CG;
+typedef const CONSTAS PlanDev CPD;
 }  // namespace mtk

@@ -13,14 +13,20 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
 #include <cstdio>
 #include <cstring>
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kernels.hpp"
@@ -82,6 +88,28 @@ const char *kc_names[KC_N] = {"k_ttr", "k_wfs", "k_layer", "k_row", "k_col", "k_
 
 }  // namespace
 
+struct gpu_tomo;
+
+namespace {
+// One launch of a stage: a template variant over a subset of the jobs.
+struct Sub {
+    int matched = 0, pos = 1, cs = 1, bs = 0, rm = 3;
+    size_t lds = 0;
+    DevBuf<WGI> wgi;
+    int nwg = 0;
+};
+typedef std::vector<std::unique_ptr<Sub>> Stage;
+// A list of jobs (system ranges of plans, possibly of different geometry)
+// solved in lockstep, and the workgroup tables of every stage.
+struct Batch {
+    std::vector<gpu_tomo *> plan;   // per job
+    std::vector<Job> hjob;
+    DevBuf<Job> jobs;
+    int rt64 = 1, spec64 = 1, maxit = 30, warm = 0, precond = 1, anyttr = 0;
+    Stage zero, ttr, wfs_r, layer, wfs_l, row, col, irow, fin;
+};
+}  // namespace
+
 struct gpu_tomo {
     int device = 0;
     int nbatch = 1, maxit = 30, warm = 0;
@@ -123,9 +151,15 @@ struct gpu_tomo {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_nsys = 0;
+    // device copy of geometry + buffers, and the launch tables of this plan's
+    // own system ranges (key: first system, count)
+    DevBuf<PlanDev> pdev;
+    std::map<std::pair<int, int>, std::unique_ptr<Batch>> own;
+    // systems per workgroup chosen by the autotuner (0: heuristic)
+    int tune_wfs = 0, tune_layer = 0, tune_col = 0;
     // profiling
     int prof = 0;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;  // profiling events
     double prof_ms[KC_N] = {0};
     int prof_cnt[KC_N] = {0};
 };
@@ -728,9 +762,39 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
         G.blkpos = g->blkpos.p;
         g->Minv.upload(group_order(half_blocks(R, nx0, ny0), g->blk_order, bs));
         G.Minv = g->Minv.p;
-        if (col_lds(G, 1, 1) > 64 * 1024) fail(MT_EINVAL, "FDPCG column group too large for LDS");
+        if (col_lds(G.maxcol, G.npsr, G.fny, 1, 1) > 64 * 1024)
+            fail(MT_EINVAL, "FDPCG column group too large for LDS");
     }
-    if (row_lds(G, 1) > 64 * 1024) fail(MT_EINVAL, "layer grid too large for the row FFT kernel");
+    if (row_lds(G.fnx, 1) > 64 * 1024) fail(MT_EINVAL, "layer grid too large for the row FFT kernel");
+}
+
+// Tiles of the layer passes (the largest layer decides; all layers are
+// equal when FDPCG is used).
+int layer_tiles(const Geom &G) {
+    int t = 1;
+    for (int l = 0; l < G.npsr; l++) t = std::max(t, ((G.xnx[l] + TILE - 1) / TILE) * ((G.xny[l] + TILE - 1) / TILE));
+    return t;
+}
+
+void upload_plandev(gpu_tomo *g) {
+    PlanDev h{};
+    h.G = g->G;
+    h.x = g->x.p;
+    h.z = g->z.p;
+    h.p = g->p.p;
+    h.g = g->g.p;
+    h.tt = g->tt.p;
+    h.r = g->r.p;
+    h.Ap = g->Ap.p;
+    h.b = g->b.p;
+    h.q = g->q.p;
+    h.specf = g->specf.p;
+    h.speci = g->speci.p;
+    h.SC = g->SC;
+    h.stop = g->stop.p;
+    h.ltiles = layer_tiles(g->G);
+    if (!g->pdev.p) g->pdev.alloc(1);
+    HIPCHK(hipMemcpy(g->pdev.p, &h, sizeof(PlanDev), hipMemcpyHostToDevice));
 }
 
 void alloc_state(gpu_tomo *g) {
@@ -776,14 +840,16 @@ void alloc_state(gpu_tomo *g) {
     SC.npart_layer = layer_tiles(G) * G.npsr;
     SC.npart_row = G.nrowblk;
     SC.npart_col = G.ngroup;
+    upload_plandev(g);
 }
 
 // ---------------------------------------------------------------------------
 // Launch sequence.  Every launch goes through rec(), which brackets it with
 // HIP events on the plan's stream when profiling is on.
 // ---------------------------------------------------------------------------
+typedef std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> EvList;
 struct Rec {
-    gpu_tomo *g;
+    EvList *evs;
     hipStream_t st;
     bool on;
     template <typename F>
@@ -795,35 +861,221 @@ struct Rec {
             HIPCHK(hipEventRecord(a, st));
             f();
             HIPCHK(hipEventRecord(b, st));
-            g->evs.push_back({kc, {a, b}});
+            evs->push_back({kc, {a, b}});
         } else {
             f();
         }
     }
 };
 
-// Operator A p -> Ap for all systems (WFS pass + layer pass).
-void enqueue_L(gpu_tomo *g, int nsys, const float *xin, void *xout, double *part, Rec &rec) {
-    const Geom &G = g->G;
-    rec(KC_WFS, [&] { launch_wfs(G, nsys, false, g->rt64, xin, nullptr, nullptr, g->q.p, rec.st); });
-    rec(KC_LAYER, [&] { launch_layer(G, nsys, true, g->rt64, g->q.p, xin, xout, part, g->SC.maxpart, rec.st); });
+// ---------------------------------------------------------------------------
+// Batches: the workgroup tables of every stage for a list of jobs.
+// ---------------------------------------------------------------------------
+struct JobGrid {
+    int gx, gy, gz;
+};
+// Partition the jobs by a variant key and emit one Sub per partition.
+template <typename KeyF, typename GridF, typename SubF>
+void build_stage(Batch &B, Stage &out, KeyF key, GridF grid, SubF fill) {
+    std::map<std::vector<int>, std::vector<int>> parts;
+    for (size_t j = 0; j < B.hjob.size(); j++) {
+        std::vector<int> k;
+        if (!key((int)j, k)) continue;  // job takes no part in this stage
+        parts[k].push_back((int)j);
+    }
+    for (auto &kv : parts) {
+        std::unique_ptr<Sub> sub(new Sub());
+        fill(*sub, kv.first, kv.second);
+        std::vector<WGI> tab;
+        for (int j : kv.second) {
+            const JobGrid jg = grid(*sub, j);
+            for (int z = 0; z < jg.gz; z++)
+                for (int y = 0; y < jg.gy; y++)
+                    for (int x = 0; x < jg.gx; x++)
+                        tab.push_back(WGI{(unsigned short)j, (unsigned short)x, (unsigned short)y, (unsigned short)z});
+        }
+        if (tab.empty()) continue;
+        sub->nwg = (int)tab.size();
+        sub->wgi.upload(tab);
+        out.push_back(std::move(sub));
+    }
 }
-// RHS b = HX' GP' W (I - TT PTT) g for all systems.
-void enqueue_R(gpu_tomo *g, int nsys, const float *grad, void *bout, Rec &rec) {
-    const Geom &G = g->G;
-    bool anyttr = false;
-    for (int k = 0; k < G.nwfs; k++) anyttr |= G.ttr[k] != 0;
-    if (anyttr) rec(KC_TTR, [&] { launch_ttr(G, nsys, grad, g->tt.p, rec.st); });
-    rec(KC_WFS_RHS, [&] { launch_wfs(G, nsys, true, g->rt64, nullptr, grad, g->tt.p, g->q.p, rec.st); });
-    rec(KC_LAYER_RHS, [&] { launch_layer(G, nsys, false, g->rt64, g->q.p, nullptr, bout, g->SC.part_bb, g->SC.maxpart, rec.st); });
+int ceil_div(int a, int b) { return (a + b - 1) / b; }
+// Systems per workgroup: the most of cmax, cmax/2, .., 1 that keeps >= thresh
+// workgroups in the launch; jw = (workgroups per system, systems) per job.
+int pick_cs(const std::vector<std::pair<long, int>> &jw, int cmax, int thresh) {
+    for (int cs = cmax; cs > 1; cs >>= 1) {
+        long tot = 0;
+        for (auto &p : jw) tot += p.first * ceil_div(p.second, cs);
+        if (tot >= thresh) return cs;
+    }
+    return 1;
+}
+
+std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const std::vector<int> &s0,
+                                  const std::vector<int> &nsys) {
+    std::unique_ptr<Batch> B(new Batch());
+    const gpu_tomo *g0 = plans.at(0);
+    B->rt64 = g0->rt64;
+    B->spec64 = g0->spec64;
+    B->maxit = g0->maxit;
+    B->warm = g0->warm;
+    B->precond = g0->G.precond;
+    for (size_t j = 0; j < plans.size(); j++) {
+        gpu_tomo *g = plans[j];
+        if (g->device != g0->device || g->rt64 != B->rt64 || g->spec64 != B->spec64 || g->maxit != B->maxit ||
+            g->warm != B->warm || g->G.precond != B->precond)
+            fail(MT_EINVAL, "plans of one batch must share device, precision, maxit, warm start and preconditioner");
+        if (s0[j] < 0 || nsys[j] <= 0 || s0[j] + nsys[j] > g->nbatch) fail(MT_EINVAL, "system range out of the plan");
+        B->plan.push_back(g);
+        B->hjob.push_back(Job{g->pdev.p, s0[j], nsys[j], 1, 0});
+        for (int k = 0; k < g->G.nwfs; k++) B->anyttr |= g->G.ttr[k] != 0;
+    }
+    if (B->hjob.size() > 65535) fail(MT_EINVAL, "too many jobs in one batch");
+    B->jobs.upload(B->hjob);
+    Batch &b = *B;
+    auto G = [&](int j) -> const Geom & { return b.plan[j]->G; };
+    auto NS = [&](int j) { return b.hjob[j].nsys; };
+    auto ptiles = [&](int j) { return ceil_div(G(j).pnx, TILE) * ceil_div(G(j).pny, TILE); };
+    auto jw_of = [&](const std::vector<int> &js, std::function<long(int)> per) {
+        std::vector<std::pair<long, int>> v;
+        for (int j : js) v.push_back({per(j), NS(j)});
+        return v;
+    };
+    // per-stage tables
+    build_stage(b, b.zero, [&](int, std::vector<int> &) { return true; },
+                [&](const Sub &, int) { return JobGrid{1, 1, 1}; }, [&](Sub &, const std::vector<int> &, const std::vector<int> &) {});
+    build_stage(b, b.ttr,
+                [&](int j, std::vector<int> &) {
+                    for (int k = 0; k < G(j).nwfs; k++)
+                        if (G(j).ttr[k]) return true;
+                    return false;
+                },
+                [&](const Sub &, int j) { return JobGrid{G(j).nwfs, 1, NS(j)}; },
+                [&](Sub &, const std::vector<int> &, const std::vector<int> &) {});
+    build_stage(b, b.wfs_r, [&](int j, std::vector<int> &k) { k = {G(j).pos}; return true; },
+                [&](const Sub &, int j) { return JobGrid{ptiles(j), G(j).nwfs, NS(j)}; },
+                [&](Sub &sb, const std::vector<int> &k, const std::vector<int> &) { sb.pos = k[0]; });
+    build_stage(b, b.wfs_l, [&](int j, std::vector<int> &k) { k = {G(j).matched, G(j).pos}; return true; },
+                [&](const Sub &sb, int j) {
+                    return JobGrid{ptiles(j), G(j).nwfs, sb.matched ? ceil_div(NS(j), sb.cs) : NS(j)};
+                },
+                [&](Sub &sb, const std::vector<int> &k, const std::vector<int> &js) {
+                    sb.matched = k[0];
+                    sb.pos = k[1];
+                    const int tuned = b.plan[js[0]]->tune_wfs;
+                    sb.cs = !sb.matched ? 1
+                            : tuned     ? std::min(tuned, sb.pos == 1 ? 4 : 2)
+                                        : pick_cs(jw_of(js, [&](int j) { return (long)ptiles(j) * G(j).nwfs; }),
+                                                  sb.pos == 1 ? 4 : 2, 512);
+                });
+    build_stage(b, b.layer, [&](int j, std::vector<int> &k) { k = {G(j).matched}; return true; },
+                [&](const Sub &sb, int j) {
+                    return JobGrid{layer_tiles(G(j)), G(j).npsr, sb.matched ? ceil_div(NS(j), sb.cs) : NS(j)};
+                },
+                [&](Sub &sb, const std::vector<int> &k, const std::vector<int> &js) {
+                    sb.matched = k[0];
+                    const int tuned = b.plan[js[0]]->tune_layer;
+                    sb.cs = !sb.matched ? 1
+                            : tuned     ? tuned
+                                        : pick_cs(jw_of(js, [&](int j) { return (long)layer_tiles(G(j)) * G(j).npsr; }),
+                                                  4, 512);
+                });
+    if (b.precond) {
+        build_stage(b, b.row, [&](int j, std::vector<int> &k) { k = {G(j).rmax > 4 ? 7 : 3}; return true; },
+                    [&](const Sub &, int j) { return JobGrid{G(j).nrowblk, 1, NS(j)}; },
+                    [&](Sub &sb, const std::vector<int> &k, const std::vector<int> &js) {
+                        sb.rm = k[0];
+                        for (int j : js) sb.lds = std::max(sb.lds, row_lds(G(j).fnx, b.spec64));
+                    });
+        build_stage(b, b.irow, [&](int j, std::vector<int> &k) { k = {G(j).rmax > 4 ? 7 : 3}; return true; },
+                    [&](const Sub &, int j) { return JobGrid{G(j).nrowblk, 1, NS(j)}; },
+                    [&](Sub &sb, const std::vector<int> &k, const std::vector<int> &js) {
+                        sb.rm = k[0];
+                        for (int j : js) sb.lds = std::max(sb.lds, row_lds(G(j).fnx, 0));
+                    });
+        build_stage(b, b.col,
+                    [&](int j, std::vector<int> &k) {
+                        const int bs = G(j).bs;
+                        k = {(bs == 2 || bs == 3 || bs == 4 || bs == 6 || bs == 8) ? bs : 0, G(j).rmax > 4 ? 7 : 3};
+                        return true;
+                    },
+                    [&](const Sub &sb, int j) { return JobGrid{G(j).ngroup, 1, ceil_div(NS(j), sb.cs)}; },
+                    [&](Sub &sb, const std::vector<int> &k, const std::vector<int> &js) {
+                        sb.bs = k[0];
+                        sb.rm = k[1];
+                        const int cmax = b.spec64 ? 2 : 4;
+                        const int tuned = b.plan[js[0]]->tune_col;
+                        sb.cs = tuned ? std::min(tuned, cmax)
+                                      : pick_cs(jw_of(js, [&](int j) { return (long)G(j).ngroup; }), cmax, 384);
+                        for (int j : js)
+                            if (col_lds(G(j).maxcol, G(j).npsr, G(j).fny, b.spec64, sb.cs) > 64 * 1024) sb.cs = 1;
+                        for (int j : js)
+                            sb.lds = std::max(sb.lds, col_lds(G(j).maxcol, G(j).npsr, G(j).fny, b.spec64, sb.cs));
+                    });
+    }
+    build_stage(b, b.fin, [&](int, std::vector<int> &) { return true; },
+                [&](const Sub &, int j) { return JobGrid{1, 1, NS(j)}; },
+                [&](Sub &, const std::vector<int> &, const std::vector<int> &) {});
+    return B;
+}
+
+// The plan's own batch for systems [s0, s0 + n).
+Batch &own_batch(gpu_tomo *g, int s0, int n) {
+    auto key = std::make_pair(s0, n);
+    auto it = g->own.find(key);
+    if (it != g->own.end()) return *it->second;
+    std::unique_ptr<Batch> B = make_batch({g}, {s0}, {n});
+    Batch &ref = *B;
+    g->own[key] = std::move(B);
+    return ref;
+}
+void drop_batches(gpu_tomo *g) {
+    g->own.clear();
+    if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    g->gexec = nullptr;
+    g->graph = nullptr;
+}
+
+LaunchTab tab(const Batch &B, const Sub &s) { return LaunchTab{B.jobs.p, s.wgi.p, s.nwg}; }
+
+// Operator A v -> Ap for all jobs (WFS pass + layer pass).
+void enqueue_L(Batch &B, int vsel, int partsel, Rec &rec) {
+    rec(KC_WFS, [&] {
+        for (auto &s : B.wfs_l) launch_wfs(tab(B, *s), false, s->matched, s->pos, s->cs, B.rt64, vsel, rec.st);
+    });
+    rec(KC_LAYER, [&] {
+        for (auto &s : B.layer)
+            launch_layer(tab(B, *s), true, s->matched, s->cs, B.rt64, vsel, BUF_AP, partsel, rec.st);
+    });
+}
+// RHS b = HX' GP' W (I - TT PTT) g for all jobs.
+void enqueue_R(Batch &B, Rec &rec) {
+    if (!B.ttr.empty()) rec(KC_TTR, [&] {
+            for (auto &s : B.ttr) launch_ttr(tab(B, *s), rec.st);
+        });
+    rec(KC_WFS_RHS, [&] {
+        for (auto &s : B.wfs_r) launch_wfs(tab(B, *s), true, 0, s->pos, 1, B.rt64, BUF_X, rec.st);
+    });
+    rec(KC_LAYER_RHS, [&] {
+        for (auto &s : B.layer) launch_layer(tab(B, *s), false, s->matched, s->cs, B.rt64, BUF_X, BUF_B, PART_BB, rec.st);
+    });
+}
+void enqueue_row(Batch &B, int kc, int mode, int iter, int xsel, Rec &rec) {
+    rec(kc, [&] {
+        for (auto &s : B.row) launch_row(tab(B, *s), mode, B.rt64, B.spec64, s->rm, iter, xsel, s->lds, rec.st);
+    });
 }
 // Preconditioner on the row-transformed residual (specf): column pass with
 // the spectral r.z partials, then the inverse row pass in the given mode.
-void enqueue_P_tail(gpu_tomo *g, int nsys, int irow_mode, int slot, float *out, Rec &rec) {
-    const Geom &G = g->G;
-    rec(KC_COL, [&] { launch_col(G, g->SC, nsys, g->spec64, g->specf.p, g->speci.p, rec.st); });
-    rec(irow_mode == IROW_MODE_UPDATE ? KC_IROW : KC_IROW_START,
-        [&] { launch_irow(G, g->SC, nsys, irow_mode, slot, g->speci.p, g->x.p, out, rec.st); });
+void enqueue_P_tail(Batch &B, int irow_mode, int slot, int osel, Rec &rec) {
+    rec(KC_COL, [&] {
+        for (auto &s : B.col) launch_col(tab(B, *s), B.spec64, s->cs, s->bs, s->rm, s->lds, rec.st);
+    });
+    rec(irow_mode == IROW_MODE_UPDATE ? KC_IROW : KC_IROW_START, [&] {
+        for (auto &s : B.irow) launch_irow(tab(B, *s), irow_mode, s->rm, slot, osel, s->lds, rec.st);
+    });
 }
 
 // The solve as lib/pcg.c:28-113 runs it (restart every 100 iterations):
@@ -833,42 +1085,33 @@ void enqueue_P_tail(gpu_tomo *g, int nsys, int irow_mode, int slot, float *out, 
 //           beta = r.z / rz[k], x += alpha p, p = z + beta p
 // Kernels per iteration: wfs, layer (Ap, p.Ap partials), row (alpha, r,
 // forward row FFT), col (column FFTs, blocks, r.z partials), irow (inverse
-// row FFT, beta, x and p).
-void enqueue_solve(gpu_tomo *g, int nsys, int warm, Rec &rec) {
-    const Geom &G = g->G;
-    Scal &SC = g->SC;
-    const int maxit = g->maxit;
-    if (!G.precond) fail(MT_EINVAL, "the GPU solve requires the FDPCG preconditioner (tomo.precond=1)");
-    launch_zero_i(g->stop.p, nsys, rec.st);
-    enqueue_R(g, nsys, g->g.p, g->b.p, rec);
+// row FFT, beta, x and p).  All jobs of the batch advance in lockstep.
+void enqueue_solve(Batch &B, Rec &rec) {
+    const int maxit = B.maxit;
+    if (!B.precond) fail(MT_EINVAL, "the GPU solve requires the FDPCG preconditioner (tomo.precond=1)");
+    for (auto &s : B.zero) launch_zero_stop(tab(B, *s), rec.st);
+    enqueue_R(B, rec);
     auto start = [&](int k, bool cold) {
         if (cold) {
-            rec(KC_ROW_START, [&] {
-                launch_row(G, SC, nsys, ROW_MODE_INIT_COLD, g->rt64, g->spec64, k, g->x.p, g->r.p, g->b.p, g->p.p, g->Ap.p,
-                           g->specf.p, g->stop.p, rec.st);
-            });
+            enqueue_row(B, KC_ROW_START, ROW_MODE_INIT_COLD, k, BUF_X, rec);
         } else {
-            enqueue_L(g, nsys, g->x.p, g->Ap.p, nullptr, rec);
-            rec(KC_ROW_START, [&] {
-                launch_row(G, SC, nsys, ROW_MODE_INIT_WARM, g->rt64, g->spec64, k, g->x.p, g->r.p, g->b.p, g->p.p, g->Ap.p,
-                           g->specf.p, g->stop.p, rec.st);
-            });
+            enqueue_L(B, BUF_X, PART_NONE, rec);
+            enqueue_row(B, KC_ROW_START, ROW_MODE_INIT_WARM, k, BUF_X, rec);
         }
-        enqueue_P_tail(g, nsys, IROW_MODE_FIRST, k, g->p.p, rec);
+        enqueue_P_tail(B, IROW_MODE_FIRST, k, BUF_P, rec);
     };
-    start(0, !warm);
+    start(0, !B.warm);
     for (int k = 0; k < maxit; k++) {
         if (k > 0 && k % 100 == 0) start(k, false);
-        enqueue_L(g, nsys, g->p.p, g->Ap.p, SC.part_pap, rec);
+        enqueue_L(B, BUF_P, PART_PAP, rec);
         const bool last = k + 1 == maxit;
-        rec(last ? KC_ROW_LAST : KC_ROW, [&] {
-            launch_row(G, SC, nsys, last ? ROW_MODE_LAST : ROW_MODE_UPDATE, g->rt64, g->spec64, k, g->x.p, g->r.p, g->b.p, g->p.p,
-                       g->Ap.p, g->specf.p, g->stop.p, rec.st);
-        });
+        enqueue_row(B, last ? KC_ROW_LAST : KC_ROW, last ? ROW_MODE_LAST : ROW_MODE_UPDATE, k, BUF_X, rec);
         if (last) break;
-        enqueue_P_tail(g, nsys, IROW_MODE_UPDATE, k, g->p.p, rec);
+        enqueue_P_tail(B, IROW_MODE_UPDATE, k, BUF_P, rec);
     }
-    rec(KC_FINAL, [&] { launch_final(G, SC, nsys, G.nrowblk, g->stop.p, rec.st); });
+    rec(KC_FINAL, [&] {
+        for (auto &s : B.fin) launch_final(tab(B, *s), rec.st);
+    });
 }
 
 void prof_collect(gpu_tomo *g) {
@@ -905,19 +1148,24 @@ int guarded(F &&f) {
     }
 }
 
-void to_dev_grad(gpu_tomo *g, int isys, const double *grad) {
-    // caller layout (all WFS, skipped ones empty) -> active-WFS layout
+// Caller layout (all WFS, skipped ones empty, double) -> active-WFS layout
+// (float); rejects non-finite input.
+void pack_grad(const gpu_tomo *g, const double *grad, float *dst) {
     const Geom &G = g->G;
-    g->hstage.assign(G.ngtot, 0.f);
     for (int k = 0; k < G.nwfs; k++) {
         const int w = g->wfs_active[k];
         const long o = g->goff_all[w];
         for (long i = 0; i < 2 * G.nsa[k]; i++) {
             const double v = grad[o + i];
             if (!std::isfinite(v)) fail(MT_ENOTFINITE, "non-finite gradient input");
-            g->hstage[G.goff[k] + i] = (float)v;
+            dst[G.goff[k] + i] = (float)v;
         }
     }
+}
+void to_dev_grad(gpu_tomo *g, int isys, const double *grad) {
+    const Geom &G = g->G;
+    g->hstage.assign(G.ngtot, 0.f);
+    pack_grad(g, grad, g->hstage.data());
     HIPCHK(hipMemcpyAsync(g->g.p + (long)isys * G.ngtot, g->hstage.data(), sizeof(float) * G.ngtot,
                           hipMemcpyHostToDevice, g->stream));
     HIPCHK(hipStreamSynchronize(g->stream));
@@ -995,6 +1243,7 @@ int gpu_tomo_update_cn2(gpu_tomo_t *g, const mt_recon_t *recon) {
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
         HIPCHK(hipStreamSynchronize(g->stream));
+        HIPCHK(hipDeviceSynchronize());
         if (recon->npsr != g->npsr) fail(MT_EINVAL, "layer count changed");
         for (int l = 0; l < recon->npsr; l++) g->G.l2c[l] = (float)recon->l2c[l];
         if (g->G.precond) {
@@ -1003,11 +1252,9 @@ int gpu_tomo_update_cn2(gpu_tomo_t *g, const mt_recon_t *recon) {
             std::vector<float2> hM = group_order(half_blocks(recon, g->G.fnx, g->G.fny), g->blk_order, bs);
             HIPCHK(hipMemcpy(g->Minv.p, hM.data(), sizeof(float2) * hM.size(), hipMemcpyHostToDevice));
         }
-        // kernel arguments changed: drop the captured graph
-        if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
-        if (g->graph) (void)hipGraphDestroy(g->graph);
-        g->gexec = nullptr;
-        g->graph = nullptr;
+        // in-place update of the device plan: tables and captured graphs,
+        // which point at it, stay valid
+        upload_plandev(g);
     });
 }
 
@@ -1024,8 +1271,8 @@ void gpu_tomo_free(gpu_tomo_t *g) {
     if (!g) return;
     (void)hipSetDevice(g->device);
     if (g->stream) (void)hipStreamSynchronize(g->stream);
-    if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
-    if (g->graph) (void)hipGraphDestroy(g->graph);
+    (void)hipDeviceSynchronize();
+    drop_batches(g);
     for (auto &e : g->evs) {
         (void)hipEventDestroy(e.second.first);
         (void)hipEventDestroy(e.second.second);
@@ -1046,12 +1293,11 @@ int gpu_tomo_solve(gpu_tomo_t *g, int isys, const double *grad, double *x, doubl
     double res = NAN;
     int rc = guarded([&] {
         HIPCHK(hipSetDevice(g->device));
-        if (g->nbatch != 1) fail(MT_EINVAL, "gpu_tomo_solve is the one-system entry point (nbatch must be 1)");
-        to_dev_grad(g, 0, grad);
-        Rec rec{g, g->stream, false};
-        enqueue_solve(g, 1, g->warm, rec);
-        from_dev_x(g, x, g->x.p);
-        HIPCHK(hipMemcpy(&res, g->res.p, sizeof(double), hipMemcpyDeviceToHost));
+        to_dev_grad(g, isys, grad);
+        Rec rec{&g->evs, g->stream, false};
+        enqueue_solve(own_batch(g, isys, 1), rec);
+        from_dev_x(g, x, g->x.p + (long)isys * g->G.nxtot);
+        HIPCHK(hipMemcpy(&res, g->res.p + isys, sizeof(double), hipMemcpyDeviceToHost));
     });
     if (cgres) *cgres = res;
     if (rc == MT_OK && !(res <= 1.0)) rc = MT_NOTCONV;  // cuda/solve.cu:23-32 warning case
@@ -1064,8 +1310,8 @@ int gpu_tomo_solve_batch(gpu_tomo_t *g, int nsys, const double *grad, double *x,
     int rc = guarded([&] {
         HIPCHK(hipSetDevice(g->device));
         for (int s = 0; s < nsys; s++) to_dev_grad(g, s, grad + (long)s * g->ngtot_all);
-        Rec rec{g, g->stream, false};
-        enqueue_solve(g, nsys, g->warm, rec);
+        Rec rec{&g->evs, g->stream, false};
+        enqueue_solve(own_batch(g, 0, nsys), rec);
         for (int s = 0; s < nsys; s++) from_dev_x(g, x + (long)s * g->nxtot, g->x.p + (long)s * g->G.nxtot);
         HIPCHK(hipMemcpy(res.data(), g->res.p, sizeof(double) * nsys, hipMemcpyDeviceToHost));
     });
@@ -1077,6 +1323,25 @@ int gpu_tomo_solve_batch(gpu_tomo_t *g, int nsys, const double *grad, double *x,
     return rc;
 }
 
+// Capture the solve of batch B on a private stream into (graph, exec).
+void capture_solve(Batch &B, hipGraph_t *graph, hipGraphExec_t *exec, EvList *evs) {
+    hipStream_t cs;
+    HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    Rec rec{evs, cs, false};
+    HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    try {
+        enqueue_solve(B, rec);
+    } catch (...) {
+        hipGraph_t junk;
+        (void)hipStreamEndCapture(cs, &junk);
+        (void)hipStreamDestroy(cs);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(cs, graph));
+    (void)hipStreamDestroy(cs);
+    HIPCHK(hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0));
+}
+
 int gpu_tomo_solve_dev(gpu_tomo_t *g, const float *d_grad, float *d_x, int nbatch, int mode, void *stream) {
     if (!g || nbatch <= 0 || nbatch > g->nbatch) return MT_EINVAL;
     return guarded([&] {
@@ -1085,33 +1350,20 @@ int gpu_tomo_solve_dev(gpu_tomo_t *g, const float *d_grad, float *d_x, int nbatc
         const Geom &G = g->G;
         if (d_grad && d_grad != g->g.p) launch_copy(d_grad, g->g.p, (long)nbatch * G.ngtot, st);
         if (d_x && d_x != g->x.p && g->warm) launch_copy(d_x, g->x.p, (long)nbatch * G.nxtot, st);
+        Batch &B = own_batch(g, 0, nbatch);
         if (mode == 1 && !g->prof) {
             if (!g->gexec || g->graph_nsys != nbatch) {
                 if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
                 if (g->graph) (void)hipGraphDestroy(g->graph);
                 g->gexec = nullptr;
                 g->graph = nullptr;
-                hipStream_t cs;
-                HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-                Rec rec{g, cs, false};
-                HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-                try {
-                    enqueue_solve(g, nbatch, g->warm, rec);
-                } catch (...) {
-                    hipGraph_t junk;
-                    (void)hipStreamEndCapture(cs, &junk);
-                    (void)hipStreamDestroy(cs);
-                    throw;
-                }
-                HIPCHK(hipStreamEndCapture(cs, &g->graph));
-                (void)hipStreamDestroy(cs);
-                HIPCHK(hipGraphInstantiate(&g->gexec, g->graph, nullptr, nullptr, 0));
+                capture_solve(B, &g->graph, &g->gexec, &g->evs);
                 g->graph_nsys = nbatch;
             }
             HIPCHK(hipGraphLaunch(g->gexec, st));
         } else {
-            Rec rec{g, st, g->prof != 0};
-            enqueue_solve(g, nbatch, g->warm, rec);
+            Rec rec{&g->evs, st, g->prof != 0};
+            enqueue_solve(B, rec);
             if (g->prof) {
                 if (st != g->stream) HIPCHK(hipStreamSynchronize(st));
                 prof_collect(g);
@@ -1135,8 +1387,8 @@ int gpu_tomo_L(gpu_tomo_t *g, const double *xin, double *xout, double alpha, dou
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
         to_dev_x(g, g->z.p, xin);
-        Rec rec{g, g->stream, false};
-        enqueue_L(g, 1, g->z.p, g->Ap.p, nullptr, rec);
+        Rec rec{&g->evs, g->stream, false};
+        enqueue_L(own_batch(g, 0, 1), BUF_Z, PART_NONE, rec);
         std::vector<double> tmp(g->nxtot);
         from_dev_rt(g, tmp.data(), g->Ap.p);
         for (long i = 0; i < g->nxtot; i++) xout[i] = beta * xout[i] + alpha * tmp[i];
@@ -1148,8 +1400,8 @@ int gpu_tomo_R(gpu_tomo_t *g, const double *grad, double *xout, double alpha) {
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
         to_dev_grad(g, 0, grad);
-        Rec rec{g, g->stream, false};
-        enqueue_R(g, 1, g->g.p, g->b.p, rec);
+        Rec rec{&g->evs, g->stream, false};
+        enqueue_R(own_batch(g, 0, 1), rec);
         from_dev_rt(g, xout, g->b.p);
         for (long i = 0; i < g->nxtot; i++) xout[i] *= alpha;
     });
@@ -1163,13 +1415,89 @@ int gpu_tomo_P(gpu_tomo_t *g, const double *xin, double *xout) {
         // b <- xin, r <- b through the cold-start row kernel (which also
         // clears its x argument: the p buffer is used as that scratch), then P
         to_dev_rt(g, g->b.p, xin);
-        Rec rec{g, g->stream, false};
-        launch_zero_i(g->stop.p, 1, g->stream);
-        launch_row(g->G, g->SC, 1, ROW_MODE_INIT_COLD, g->rt64, g->spec64, 0, g->p.p, g->r.p, g->b.p, g->p.p, g->Ap.p,
-                   g->specf.p, g->stop.p, g->stream);
-        enqueue_P_tail(g, 1, IROW_MODE_Z, 0, g->z.p, rec);
+        Batch &B = own_batch(g, 0, 1);
+        Rec rec{&g->evs, g->stream, false};
+        for (auto &s : B.zero) launch_zero_stop(tab(B, *s), g->stream);
+        enqueue_row(B, KC_ROW_START, ROW_MODE_INIT_COLD, 0, BUF_P, rec);
+        enqueue_P_tail(B, IROW_MODE_Z, 0, BUF_Z, rec);
         from_dev_x(g, xout, g->z.p);
     });
+}
+
+int gpu_tomo_set_grad(gpu_tomo_t *g, int isys, const double *grad) {
+    if (!g || !grad || isys < 0 || isys >= g->nbatch) return MT_EINVAL;
+    return guarded([&] {
+        HIPCHK(hipSetDevice(g->device));
+        to_dev_grad(g, isys, grad);
+    });
+}
+
+int gpu_tomo_get_x(gpu_tomo_t *g, int isys, double *x, double *cgres) {
+    if (!g || !x || isys < 0 || isys >= g->nbatch) return MT_EINVAL;
+    return guarded([&] {
+        HIPCHK(hipSetDevice(g->device));
+        HIPCHK(hipDeviceSynchronize());
+        from_dev_x(g, x, g->x.p + (long)isys * g->G.nxtot);
+        if (cgres) HIPCHK(hipMemcpy(cgres, g->res.p + isys, sizeof(double), hipMemcpyDeviceToHost));
+    });
+}
+
+}  // extern "C"
+
+struct gpu_tomo_group {
+    std::unique_ptr<Batch> B;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipStream_t stream = nullptr;
+    int device = 0;
+    EvList evs;
+};
+
+extern "C" {
+
+int gpu_tomo_group_create(int n, gpu_tomo_t *const *plans, const int *s0, const int *nsys,
+                          gpu_tomo_group_t **out) {
+    if (!out || n <= 0 || !plans || !s0 || !nsys) return MT_EINVAL;
+    *out = nullptr;
+    std::unique_ptr<gpu_tomo_group> grp(new (std::nothrow) gpu_tomo_group());
+    if (!grp) return MT_ENOMEM;
+    int rc = guarded([&] {
+        std::vector<gpu_tomo *> pl(plans, plans + n);
+        for (auto *g : pl)
+            if (!g) fail(MT_EINVAL, "null plan");
+        grp->device = pl[0]->device;
+        HIPCHK(hipSetDevice(grp->device));
+        grp->B = make_batch(pl, std::vector<int>(s0, s0 + n), std::vector<int>(nsys, nsys + n));
+        HIPCHK(hipStreamCreateWithFlags(&grp->stream, hipStreamNonBlocking));
+    });
+    if (rc == MT_OK) *out = grp.release();
+    return rc;
+}
+
+int gpu_tomo_group_solve(gpu_tomo_group_t *grp, int mode, void *stream) {
+    if (!grp) return MT_EINVAL;
+    return guarded([&] {
+        HIPCHK(hipSetDevice(grp->device));
+        hipStream_t st = stream ? (hipStream_t)stream : grp->stream;
+        if (mode == 1) {
+            if (!grp->gexec) capture_solve(*grp->B, &grp->graph, &grp->gexec, &grp->evs);
+            HIPCHK(hipGraphLaunch(grp->gexec, st));
+        } else {
+            Rec rec{&grp->evs, st, false};
+            enqueue_solve(*grp->B, rec);
+        }
+        if (!stream) HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+void gpu_tomo_group_free(gpu_tomo_group_t *grp) {
+    if (!grp) return;
+    (void)hipSetDevice(grp->device);
+    (void)hipDeviceSynchronize();
+    if (grp->gexec) (void)hipGraphExecDestroy(grp->gexec);
+    if (grp->graph) (void)hipGraphDestroy(grp->graph);
+    if (grp->stream) (void)hipStreamDestroy(grp->stream);
+    delete grp;
 }
 
 int gpu_tomo_precision(gpu_tomo_t *g, int mode) {
@@ -1177,12 +1505,10 @@ int gpu_tomo_precision(gpu_tomo_t *g, int mode) {
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
         HIPCHK(hipStreamSynchronize(g->stream));
+        HIPCHK(hipDeviceSynchronize());
         g->rt64 = mode >= 1;
         g->spec64 = mode == 1;
-        if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
-        if (g->graph) (void)hipGraphDestroy(g->graph);
-        g->gexec = nullptr;
-        g->graph = nullptr;
+        drop_batches(g);
     });
 }
 
@@ -1205,6 +1531,407 @@ int gpu_tomo_profile_read(gpu_tomo_t *g, const char **names, double *ms, double 
     }
     *n = k;
     return MT_OK;
+}
+
+}  // extern "C"
+
+// ===========================================================================
+// Request loop (continuous batching).  Each AO loop owns a plan with one
+// warm-start slot.  Every stream ("flight") holds a batch over all admitted
+// loops, captured once in a hipGraph; a launch switches on the jobs of the
+// loops that have a request in it (the others' workgroups exit at entry).
+// A worker thread collects the pending requests of loops that are not on the
+// GPU, launches them on a free flight and retires flights as their events
+// complete, so requests that arrive while others run are admitted as soon
+// as a stream and their loop are free, and flights overlap on the device.
+// Loops are admitted by HBM footprint against a budget.
+// ===========================================================================
+namespace {
+typedef std::chrono::steady_clock Clock;
+
+struct SrvLoop {
+    gpu_tomo *plan = nullptr;
+    size_t bytes = 0;
+    bool busy = false;                 // a request of this loop is on the GPU
+    float *hgrad = nullptr, *hx = nullptr;  // pinned staging
+    double *hres = nullptr;            // pinned
+    std::vector<double> xlast;         // last completed solution (host)
+    double reslast = NAN;
+};
+struct SrvReq {
+    int loop;
+    long tag;
+    std::vector<float> grad;
+    Clock::time_point t_submit;
+};
+struct SrvDone {
+    int loop;
+    long tag;
+    double res, lat_us;
+};
+struct Flight {
+    hipStream_t st = nullptr;
+    hipEvent_t done = nullptr;
+    bool active = false;
+    std::vector<SrvReq> reqs;
+    std::vector<SrvLoop *> lp;   // the requests' loops
+    // batch over all admitted loops (rebuilt when loops change)
+    long gen = -1;
+    std::unique_ptr<Batch> B;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    Job *hjobs = nullptr;        // pinned copy of the job table (on flags set per launch)
+    std::vector<int> job_of;     // loop id -> job index (-1: not in the batch)
+};
+size_t plan_bytes(const gpu_tomo *g) {
+    size_t b = 0;
+    b += g->hx_nx.n * sizeof(int2) + g->hx_ny.n * sizeof(int2) + (g->hx_fx.n + g->hx_fy.n) * sizeof(float);
+    b += (g->hxt_x.n + g->hxt_y.n) * sizeof(int4) + g->mray.n * sizeof(MRay) + g->saidx.n * sizeof(int);
+    b += (g->gpx.n + g->gpy.n + g->ptt.n) * sizeof(float) + (g->neai.n + g->nead.n) * sizeof(float4);
+    b += (g->gpd.n + g->twx.n + g->twy.n + g->Minv.n) * sizeof(float2) + (g->twx64.n + g->twy64.n) * sizeof(double2);
+    b += (g->grp_col.n + g->colkx.n + g->grp_blk.n + g->blkpos.n) * sizeof(int) + g->rowblk.n * sizeof(int2);
+    b += (g->x.n + g->z.n + g->p.n + g->g.n + g->tt.n) * sizeof(float);
+    b += (g->r.n + g->Ap.n + g->b.n + g->q.n) * sizeof(double) + g->specf.n * sizeof(double2) + g->speci.n * sizeof(float2);
+    b += (g->part_pap.n + g->part_rz.n + g->part_rr.n + g->part_bb.n + g->rz.n + g->alpha.n + g->res.n) * sizeof(double);
+    return b;
+}
+}  // namespace
+
+struct gpu_tomo_server {
+    int device = 0, max_inflight = 2, max_batch = 64, use_graph = 1;
+    size_t budget = 0, used = 0;
+    std::vector<std::unique_ptr<SrvLoop>> loops;  // index = loop id (plan null once removed)
+    std::deque<SrvReq> pending;
+    std::vector<SrvDone> done;
+    std::vector<Flight> flights;
+    long gen = 0;                      // bumped when loops are added or removed
+    std::mutex mu;
+    std::condition_variable cv, cv_done;
+    std::thread worker;
+    std::atomic<bool> stop{false};
+    long inflight_reqs = 0;
+    std::string err;
+    EvList evs;
+};
+
+namespace {
+void flight_drop(Flight &f) {
+    if (f.exec) (void)hipGraphExecDestroy(f.exec);
+    if (f.graph) (void)hipGraphDestroy(f.graph);
+    f.exec = nullptr;
+    f.graph = nullptr;
+    if (f.hjobs) (void)hipHostFree(f.hjobs);
+    f.hjobs = nullptr;
+    f.B.reset();
+}
+// (Re)build an idle flight's batch over all admitted loops (lock held).
+void flight_build(gpu_tomo_server *S, Flight &f) {
+    flight_drop(f);
+    std::vector<gpu_tomo *> pl;
+    f.job_of.assign(S->loops.size(), -1);
+    for (size_t i = 0; i < S->loops.size(); i++)
+        if (S->loops[i]->plan) {
+            f.job_of[i] = (int)pl.size();
+            pl.push_back(S->loops[i]->plan);
+        }
+    f.gen = S->gen;
+    if (pl.empty()) return;
+    f.B = make_batch(pl, std::vector<int>(pl.size(), 0), std::vector<int>(pl.size(), 1));
+    HIPCHK(hipHostMalloc((void **)&f.hjobs, sizeof(Job) * pl.size()));
+    if (S->use_graph) capture_solve(*f.B, &f.graph, &f.exec, &S->evs);
+}
+
+// Launch the requests in f.reqs on f.st (worker thread; f.lp resolved and
+// the batch built under the lock).
+void srv_launch(gpu_tomo_server *S, Flight &f) {
+    Batch &B = *f.B;
+    for (size_t j = 0; j < B.hjob.size(); j++) {
+        f.hjobs[j] = B.hjob[j];
+        f.hjobs[j].on = 0;
+    }
+    for (auto &r : f.reqs) f.hjobs[f.job_of[r.loop]].on = 1;
+    HIPCHK(hipMemcpyAsync(B.jobs.p, f.hjobs, sizeof(Job) * B.hjob.size(), hipMemcpyHostToDevice, f.st));
+    for (size_t i = 0; i < f.reqs.size(); i++) {
+        SrvLoop &L = *f.lp[i];
+        std::memcpy(L.hgrad, f.reqs[i].grad.data(), sizeof(float) * f.reqs[i].grad.size());
+        HIPCHK(hipMemcpyAsync(L.plan->g.p, L.hgrad, sizeof(float) * f.reqs[i].grad.size(), hipMemcpyHostToDevice,
+                              f.st));
+    }
+    if (S->use_graph) {
+        HIPCHK(hipGraphLaunch(f.exec, f.st));
+    } else {
+        Rec rec{&S->evs, f.st, false};
+        enqueue_solve(B, rec);
+    }
+    for (size_t i = 0; i < f.reqs.size(); i++) {
+        SrvLoop &L = *f.lp[i];
+        HIPCHK(hipMemcpyAsync(L.hx, L.plan->x.p, sizeof(float) * L.plan->G.nxtot, hipMemcpyDeviceToHost, f.st));
+        HIPCHK(hipMemcpyAsync(L.hres, L.plan->res.p, sizeof(double), hipMemcpyDeviceToHost, f.st));
+    }
+    HIPCHK(hipEventRecord(f.done, f.st));
+    f.active = true;
+}
+
+void srv_worker(gpu_tomo_server *S) {
+    (void)hipSetDevice(S->device);
+    try {
+        while (!S->stop) {
+            bool did = false;
+            // retire completed flights
+            for (auto &f : S->flights) {
+                if (!f.active) continue;
+                const hipError_t q = hipEventQuery(f.done);
+                if (q == hipErrorNotReady) continue;
+                HIPCHK(q);
+                const auto now = Clock::now();
+                std::lock_guard<std::mutex> lk(S->mu);
+                for (size_t i = 0; i < f.reqs.size(); i++) {
+                    SrvLoop &L = *f.lp[i];
+                    const SrvReq &r = f.reqs[i];
+                    L.xlast.assign(L.hx, L.hx + L.plan->G.nxtot);
+                    L.reslast = *L.hres;
+                    L.busy = false;
+                    S->done.push_back(SrvDone{r.loop, r.tag, L.reslast,
+                                              std::chrono::duration<double, std::micro>(now - r.t_submit).count()});
+                }
+                S->inflight_reqs -= (long)f.reqs.size();
+                f.reqs.clear();
+                f.lp.clear();
+                f.active = false;
+                did = true;
+                S->cv_done.notify_all();
+            }
+            // admit pending requests of idle loops into a free flight
+            for (auto &f : S->flights) {
+                if (f.active) continue;
+                {
+                    std::lock_guard<std::mutex> lk(S->mu);
+                    std::vector<int> taken;
+                    for (auto it = S->pending.begin(); it != S->pending.end() && (int)f.reqs.size() < S->max_batch;) {
+                        SrvLoop &L = *S->loops[it->loop];
+                        if (L.busy || std::find(taken.begin(), taken.end(), it->loop) != taken.end()) {
+                            ++it;
+                            continue;
+                        }
+                        L.busy = true;
+                        taken.push_back(it->loop);
+                        f.lp.push_back(&L);
+                        f.reqs.push_back(std::move(*it));
+                        it = S->pending.erase(it);
+                    }
+                    S->inflight_reqs += (long)f.reqs.size();
+                    if (!f.reqs.empty() && f.gen != S->gen) flight_build(S, f);
+                }
+                if (f.reqs.empty()) break;
+                srv_launch(S, f);
+                did = true;
+            }
+            if (!did) {
+                std::unique_lock<std::mutex> lk(S->mu);
+                bool anyactive = false;
+                for (auto &f : S->flights) anyactive |= f.active;
+                if (anyactive) {
+                    lk.unlock();
+                    std::this_thread::sleep_for(std::chrono::microseconds(5));
+                } else {
+                    S->cv.wait_for(lk, std::chrono::milliseconds(1));
+                }
+            }
+        }
+    } catch (const HipError &e) {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->err = e.msg;
+        S->stop = true;
+        S->cv_done.notify_all();
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int gpu_tomo_server_create(int device, double mem_budget_gb, int max_inflight, int max_batch, int use_graph,
+                           gpu_tomo_server_t **out) {
+    if (!out || max_inflight < 1 || max_inflight > 8 || max_batch < 1 || mem_budget_gb <= 0) return MT_EINVAL;
+    *out = nullptr;
+    std::unique_ptr<gpu_tomo_server> S(new (std::nothrow) gpu_tomo_server());
+    if (!S) return MT_ENOMEM;
+    int rc = guarded([&] {
+        int ndev = 0;
+        HIPCHK(hipGetDeviceCount(&ndev));
+        if (device < 0 || device >= ndev) fail(MT_EINVAL, "no such device");
+        S->device = device;
+        HIPCHK(hipSetDevice(device));
+        S->budget = (size_t)(mem_budget_gb * 1e9);
+        S->max_inflight = max_inflight;
+        S->max_batch = max_batch;
+        S->use_graph = use_graph ? 1 : 0;
+        S->flights.resize(max_inflight);
+        for (auto &f : S->flights) {
+            HIPCHK(hipStreamCreateWithFlags(&f.st, hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&f.done, hipEventDisableTiming));
+        }
+        S->worker = std::thread(srv_worker, S.get());
+    });
+    if (rc == MT_OK) *out = S.release();
+    return rc;
+}
+
+int gpu_tomo_server_add_loop(gpu_tomo_server_t *S, const mt_recon_t *recon, int maxit, int *loop) {
+    if (!S || !recon || !loop) return MT_EINVAL;
+    {
+        // loops of one server advance in lockstep batches: one iteration count
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (auto &L : S->loops)
+            if (L->plan && L->plan->maxit != maxit) {
+                mt::set_error("all loops of a server use the same maxit");
+                return MT_EINVAL;
+            }
+    }
+    gpu_tomo_t *g = nullptr;
+    int rc = gpu_tomo_setup(recon, S->device, 1, maxit, 1, &g);
+    if (rc != MT_OK) return rc;
+    const size_t bytes = plan_bytes(g);
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (S->used + bytes > S->budget) {
+        gpu_tomo_free(g);
+        mt::set_error("loop refused: HBM budget exceeded");
+        return MT_EFULL;
+    }
+    rc = guarded([&] {
+        std::unique_ptr<SrvLoop> L(new SrvLoop());
+        L->plan = g;
+        L->bytes = bytes;
+        HIPCHK(hipHostMalloc((void **)&L->hgrad, sizeof(float) * g->G.ngtot));
+        HIPCHK(hipHostMalloc((void **)&L->hx, sizeof(float) * g->G.nxtot));
+        HIPCHK(hipHostMalloc((void **)&L->hres, sizeof(double)));
+        std::memset(L->hgrad, 0, sizeof(float) * g->G.ngtot);
+        S->used += bytes;
+        S->loops.push_back(std::move(L));
+        *loop = (int)S->loops.size() - 1;
+        S->gen++;
+    });
+    if (rc != MT_OK) gpu_tomo_free(g);
+    return rc;
+}
+
+int gpu_tomo_server_remove_loop(gpu_tomo_server_t *S, int loop) {
+    if (!S) return MT_EINVAL;
+    std::unique_lock<std::mutex> lk(S->mu);
+    if (loop < 0 || loop >= (int)S->loops.size() || !S->loops[loop]->plan) return MT_EINVAL;
+    for (auto &r : S->pending)
+        if (r.loop == loop) return MT_ESTATE;
+    if (S->loops[loop]->busy) return MT_ESTATE;
+    SrvLoop &L = *S->loops[loop];
+    // flights still holding a batch with this loop switch it off (its job is
+    // never on again) and rebuild before their next launch
+    S->gen++;
+    (void)hipHostFree(L.hgrad);
+    (void)hipHostFree(L.hx);
+    (void)hipHostFree(L.hres);
+    gpu_tomo_free(L.plan);
+    S->used -= L.bytes;
+    L = SrvLoop();
+    return MT_OK;
+}
+
+int gpu_tomo_server_submit(gpu_tomo_server_t *S, int loop, const double *grad, long tag) {
+    if (!S || !grad) return MT_EINVAL;
+    SrvReq r;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        if (!S->err.empty()) {
+            mt::set_error(S->err);
+            return MT_EHIP;
+        }
+        if (loop < 0 || loop >= (int)S->loops.size() || !S->loops[loop]->plan) return MT_EINVAL;
+        r.grad.assign(S->loops[loop]->plan->G.ngtot, 0.f);
+        const gpu_tomo *g = S->loops[loop]->plan;
+        int rc = guarded([&] { pack_grad(g, grad, r.grad.data()); });
+        if (rc != MT_OK) return rc;
+    }
+    r.loop = loop;
+    r.tag = tag;
+    r.t_submit = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->pending.push_back(std::move(r));
+    }
+    S->cv.notify_one();
+    return MT_OK;
+}
+
+int gpu_tomo_server_poll(gpu_tomo_server_t *S, int maxn, int *loop, long *tag, double *cgres, double *latency_us,
+                         int *n) {
+    if (!S || !n) return MT_EINVAL;
+    std::lock_guard<std::mutex> lk(S->mu);
+    int k = 0;
+    while (k < maxn && !S->done.empty()) {
+        const SrvDone d = S->done.front();
+        S->done.erase(S->done.begin());
+        if (loop) loop[k] = d.loop;
+        if (tag) tag[k] = d.tag;
+        if (cgres) cgres[k] = d.res;
+        if (latency_us) latency_us[k] = d.lat_us;
+        k++;
+    }
+    *n = k;
+    if (!S->err.empty()) {
+        mt::set_error(S->err);
+        return MT_EHIP;
+    }
+    return MT_OK;
+}
+
+int gpu_tomo_server_drain(gpu_tomo_server_t *S) {
+    if (!S) return MT_EINVAL;
+    std::unique_lock<std::mutex> lk(S->mu);
+    S->cv_done.wait(lk, [&] { return (S->pending.empty() && S->inflight_reqs == 0) || !S->err.empty(); });
+    if (!S->err.empty()) {
+        mt::set_error(S->err);
+        return MT_EHIP;
+    }
+    return MT_OK;
+}
+
+int gpu_tomo_server_get_x(gpu_tomo_server_t *S, int loop, double *x, double *cgres) {
+    if (!S || !x) return MT_EINVAL;
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (loop < 0 || loop >= (int)S->loops.size() || !S->loops[loop]->plan) return MT_EINVAL;
+    const SrvLoop &L = *S->loops[loop];
+    if (L.xlast.empty()) return MT_ESTATE;
+    std::copy(L.xlast.begin(), L.xlast.end(), x);
+    if (cgres) *cgres = L.reslast;
+    return MT_OK;
+}
+
+int gpu_tomo_server_usage(gpu_tomo_server_t *S, double *used_gb, double *budget_gb) {
+    if (!S) return MT_EINVAL;
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (used_gb) *used_gb = S->used / 1e9;
+    if (budget_gb) *budget_gb = S->budget / 1e9;
+    return MT_OK;
+}
+
+void gpu_tomo_server_free(gpu_tomo_server_t *S) {
+    if (!S) return;
+    S->stop = true;
+    S->cv.notify_all();
+    if (S->worker.joinable()) S->worker.join();
+    (void)hipSetDevice(S->device);
+    (void)hipDeviceSynchronize();
+    for (auto &f : S->flights) flight_drop(f);
+    for (auto &L : S->loops) {
+        if (!L->plan) continue;
+        (void)hipHostFree(L->hgrad);
+        (void)hipHostFree(L->hx);
+        (void)hipHostFree(L->hres);
+        gpu_tomo_free(L->plan);
+    }
+    for (auto &f : S->flights) {
+        if (f.done) (void)hipEventDestroy(f.done);
+        if (f.st) (void)hipStreamDestroy(f.st);
+    }
+    delete S;
 }
 
 }  // extern "C"

@@ -169,14 +169,15 @@ class GpuTomo:
         except Exception:
             pass
 
-    def tomo(self, grad):
-        """gpu_tomo: one solve from host gradients. Returns (x, cgres, status)."""
+    def tomo(self, grad, isys=0):
+        """gpu_tomo: one solve of slot isys from host gradients.
+        Returns (x, cgres, status)."""
         g = np.ascontiguousarray(grad, dtype=np.float64)
         if g.size != self.ng:
             raise ValueError("gradient vector has %d entries, expected %d" % (g.size, self.ng))
         x = np.zeros(self.nx)
         res = C.c_double()
-        rc = _abi.lib().gpu_tomo_solve(self._h, 0, _dp(g), _dp(x), C.byref(res))
+        rc = _abi.lib().gpu_tomo_solve(self._h, int(isys), _dp(g), _dp(x), C.byref(res))
         _abi.check(rc, "gpu_tomo_solve")
         return x, res.value, rc
 
@@ -192,6 +193,20 @@ class GpuTomo:
         _abi.check(_abi.lib().gpu_tomo_solve_batch(self._h, g.shape[0], _dp(g), _dp(x), _dp(res)),
                    "gpu_tomo_solve_batch")
         return x
+
+    def set_grad(self, isys, grad):
+        """Host gradients into system slot isys (gpu_tomo_set_grad)."""
+        g = np.ascontiguousarray(grad, dtype=np.float64)
+        if g.size != self.ng:
+            raise ValueError("gradient vector has %d entries, expected %d" % (g.size, self.ng))
+        _abi.check(_abi.lib().gpu_tomo_set_grad(self._h, int(isys), _dp(g)), "gpu_tomo_set_grad")
+
+    def get_x(self, isys):
+        """Solution and residual of slot isys after a solve (gpu_tomo_get_x)."""
+        x = np.zeros(self.nx)
+        res = C.c_double()
+        _abi.check(_abi.lib().gpu_tomo_get_x(self._h, int(isys), _dp(x), C.byref(res)), "gpu_tomo_get_x")
+        return x, res.value
 
     def reset(self):
         _abi.check(_abi.lib().gpu_tomo_reset(self._h), "gpu_tomo_reset")
@@ -240,3 +255,103 @@ class GpuTomo:
         calls = (C.c_double * 16)()
         _abi.lib().gpu_tomo_profile_read(self._h, names, ms, calls, C.byref(n))
         return {names[i].decode(): (ms[i], int(calls[i])) for i in range(n.value)}
+
+
+class GpuGroup:
+    """Uneven batch (gpu_tomo_group_*): system ranges of several plans of
+    any geometry solved in lockstep, one launch per kernel per iteration."""
+
+    def __init__(self, members):
+        """members: list of (GpuTomo, s0, nsys)."""
+        self.members = list(members)
+        n = len(self.members)
+        plans = (C.c_void_p * n)(*[m[0]._h.value for m in self.members])
+        s0 = (C.c_int * n)(*[int(m[1]) for m in self.members])
+        ns = (C.c_int * n)(*[int(m[2]) for m in self.members])
+        h = C.c_void_p()
+        _abi.check(_abi.lib().gpu_tomo_group_create(n, plans, s0, ns, C.byref(h)), "gpu_tomo_group_create")
+        self._h = h
+
+    def solve(self, mode=0, stream=None):
+        _abi.check(_abi.lib().gpu_tomo_group_solve(self._h, int(mode), C.c_void_p(stream or 0)),
+                   "gpu_tomo_group_solve")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _abi.lib().gpu_tomo_group_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Request loop with continuous batching (gpu_tomo_server_*)."""
+
+    def __init__(self, device=0, mem_budget_gb=16.0, max_inflight=2, max_batch=64, use_graph=True):
+        h = C.c_void_p()
+        _abi.check(_abi.lib().gpu_tomo_server_create(device, float(mem_budget_gb), int(max_inflight),
+                                                     int(max_batch), 1 if use_graph else 0, C.byref(h)),
+                   "gpu_tomo_server_create")
+        self._h = h
+        self.nx = {}
+        self.ng = {}
+
+    def add_loop(self, recon, maxit=10):
+        """Admit an AO loop (warm start, `maxit` iterations per step); raises
+        MtError with code MT_EFULL (-6) when the HBM budget is exceeded."""
+        lid = C.c_int()
+        _abi.check(_abi.lib().gpu_tomo_server_add_loop(self._h, C.byref(recon.rec), int(maxit), C.byref(lid)),
+                   "gpu_tomo_server_add_loop")
+        self.nx[lid.value] = recon.nx
+        self.ng[lid.value] = recon.ng
+        return lid.value
+
+    def remove_loop(self, lid):
+        _abi.check(_abi.lib().gpu_tomo_server_remove_loop(self._h, int(lid)), "gpu_tomo_server_remove_loop")
+
+    def submit(self, lid, grad, tag=0):
+        g = np.ascontiguousarray(grad, dtype=np.float64)
+        if g.size != self.ng[lid]:
+            raise ValueError("gradient vector has %d entries, expected %d" % (g.size, self.ng[lid]))
+        _abi.check(_abi.lib().gpu_tomo_server_submit(self._h, int(lid), _dp(g), int(tag)), "gpu_tomo_server_submit")
+
+    def poll(self, maxn=1024):
+        """Completed requests since the last poll: list of (loop, tag, cgres, latency_us)."""
+        loop = (C.c_int * maxn)()
+        tag = (C.c_long * maxn)()
+        res = (C.c_double * maxn)()
+        lat = (C.c_double * maxn)()
+        n = C.c_int()
+        _abi.check(_abi.lib().gpu_tomo_server_poll(self._h, maxn, loop, tag, res, lat, C.byref(n)),
+                   "gpu_tomo_server_poll")
+        return [(loop[i], tag[i], res[i], lat[i]) for i in range(n.value)]
+
+    def drain(self):
+        _abi.check(_abi.lib().gpu_tomo_server_drain(self._h), "gpu_tomo_server_drain")
+
+    def get_x(self, lid):
+        x = np.zeros(self.nx[lid])
+        res = C.c_double()
+        _abi.check(_abi.lib().gpu_tomo_server_get_x(self._h, int(lid), _dp(x), C.byref(res)),
+                   "gpu_tomo_server_get_x")
+        return x, res.value
+
+    def usage(self):
+        u, b = C.c_double(), C.c_double()
+        _abi.lib().gpu_tomo_server_usage(self._h, C.byref(u), C.byref(b))
+        return u.value, b.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _abi.lib().gpu_tomo_server_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
