@@ -286,6 +286,7 @@ def main():
     if args.precision is not None:
         plain.precision(args.precision)
         batched.precision(args.precision)
+    tuned = batched.autotune(B)
     empty = dict(solves_per_s=float("nan"), p50_ms=float("nan"), p99_ms=float("nan"), ms_per_step=float("nan"))
     res_plain = run_mode(plain, 1, 0, args.steps, args.warmup) if "plain" in modes else empty
     res_graph = run_mode(plain, 1, 1, args.steps, args.warmup) if "graph" in modes else empty
@@ -354,7 +355,8 @@ def main():
         "data": "synthetic (seeded von Karman screens, geometric SH slopes + 0.05\" noise)",
         "config": {"workload": "large30: D=30 m, 6 LGS at 90 km, 6 layers 0-16 km, 60x60 subaps, "
                                "xloc 6x96x96, 30 FDPCG-PCG iterations, cold start",
-                   "batch": B, "global_batch": B, "parallelism": "batch of independent solves"},
+                   "batch": B, "global_batch": B, "parallelism": "batch of independent solves",
+                   "autotuned_systems_per_workgroup": {"wfs": tuned[0], "layer": tuned[1], "col": tuned[2]}},
         "p50_ms": round(res_batch["p50_ms"], 4),
         "p99_ms": round(res_batch["p99_ms"], 4),
         "modes": {"plain": res_plain, "graph": res_graph, "batched": res_batch},

@@ -314,6 +314,18 @@ int  gpu_tomo_server_usage(gpu_tomo_server_t *s, double *used_gb,
                            double *budget_gb);
 void gpu_tomo_server_free(gpu_tomo_server_t *s);
 
+/* Autotuner: times the solve of nsys systems under each choice of systems
+ * per workgroup (1, 2, 4) of the WFS, layer and column kernels and keeps the
+ * fastest for this plan (choice[3] receives wfs, layer, col).  Overwrites
+ * the gradients and solutions of slots [0, nsys).  No reference
+ * counterpart. */
+int  gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice);
+
+/* Plan facts: matched-sampling ray path (1) or generic ray tables (0), FFT
+ * sizes, FDPCG block size, largest FFT radix. */
+int  gpu_tomo_info(const gpu_tomo_t *g, int *matched, int *fnx, int *fny,
+                   int *bs, int *rmax);
+
 /* Sizes of one system's vectors. */
 int  gpu_tomo_sizes(const gpu_tomo_t *g, long *nx, long *ng);
 

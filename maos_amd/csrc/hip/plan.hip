@@ -1281,6 +1281,16 @@ void gpu_tomo_free(gpu_tomo_t *g) {
     delete g;
 }
 
+int gpu_tomo_info(const gpu_tomo_t *g, int *matched, int *fnx, int *fny, int *bs, int *rmax) {
+    if (!g) return MT_EINVAL;
+    if (matched) *matched = g->G.matched;
+    if (fnx) *fnx = g->G.fnx;
+    if (fny) *fny = g->G.fny;
+    if (bs) *bs = g->G.bs;
+    if (rmax) *rmax = g->G.rmax;
+    return MT_OK;
+}
+
 int gpu_tomo_sizes(const gpu_tomo_t *g, long *nx, long *ng) {
     if (!g) return MT_EINVAL;
     if (nx) *nx = g->nxtot;
@@ -1498,6 +1508,62 @@ void gpu_tomo_group_free(gpu_tomo_group_t *grp) {
     if (grp->graph) (void)hipGraphDestroy(grp->graph);
     if (grp->stream) (void)hipStreamDestroy(grp->stream);
     delete grp;
+}
+
+int gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice) {
+    if (!g || nsys <= 0 || nsys > g->nbatch) return MT_EINVAL;
+    return guarded([&] {
+        HIPCHK(hipSetDevice(g->device));
+        HIPCHK(hipDeviceSynchronize());
+        // representative input: a small constant slope on every subaperture
+        // (the work of a solve does not depend on the values); the slots'
+        // previous gradients and solutions are overwritten
+        float v = 1e-7f;
+        unsigned bits;
+        std::memcpy(&bits, &v, 4);
+        HIPCHK(hipMemsetD32((hipDeviceptr_t)g->g.p, (int)bits, (size_t)nsys * g->G.ngtot));
+        const int save_prof = g->prof;
+        g->prof = 1;
+        // time one stage class under each candidate; other stages keep their choice
+        auto stage_ms = [&](int kc) {
+            drop_batches(g);
+            Batch &B = own_batch(g, 0, nsys);
+            double best = 1e30;
+            for (int rep = 0; rep < 3; rep++) {
+                Rec rec{&g->evs, g->stream, true};
+                enqueue_solve(B, rec);
+                prof_collect(g);
+                best = std::min(best, g->prof_ms[kc]);
+            }
+            return best;
+        };
+        struct Knob {
+            int *field, kc, cmax;
+        };
+        const int colmax = g->spec64 ? 2 : 4;
+        Knob knobs[3] = {{&g->tune_wfs, KC_WFS, g->G.pos == 1 ? 4 : 2}, {&g->tune_layer, KC_LAYER, 4},
+                         {&g->tune_col, KC_COL, colmax}};
+        for (auto &k : knobs) {
+            int bestc = 1;
+            double bestt = 1e30;
+            for (int c = 1; c <= k.cmax; c <<= 1) {
+                *k.field = c;
+                const double t = stage_ms(k.kc);
+                if (t < bestt * 0.98) {  // prefer the smaller choice within 2%
+                    bestt = t;
+                    bestc = c;
+                }
+            }
+            *k.field = bestc;
+        }
+        drop_batches(g);
+        g->prof = save_prof;
+        if (choice) {
+            choice[0] = g->tune_wfs;
+            choice[1] = g->tune_layer;
+            choice[2] = g->tune_col;
+        }
+    });
 }
 
 int gpu_tomo_precision(gpu_tomo_t *g, int mode) {

@@ -158,6 +158,19 @@ class GpuTomo:
         forward spectrum (default); 2: FP64 residual, FP32 spectrum."""
         _abi.check(_abi.lib().gpu_tomo_precision(self._h, int(mode)), "gpu_tomo_precision")
 
+    def info(self):
+        """gpu_tomo_info: dict(matched, fnx, fny, bs, rmax)."""
+        v = [C.c_int() for _ in range(5)]
+        _abi.check(_abi.lib().gpu_tomo_info(self._h, *[C.byref(a) for a in v]), "gpu_tomo_info")
+        return dict(zip(("matched", "fnx", "fny", "bs", "rmax"), [a.value for a in v]))
+
+    def autotune(self, nsys=None):
+        """gpu_tomo_autotune: pick systems per workgroup of the WFS, layer
+        and column kernels by timing; returns (wfs, layer, col)."""
+        ch = (C.c_int * 3)()
+        _abi.check(_abi.lib().gpu_tomo_autotune(self._h, int(nsys or self.nbatch), ch), "gpu_tomo_autotune")
+        return tuple(ch)
+
     def close(self):
         if getattr(self, "_h", None):
             _abi.lib().gpu_tomo_free(self._h)

@@ -18,15 +18,24 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.fixture(scope="module", params=["small8", "large30", "lgs8_flat"])
+EXTRA = {
+    "lgs8_flat": configs.lgs8_flat,
+    "b8_w2_l4": lambda: configs.batch_member(8.0, 2, 4),    # FFT 28 (radix 7), bs 4
+    "b10_w3_l2": lambda: configs.batch_member(10.0, 3, 2),  # bs 2
+    "b8_w5_l5": lambda: configs.batch_member(8.0, 5, 5),    # bs 5 (runtime block size)
+}
+
+
+@pytest.fixture(scope="module", params=["small8", "large30", "lgs8_flat", "b8_w2_l4", "b10_w3_l2", "b8_w5_l5"])
 def sys_(request):
-    if request.param == "lgs8_flat":
-        cfg = configs.lgs8_flat()
+    if request.param in EXTRA:
+        cfg = EXTRA[request.param]()
         r = tomo.Recon(cfg)
     else:
         cfg, r = request.getfixturevalue(request.param)
-    print("case", cfg["name"], flush=True)
-    return cfg, r, tomo.GpuTomo(r, maxit=cfg["maxit"]), Oracle(r, nthread=8)
+    g = tomo.GpuTomo(r, maxit=cfg["maxit"])
+    print("case", cfg["name"], g.info(), flush=True)
+    return cfg, r, g, Oracle(r, nthread=8)
 
 
 def test_L(sys_):
