@@ -58,6 +58,92 @@ def algorithmic_bytes(r):
     }
 
 
+def measured_traffic(kernel, batch):
+    """HBM bytes per launch of `kernel` at this batch size, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (each scaled by the counter's
+    reading on a copy of known size), or None when not measured."""
+    path = os.path.join(ROOT, "profiles", "traffic_large30.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        k = t["kernels"][kernel]
+        if int(t.get("batch", -1)) != batch:
+            return None
+        return int(k["traffic_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+KERNEL_CLASSES = ("k_ttr", "k_wfs", "k_layer", "k_row", "k_col", "k_irow", "k_final", "k_zero_stop", "k_copy")
+
+
+def _pmc_rows(d, counter):
+    """(kernel name, value) per dispatch from a rocprofv3 --pmc csv run."""
+    import csv
+    import glob
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit("no counter_collection.csv under " + d)
+    out = []
+    for fn in files:
+        with open(fn) as f:
+            for row in csv.DictReader(f):
+                if row["Counter_Name"] == counter:
+                    out.append((row["Kernel_Name"], float(row["Counter_Value"])))
+    return out
+
+
+def _kernel_class(name):
+    base = name.split("(")[0].split("<")[0].replace("void ", "").strip().split("::")[-1]
+    for k in sorted(KERNEL_CLASSES, key=len, reverse=True):
+        if base == k or base.startswith(k + "_"):
+            return k
+    return None
+
+
+def traffic_summary(d, batch):
+    """Measured HBM bytes per launch of each solve kernel: the mean FETCH_SIZE
+    and WRITE_SIZE (KiB per dispatch) of the class's most frequent
+    instantiation, each scaled by bytes / reading of the same counter on an
+    elementwise copy of 1 GiB (read 1 GiB, write 1 GiB)."""
+    import collections
+    copy_bytes = 4 << 28
+
+    def calib(sub, counter):
+        vals = [v for k, v in _pmc_rows(os.path.join(d, sub), counter) if "neg" in k]
+        if not vals:
+            raise SystemExit("no copy kernel in " + sub)
+        return copy_bytes / (1024.0 * float(np.median(vals)))
+
+    fr, wr = calib("copy_fetch", "FETCH_SIZE"), calib("copy_write", "WRITE_SIZE")
+    res = {}
+    for sub, counter, scale, key in (("fetch", "FETCH_SIZE", fr, "fetch_bytes"),
+                                     ("write", "WRITE_SIZE", wr, "write_bytes")):
+        by = collections.defaultdict(lambda: collections.defaultdict(list))
+        for name, v in _pmc_rows(os.path.join(d, sub), counter):
+            c = _kernel_class(name)
+            if c:
+                by[c][name].append(v)
+        for c, names in by.items():
+            name, vals = max(names.items(), key=lambda kv: len(kv[1]))
+            e = res.setdefault(c, {})
+            e[key] = int(round(1024.0 * float(np.mean(vals)) * scale))
+            e[key + "_dispatches"] = len(vals)
+            e.setdefault("instantiation", name.split("(")[0].replace("void ", ""))
+    for e in res.values():
+        e["traffic_bytes"] = e.get("fetch_bytes", 0) + e.get("write_bytes", 0)
+    out = {"batch": batch, "fetch_scale": round(fr, 4), "write_scale": round(wr, 4), "kernels": res}
+    with open(os.path.join(ROOT, "profiles", "traffic_large30.json"), "w") as f:
+        json.dump(out, f, indent=1, sort_keys=True)
+    print("copy calibration: bytes per counter byte  fetch %.4f  write %.4f" % (fr, wr))
+    print("%-10s %12s %12s %12s" % ("kernel", "fetch MB", "write MB", "traffic MB"))
+    for c in KERNEL_CLASSES:
+        if c in res:
+            e = res[c]
+            print("%-10s %12.3f %12.3f %12.3f" % (c, e.get("fetch_bytes", 0) / 1e6, e.get("write_bytes", 0) / 1e6,
+                                                 e["traffic_bytes"] / 1e6))
+
+
 def cpu_baseline(r, grads, maxit, seconds):
     from oracle.oracle import Oracle
     try:
@@ -147,6 +233,41 @@ def bench_uneven(steps, check=3):
                 p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
                 sequential_plain_solves_per_s=n / seq, speedup=float((n * len(lat) / lat.sum()) / (n / seq)),
                 rel_l2_vs_oracle_first=errs)
+
+
+def bench_precision(seeds=8, iters=(10, 30, 60)):
+    """Precision study on the 30 m system: relative L2 of the GPU solution
+    and residual-norm ratio against the FP64 CPU oracle after the same
+    iteration count, per precision mode, over seeds (median and max)."""
+    from maos_amd import configs, tomo, workload
+    from oracle.oracle import Oracle
+    cfg = configs.large30()
+    r = tomo.Recon(cfg)
+    o = Oracle(r, nthread=16)
+    grads = [workload.make_inputs(cfg, r, 500 + k) for k in range(seeds)]
+    ref = {(it, k): o.solve(grads[k], it) for it in iters for k in range(seeds)}
+    modes = {1: "FP32 operators, FP64 residual recurrence + forward spectrum + reductions (default)",
+             2: "FP32 operators and spectrum, FP64 residual recurrence + reductions",
+             0: "FP32 bulk, FP64 reductions and alpha/beta",
+             3: "FP32 everywhere (reductions too)"}
+    out = {"workload": "large30, %d seeds, cold start; reference: FP64 CPU oracle at the same iteration count" % seeds,
+           "modes": {}}
+    for mode, desc in modes.items():
+        rows = {}
+        for it in iters:
+            g = tomo.GpuTomo(r, maxit=it)
+            g.precision(mode)
+            errs, ratios = [], []
+            for k in range(seeds):
+                x, res, _ = g.tomo(grads[k])
+                xo, reso = ref[(it, k)]
+                errs.append(float(np.linalg.norm(x - xo) / np.linalg.norm(xo)))
+                ratios.append(float(res / reso))
+            g.close()
+            rows[str(it)] = dict(rel_l2_median=float(np.median(errs)), rel_l2_max=float(np.max(errs)),
+                                 residual_ratio_median=float(np.median(ratios)))
+        out["modes"][str(mode)] = dict(description=desc, iterations=rows)
+    return out
 
 
 def served_loops():
@@ -240,9 +361,27 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", type=int, default=None,
                     help="solve precision mode (0, 1, 2; default: the library default)")
-    ap.add_argument("--modes", default="plain,graph,batched,uneven,served",
+    ap.add_argument("--modes", default="plain,graph,batched,uneven,served,precision",
                     help="comma list of modes to time (profiling runs use one)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 3 device copies of 1 GiB (known HBM traffic) for counter scaling")
+    ap.add_argument("--traffic-summary", metavar="DIR", default=None,
+                    help="no GPU: fold rocprofv3 FETCH_SIZE / WRITE_SIZE runs under DIR/{fetch,write,"
+                         "copy_fetch,copy_write} into profiles/traffic_large30.json")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = 1 << 28  # floats: 1 GiB
+        a = torch.ones(n, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            torch.neg(a, out=b)  # one elementwise kernel: reads a, writes b
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "copies": 3}))
+        return
+    if args.traffic_summary:
+        traffic_summary(args.traffic_summary, args.batch)
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the solve runs on one MI355X; "
               "independent solves would shard across GPUs with no communication). "
@@ -317,7 +456,7 @@ def main():
     dom = max((k for k in kernels if k in ab), key=lambda k: kernels[k]["total_ms"])
     kd = kernels[dom]
     roofline = dict(bound="hbm", achieved=round(kd["GBs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(kd["GBs"] / HBM_PEAK_GBS, 4), traffic=None, kernel=dom,
+                    frac=round(kd["GBs"] / HBM_PEAK_GBS, 4), traffic=measured_traffic(dom, B), kernel=dom,
                     avg_us=round(kd["avg_us"], 2), alg_bytes_per_launch=int(ab[dom][0] * B + ab[dom][1]))
 
     uneven = None
@@ -326,6 +465,13 @@ def main():
             uneven = bench_uneven(max(5, args.steps // 20))
         except Exception as e:  # reported, not fatal for the headline line
             uneven = {"error": repr(e)}
+
+    precision = None
+    if "precision" in modes:
+        try:
+            precision = bench_precision()
+        except Exception as e:  # reported, not fatal for the headline line
+            precision = {"error": repr(e)}
 
     served = None
     if "served" in modes:
@@ -367,6 +513,7 @@ def main():
         "cpu_baseline": cpu,
         "uneven64": uneven,
         "served16": served,
+        "precision_study": precision,
     }
     print(json.dumps(out))
 

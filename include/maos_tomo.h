@@ -250,12 +250,14 @@ int  gpu_tomo_R(gpu_tomo_t *g, const double *grad, double *xout, double alpha);
 int  gpu_tomo_P(gpu_tomo_t *g, const double *xin, double *xout);
 
 /* Precision of the solve.  Operators (HX, GP, W, L2), x, p, z and the
- * inverse FDPCG transform are FP32 and dot products / alpha / beta FP64 in
- * every mode; the modes differ in the residual recurrence (r, A*p, b) and
- * the forward FDPCG spectrum:
+ * inverse FDPCG transform are FP32 in every mode, dot products / alpha /
+ * beta FP64 in modes 0-2; the modes differ in the residual recurrence
+ * (r, A*p, b), the forward FDPCG spectrum and the reductions:
  *   0  r, A*p, b FP32; forward spectrum FP32
  *   1  r, A*p, b FP64; forward spectrum FP64   (default)
  *   2  r, A*p, b FP64; forward spectrum FP32
+ *   3  as 0, and the dot products, alpha and beta in FP32 too (FP32
+ *      everywhere; for the precision study)
  * A 30-iteration FDPCG solve of the 30 m system drifts ~2e-3 (relative L2)
  * from the FP64 solve with mode 0 and stays ~1e-5 with mode 1. */
 int  gpu_tomo_precision(gpu_tomo_t *g, int mode);

@@ -37,13 +37,15 @@ template <> struct CT<double2> {
 };
 
 // Job and block of this workgroup (see Job / WGI in kernels.hpp).
+// The workgroup's table entry is read as one 8-byte scalar load (entries
+// are 8-byte aligned: {job, bx} in the low word, {by, bz} in the high one).
 #define KJOB                                                   \
-    const WGI wi_ = wgi[blockIdx.x];                           \
-    const Job J = jobs[wi_.job];                               \
+    const unsigned long long wi_ = ((const CONSTAS unsigned long long *)wgi)[blockIdx.x]; \
+    const Job J = jobs[wi_ & 0xffffu];                         \
     if (!J.on) return;                                         \
     CPD &PD = *(const CONSTAS PlanDev *)J.pd;                  \
     CG &G = PD.G;                                              \
-    const int bx = wi_.bx, by = wi_.by, bz = wi_.bz;           \
+    const int bx = (int)((wi_ >> 16) & 0xffffu), by = (int)((wi_ >> 32) & 0xffffu), bz = (int)(wi_ >> 48); \
     (void)bx;                                                  \
     (void)by;                                                  \
     (void)bz
@@ -106,27 +108,52 @@ __device__ __forceinline__ Scal ldsc(const CONSTAS Scal &c) {
 // Deterministic FP64 reductions (fixed order: per-lane strided sums, then a
 // 64-lane shuffle tree, then wavefront partials summed by lane 0 in order).
 // ---------------------------------------------------------------------------
-__device__ double block_sum(double v, double *sh) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+__device__ double block_sum(double v, double *sh, int f32 = 0) {
+    if (f32) {
+        // precision mode 3: the reduction itself in FP32 (same order)
+        float f = (float)v;
+        for (int o = 32; o > 0; o >>= 1) f += __shfl_down(f, o, 64);
+        v = f;
+    } else {
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    }
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     __syncthreads();
     if (lane == 0) sh[wid] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double s = 0;
+        float sf = 0.f;
         const int nw = (blockDim.x + 63) >> 6;
-        for (int i = 0; i < nw; i++) s += sh[i];
-        sh[0] = s;
+        for (int i = 0; i < nw; i++) {
+            s += sh[i];
+            sf += (float)sh[i];
+        }
+        sh[0] = f32 ? (double)sf : s;
     }
     __syncthreads();
     double r = sh[0];
     __syncthreads();
     return r;
 }
-__device__ double reduce_partials(const double *part, int n, double *sh) {
+__device__ double reduce_partials(const double *part, int n, double *sh, int f32 = 0) {
     double v = 0;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) v += part[i];
-    return block_sum(v, sh);
+    float vf = 0.f;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        v += part[i];
+        vf += (float)part[i];
+    }
+    return block_sum(f32 ? (double)vf : v, sh, f32);
+}
+// Product / fused accumulation of a dot-product term, in FP32 for mode 3.
+__device__ __forceinline__ double dmul(double a, double b, int f32) {
+    return f32 ? (double)((float)a * (float)b) : a * b;
+}
+__device__ __forceinline__ double dfma(double a, double b, double c, int f32) {
+    return f32 ? (double)fmaf((float)a, (float)b, (float)c) : a * b + c;
+}
+__device__ __forceinline__ double ddiv(double a, double b, int f32) {
+    return f32 ? (double)((float)a / (float)b) : a / b;
 }
 
 // ---------------------------------------------------------------------------
@@ -463,14 +490,14 @@ __global__ __launch_bounds__(256) void k_layer(KARGS, int psel_, int osel, int p
                 const RT sd = P(-1, -1) + P(1, -1) + P(-1, 1) + P(1, 1);
                 const RT s2 = P(-2, 0) + P(2, 0) + P(0, -2) + P(0, 2);
                 acc += (RT)G.l2c[l] * (RT(20) * c - RT(8) * s4 + RT(2) * sd + s2);
-                dot = (double)c * (double)acc;
+                dot = dmul(c, acc, G.dot32);
             } else {
-                dot = (double)acc * (double)acc;
+                dot = dmul(acc, acc, G.dot32);
             }
             out[o + jy * nxl + jx] = acc;
         }
     }
-    dot = block_sum(dot, sh);
+    dot = block_sum(dot, sh, G.dot32);
     if (threadIdx.x == 0 && part) part[(long)s * partstride + by * PD.ltiles + bx] = dot;
 }
 
@@ -491,7 +518,8 @@ __global__ __launch_bounds__(256) void k_wfs_m(KARGS, int xsel) {
     constexpr int S = TILE + POS;       // subaperture-corner window edge
     constexpr int S2 = S * S;
     constexpr int NP = POS + 1, NP2 = NP * NP;
-    constexpr int UG = (S2 * NP2 + 255) / 256, UN = (S2 + 255) / 256, UO = (H * H + 255) / 256;
+    constexpr int UG = (S2 * NP2 + 255) / 256, UN = (S2 + 255) / 256;
+    static_assert(H % 2 == 0 && H * H / 2 <= 256, "one two-row item per thread");
     __shared__ float2 gps[S2 * NP2];  // GP coefficients of the window's corners
     __shared__ RT opd[CS][H * H];
     __shared__ RT wgx[CS][S2], wgy[CS][S2];
@@ -501,30 +529,35 @@ __global__ __launch_bounds__(256) void k_wfs_m(KARGS, int xsel) {
     const int tx0 = (bx % ntx) * TILE, ty0 = (bx / ntx) * TILE;
     const float2 *gpw = G.gpd + (long)G.gcls[w] * G.nploc * NP2;
     const float4 *nw4 = G.nead + (long)G.gcls[w] * G.nploc;
-    // geometry of the window (shared by the CS systems)
+    // geometry of the window (shared by the CS systems).  Loads here and in
+    // the HX pass are unconditional, from indices clamped into range, and
+    // masked afterwards: loads under a branch are each waited for before
+    // the branch closes, which would serialise them.
     float2 gv[UG];
     float4 nv[UN];
 #pragma unroll
     for (int u = 0; u < UG; u++) {
-        const int i = threadIdx.x + 256 * u;
-        if (i < S2 * NP2) {
-            const int c = i / NP2, z = i - c * NP2;
-            const int cx = tx0 - POS + c % S, cy = ty0 - POS + c / S;
-            gv[u] = (cx >= 0 && cx < pnx && cy >= 0 && cy < pny) ? gpw[(long)(cy * pnx + cx) * NP2 + z]
-                                                               : make_float2(0.f, 0.f);
-        }
+        const int i = min((int)threadIdx.x + 256 * u, S2 * NP2 - 1);
+        const int c = i / NP2, z = i - c * NP2;
+        const int cx = tx0 - POS + c % S, cy = ty0 - POS + c / S;
+        const int cxc = min(max(cx, 0), pnx - 1), cyc = min(max(cy, 0), pny - 1);
+        const float2 v = gpw[(long)(cyc * pnx + cxc) * NP2 + z];
+        gv[u] = (cx == cxc && cy == cyc) ? v : make_float2(0.f, 0.f);
     }
 #pragma unroll
     for (int u = 0; u < UN; u++) {
         const int i = threadIdx.x + 256 * u;
-        const int cx = tx0 - POS + i % S, cy = ty0 - POS + i / S;
-        nv[u] = (i < S2 && cx >= 0 && cx < pnx && cy >= 0 && cy < pny) ? nw4[cy * pnx + cx]
-                                                                       : make_float4(0.f, 0.f, 0.f, 0.f);
+        const int ic = min(i, S2 - 1);
+        const int cx = tx0 - POS + ic % S, cy = ty0 - POS + ic / S;
+        const int cxc = min(max(cx, 0), pnx - 1), cyc = min(max(cy, 0), pny - 1);
+        const float4 v = nw4[cyc * pnx + cxc];
+        nv[u] = (i < S2 && cx == cxc && cy == cyc) ? v : make_float4(0.f, 0.f, 0.f, 0.f);
     }
 #pragma unroll
     for (int u = 0; u < UG; u++) {
-        const int i = threadIdx.x + 256 * u;
-        if (i < S2 * NP2) gps[i] = gv[u];
+        // past the end: the last entry again, with its own value
+        const int i = min((int)threadIdx.x + 256 * u, S2 * NP2 - 1);
+        gps[i] = gv[u];
     }
     // HX on the opd window, gathered straight from the layers (L1/L2 hits):
     // ploc point i reads nodes i+c and i+c+1 (the latter wrapping to 0 at the
@@ -532,58 +565,62 @@ __global__ __launch_bounds__(256) void k_wfs_m(KARGS, int xsel) {
     // wholly inside the ray interval (plain loads) or not (indices clamped
     // into the grid, points outside the interval contribute zero).  Index
     // work is shared by the CS systems.
-    RT v[UO][CS];
+    // Work item i: column i % H of the window, rows 2*(i / H) and +1 (H is
+    // even), so on the plain path the middle sample row serves both points.
+    constexpr int NI = H * H / 2;
+    const int ia = threadIdx.x % H, ib = threadIdx.x / H;
+    const bool item = threadIdx.x < NI;
+    const int hx = tx0 - POS + ia, hy0 = ty0 - POS + 2 * ib;
+    RT v[2][CS];
 #pragma unroll
-    for (int u = 0; u < UO; u++)
+    for (int r = 0; r < 2; r++)
 #pragma unroll
-        for (int si = 0; si < CS; si++) v[u][si] = 0;
-    for (int l = 0; l < G.npsr; l++) {
-        const MRay m = mray(G, w * MAXL + l);
-        const int nxl = G.xnx[l], nyl = G.xny[l];
-        const float *xl = x + (long)s0 * G.nxtot + G.xoff[l];
-        const RT fx = m.fx, fy = m.fy;
-        const bool fast = tx0 - POS >= m.ax0 && tx0 + TILE + POS <= (m.wx ? m.ax1 - 1 : m.ax1) &&
-                          ty0 - POS >= m.ay0 && ty0 + TILE + POS <= (m.wy ? m.ay1 - 1 : m.ay1);
-#pragma unroll
-        for (int u = 0; u < UO; u++) {
-            const int i = threadIdx.x + 256 * u;
-            if (i >= H * H) continue;
-            const int px = tx0 - POS + i % H, py = ty0 - POS + i / H;
+        for (int si = 0; si < CS; si++) v[r][si] = 0;
+    if (item) {
+        for (int l = 0; l < G.npsr; l++) {
+            const MRay m = mray(G, w * MAXL + l);
+            const int nxl = G.xnx[l], nyl = G.xny[l];
+            const float *xl = x + (long)s0 * G.nxtot + G.xoff[l];
+            const RT fx = m.fx, fy = m.fy;
+            const bool fast = tx0 - POS >= m.ax0 && tx0 + TILE + POS <= (m.wx ? m.ax1 - 1 : m.ax1) &&
+                              ty0 - POS >= m.ay0 && ty0 + TILE + POS <= (m.wy ? m.ay1 - 1 : m.ay1);
+            // systems past ns read system ns-1 (in range; never stored)
             if (fast) {
-                const int o = (py + m.cy) * nxl + (px + m.cx);
+                const int o = (hy0 + m.cy) * nxl + (hx + m.cx);
 #pragma unroll
                 for (int si = 0; si < CS; si++) {
-                    if (si < ns) {
-                        const float *t = xl + (long)si * G.nxtot + o;
-                        const RT a = t[0], b = t[1], c = t[nxl], d = t[nxl + 1];
-                        v[u][si] += (RT(1) - fy) * ((RT(1) - fx) * a + fx * b) + fy * ((RT(1) - fx) * c + fx * d);
-                    }
+                    const float *t = xl + (long)min(si, ns - 1) * G.nxtot + o;
+                    RT h[3];
+#pragma unroll
+                    for (int k = 0; k < 3; k++) h[k] = (RT(1) - fx) * (RT)t[k * nxl] + fx * (RT)t[k * nxl + 1];
+                    v[0][si] += (RT(1) - fy) * h[0] + fy * h[1];
+                    v[1][si] += (RT(1) - fy) * h[1] + fy * h[2];
                 }
             } else {
-                const bool ok = px >= m.ax0 && px < m.ax1 && py >= m.ay0 && py < m.ay1;
-                const int n0x = min(max(px + m.cx, 0), nxl - 1), n0y = min(max(py + m.cy, 0), nyl - 1);
-                const int n1x = n0x + 1 == nxl ? 0 : n0x + 1;
-                const int n1y = n0y + 1 == nyl ? 0 : n0y + 1;
-                const int o00 = n0y * nxl + n0x, o10 = n0y * nxl + n1x, o01 = n1y * nxl + n0x,
-                          o11 = n1y * nxl + n1x;
 #pragma unroll
-                for (int si = 0; si < CS; si++) {
-                    if (si < ns) {
-                        const float *t = xl + (long)si * G.nxtot;
+                for (int r = 0; r < 2; r++) {
+                    const int hy = hy0 + r;
+                    const bool ok = hx >= m.ax0 && hx < m.ax1 && hy >= m.ay0 && hy < m.ay1;
+                    const int n0x = min(max(hx + m.cx, 0), nxl - 1), n0y = min(max(hy + m.cy, 0), nyl - 1);
+                    const int n1x = n0x + 1 == nxl ? 0 : n0x + 1;
+                    const int n1y = n0y + 1 == nyl ? 0 : n0y + 1;
+                    const int o00 = n0y * nxl + n0x, o10 = n0y * nxl + n1x, o01 = n1y * nxl + n0x,
+                              o11 = n1y * nxl + n1x;
+#pragma unroll
+                    for (int si = 0; si < CS; si++) {
+                        const float *t = xl + (long)min(si, ns - 1) * G.nxtot;
                         const RT a = t[o00], b = t[o10], c = t[o01], d = t[o11];
                         const RT h = (RT(1) - fy) * ((RT(1) - fx) * a + fx * b) + fy * ((RT(1) - fx) * c + fx * d);
-                        v[u][si] += ok ? h : RT(0);
+                        v[r][si] += ok ? h : RT(0);
                     }
                 }
             }
         }
-    }
 #pragma unroll
-    for (int u = 0; u < UO; u++) {
-        const int i = threadIdx.x + 256 * u;
+        for (int r = 0; r < 2; r++)
 #pragma unroll
-        for (int si = 0; si < CS; si++)
-            if (i < H * H && si < ns) opd[si][i] = v[u][si];
+            for (int si = 0; si < CS; si++)
+                if (si < ns) opd[si][(2 * ib + r) * H + ia] = v[r][si];
     }
     __syncthreads();
     // W GP per subaperture corner of the window (zero coefficients where none)
@@ -763,16 +800,182 @@ __global__ __launch_bounds__(256) void k_layer_m(KARGS, int psel_, int osel, int
                 const RT sd = P(-1, -1) + P(1, -1) + P(-1, 1) + P(1, 1);
                 const RT s2 = P(-2, 0) + P(2, 0) + P(0, -2) + P(0, 2);
                 a += (RT)G.l2c[l] * (RT(20) * c - RT(8) * s4 + RT(2) * sd + s2);
-                dot[si] = (double)c * (double)a;
+                dot[si] = dmul(c, a, G.dot32);
             } else {
-                dot[si] = (double)a * (double)a;
+                dot[si] = dmul(a, a, G.dot32);
             }
             out[(long)(s0 + si) * G.nxtot + G.xoff[l] + jy * nxl + jx] = a;
         }
     }
 #pragma unroll
     for (int si = 0; si < CS; si++) {
-        const double d = block_sum(dot[si], sh);
+        const double d = block_sum(dot[si], sh, G.dot32);
+        if (threadIdx.x == 0 && part && si < ns)
+            part[(long)(s0 + si) * partstride + by * PD.ltiles + bx] = d;
+    }
+}
+
+// Layer pass of the matched path on 16 x 64 tiles: each thread owns four
+// vertically adjacent nodes of one column, so the five sample rows of a WFS
+// field serve four nodes (the horizontal interpolation of a row is shared by
+// the two nodes it falls between).  Same arithmetic per node as k_layer_m.
+template <bool LHS, typename RT, int CS>
+__global__ __launch_bounds__(256) void k_layer_m4(KARGS, int psel_, int osel, int partsel) {
+    KJOB;
+    const RT *q = (const RT *)PD.q;
+    const float *p = vsel(PD, psel_);
+    RT *out = rsel<RT>(PD, osel);
+    double *part = psel(PD, partsel);
+    const int partstride = PD.SC.maxpart;
+    constexpr int R = 4;
+    constexpr int PW = TILE + 4, PHH = LTY + 4, P2 = PW * PHH;  // p window (L2'L2 reach 2)
+    constexpr int UP = (P2 + 255) / 256;
+    __shared__ float ps[CS][LHS ? P2 : 1];
+    __shared__ double sh[8];
+    const int l = by, s0 = J.s0 + bz * CS, ns = min(CS, J.s0 + J.nsys - s0);
+    const int nxl = G.xnx[l], nyl = G.xny[l];
+    const int ntx = (nxl + TILE - 1) / TILE, nty = (nyl + LTY - 1) / LTY;
+    const bool active = bx < ntx * nty;
+    const int jx0 = (bx % ntx) * TILE, jy0 = (bx / ntx) * LTY;
+    const int tx = threadIdx.x % TILE, ty = threadIdx.x / TILE;
+    const int jx = jx0 + tx, jyb = jy0 + R * ty;
+    const int nr = active && jx < nxl ? max(0, min(R, nyl - jyb)) : 0;  // rows of this thread
+    if (LHS && active) {
+        // p windows, periodic (layers are >= LTY + 2 rows and TILE + 2 columns)
+        int off[UP];
+#pragma unroll
+        for (int u = 0; u < UP; u++) {
+            const int i = min((int)threadIdx.x + 256 * u, P2 - 1);
+            int xx = jx0 - 2 + i % PW, yy = jy0 - 2 + i / PW;
+            xx += xx < 0 ? nxl : 0;
+            xx -= xx >= nxl ? nxl : 0;
+            yy += yy < 0 ? nyl : 0;
+            yy -= yy >= nyl ? nyl : 0;
+            off[u] = yy * nxl + xx;
+        }
+        // unconditional loads (systems past ns read system ns-1, entries
+        // past the window the last entry, stored again with its own value):
+        // loads under a branch are each waited for before the branch closes
+        float pv[CS][UP];
+#pragma unroll
+        for (int si = 0; si < CS; si++) {
+            const float *pl = p + (long)(s0 + min(si, ns - 1)) * G.nxtot + G.xoff[l];
+#pragma unroll
+            for (int u = 0; u < UP; u++) pv[si][u] = pl[off[u]];
+        }
+#pragma unroll
+        for (int si = 0; si < CS; si++)
+#pragma unroll
+            for (int u = 0; u < UP; u++) ps[si][min((int)threadIdx.x + 256 * u, P2 - 1)] = pv[si][u];
+    }
+    RT acc[CS][R];
+#pragma unroll
+    for (int si = 0; si < CS; si++)
+#pragma unroll
+        for (int r = 0; r < R; r++) acc[si][r] = 0;
+    const int tw = min(TILE, nxl - jx0), th = min(LTY, nyl - jy0);
+    const long qstride = (long)G.nwfs * G.nploc;
+    if (nr > 0) {
+        for (int w = 0; w < G.nwfs; w++) {
+            const MRay m = mray(G, w * MAXL + l);
+            const bool edgex = m.wx && jx0 == 0, edgey = m.wy && jy0 == 0;
+            if (!edgex && (jx0 + tw - 1 - m.cx < m.ax0 || jx0 - m.cx - 1 >= m.ax1)) continue;
+            if (!edgey && (jy0 + th - 1 - m.cy < m.ay0 || jy0 - m.cy - 1 >= m.ay1)) continue;
+            const RT *qw = q + ((long)s0 * G.nwfs + w) * G.nploc;
+            const RT fx = m.fx, fy = m.fy;
+            const int ix0 = jx - m.cx - 1, ix1 = jx - m.cx;
+            if (edgex || edgey) {
+                // tiles holding the wrapped edge ray: node by node
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    if (r >= nr) continue;
+                    const int jy = jyb + r;
+                    int iy0 = jy - m.cy - 1, jx0r = ix0;
+                    if (m.wx && jx == 0) jx0r = m.ax1 - 1;
+                    if (m.wy && jy == 0) iy0 = m.ay1 - 1;
+                    const int iy1 = jy - m.cy;
+                    const RT wx0 = (jx0r >= m.ax0 && jx0r < m.ax1) ? fx : RT(0);
+                    const RT wx1 = (ix1 >= m.ax0 && ix1 < m.ax1) ? RT(1) - fx : RT(0);
+                    const RT wy0 = (iy0 >= m.ay0 && iy0 < m.ay1) ? fy : RT(0);
+                    const RT wy1 = (iy1 >= m.ay0 && iy1 < m.ay1) ? RT(1) - fy : RT(0);
+                    const int cx0 = min(max(jx0r, 0), G.pnx - 1), cx1 = min(max(ix1, 0), G.pnx - 1);
+                    const int cy0 = min(max(iy0, 0), G.pny - 1), cy1 = min(max(iy1, 0), G.pny - 1);
+#pragma unroll
+                    for (int si = 0; si < CS; si++) {
+                        if (si < ns) {
+                            const RT *t = qw + si * qstride;
+                            acc[si][r] += wy0 * (wx1 * t[cy0 * G.pnx + cx1] + wx0 * t[cy0 * G.pnx + cx0]) +
+                                          wy1 * (wx1 * t[cy1 * G.pnx + cx1] + wx0 * t[cy1 * G.pnx + cx0]);
+                        }
+                    }
+                }
+                continue;
+            }
+            const bool fast = jx0 - m.cx - 1 >= m.ax0 && jx0 + tw - m.cx <= m.ax1 && jy0 - m.cy - 1 >= m.ay0 &&
+                              jy0 + th - m.cy <= m.ay1;
+            // sample rows k = 0..nr of this thread: iy = jyb - cy - 1 + k
+            RT wxa = fx, wxb = RT(1) - fx;
+            int ca = ix0, cb = ix1;
+            if (!fast) {
+                wxa = (ix0 >= m.ax0 && ix0 < m.ax1) ? fx : RT(0);
+                wxb = (ix1 >= m.ax0 && ix1 < m.ax1) ? RT(1) - fx : RT(0);
+                ca = min(max(ix0, 0), G.pnx - 1);
+                cb = min(max(ix1, 0), G.pnx - 1);
+            }
+            // Rows are clamped into the grid (a no-op on the plain path for
+            // rows up to nr; rows past it only feed nodes that are not
+            // stored) and systems past ns read system ns-1, so the loads
+            // are unconditional.
+            int rowo[R + 1];
+            RT my[R + 1];
+#pragma unroll
+            for (int k = 0; k <= R; k++) {
+                const int iy = jyb - m.cy - 1 + k;
+                my[k] = (fast || (iy >= m.ay0 && iy < m.ay1)) ? RT(1) : RT(0);
+                rowo[k] = min(max(iy, 0), G.pny - 1) * G.pnx;
+            }
+#pragma unroll
+            for (int si = 0; si < CS; si++) {
+                const RT *t = qw + min(si, ns - 1) * qstride;
+                RT h[R + 1];
+#pragma unroll
+                for (int k = 0; k <= R; k++) h[k] = wxb * t[rowo[k] + cb] + wxa * t[rowo[k] + ca];
+#pragma unroll
+                for (int r = 0; r < R; r++)
+                    acc[si][r] += (my[r] * fy) * h[r] + (my[r + 1] * (RT(1) - fy)) * h[r + 1];
+            }
+        }
+    }
+    if (LHS) __syncthreads();
+    double dot[CS];
+#pragma unroll
+    for (int si = 0; si < CS; si++) {
+        dot[si] = 0;
+        if (si < ns) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                if (r >= nr) continue;
+                const int jy = jyb + r;
+                RT a = acc[si][r];
+                if (LHS) {
+                    const float *P0 = ps[si] + (R * ty + r + 2) * PW + tx + 2;
+                    auto P = [&](int dx, int dy) { return (RT)P0[dy * PW + dx]; };
+                    const RT c = P(0, 0);
+                    const RT s4 = P(-1, 0) + P(1, 0) + P(0, -1) + P(0, 1);
+                    const RT sd = P(-1, -1) + P(1, -1) + P(-1, 1) + P(1, 1);
+                    const RT s2 = P(-2, 0) + P(2, 0) + P(0, -2) + P(0, 2);
+                    a += (RT)G.l2c[l] * (RT(20) * c - RT(8) * s4 + RT(2) * sd + s2);
+                    dot[si] = dfma(c, a, dot[si], G.dot32);
+                } else {
+                    dot[si] = dfma(a, a, dot[si], G.dot32);
+                }
+                out[(long)(s0 + si) * G.nxtot + G.xoff[l] + jy * nxl + jx] = a;
+            }
+        }
+    }
+#pragma unroll
+    for (int si = 0; si < CS; si++) {
+        const double d = block_sum(dot[si], sh, G.dot32);
         if (threadIdx.x == 0 && part && si < ns)
             part[(long)(s0 + si) * partstride + by * PD.ltiles + bx] = d;
     }
@@ -838,9 +1041,9 @@ __global__ __launch_bounds__(256) void k_row(KARGS, int iter, int xsel) {
     load(0);
     double alpha = 0.;
     if (MODE >= ROW_UPDATE) {
-        const double pap = reduce_partials(SC.part_pap + (long)s * SC.maxpart, SC.npart_layer, sh);
+        const double pap = reduce_partials(SC.part_pap + (long)s * SC.maxpart, SC.npart_layer, sh, G.dot32);
         const double rz = SC.rz[(long)s * (SC.maxit + 2) + iter];
-        double a = rz / pap;
+        double a = ddiv(rz, pap, G.dot32);
         const int stopped = stop[s];
         if (!isfinite(a) || stopped) {
             // lib/pcg.c:74-77: a non-finite alpha ends the solve; every
@@ -874,13 +1077,13 @@ __global__ __launch_bounds__(256) void k_row(KARGS, int iter, int xsel) {
                     rv = v0[u] - (RT)alpha * v1[u];
                 }
                 r[idx] = rv;
-                if (MODE == ROW_LAST) dot += (double)rv * (double)rv;
+                if (MODE == ROW_LAST) dot = dfma(rv, rv, dot, G.dot32);
                 else bufA[2 * ((row >> 1) * n + col) + (row & 1)] = (FR)rv;
             }
         }
     }
     if (MODE == ROW_LAST) {
-        dot = block_sum(dot, sh);
+        dot = block_sum(dot, sh, G.dot32);
         if (threadIdx.x == 0) SC.part_rr[(long)s * SC.maxpart + bx] = dot;
         return;
     }
@@ -908,7 +1111,95 @@ __global__ __launch_bounds__(256) void k_row(KARGS, int iter, int xsel) {
 // Everything in LDS: two buffers of CS * ncol * npsr * ny entries; the
 // multiply writes the buffer the forward FFT did not end in, and the inverse
 // FFT ping-pongs between it and the (then free) forward result.
+// Global memory is touched in two rounds only: the block rows (into
+// registers) and the spectrum columns at the start, the inverse columns at
+// the end; every index comes from registers or LDS.
 // ---------------------------------------------------------------------------
+
+// One output row m of a block: y = sum_k M[m, k] v_k over the block's bins
+// (k-th bin at LDS position pk[k], conjugated where flagged), written to C
+// unless another block writes that bin, and its term of the spectral r.z:
+// sc * w * Re(conj(R_m) y_m) with R the unscaled forward spectrum, y the
+// scaled output (sc = fscale, or 1), w = 1 on the kx = 0 and nx/2 columns and
+// 2 elsewhere (Parseval over the half spectrum); DC bins count zero when
+// zero_dc (the blocks' DC columns are zero then, as if the input were).
+// mk[k * ms]: row m's k-th coefficient.  KB: block size, 0 when runtime bs.
+template <typename FT, int CS, int KB>
+__device__ __forceinline__ void col_row(const float2 *mk, int ms, const int *pk, int pm, int bs, const FT *res,
+                                        int per, int ns, float2 *C, double *dot, typename CT<FT>::R sc, int zd,
+                                        int d32) {
+    typedef typename CT<FT>::R FR;
+    const int kb = KB > 0 ? KB : bs;
+    FT y[CS];
+#pragma unroll
+    for (int si = 0; si < CS; si++) y[si] = CT<FT>::make(0, 0);
+#pragma unroll
+    for (int k = 0; k < kb; k++) {
+        const int pe = pk[k];
+        const int pos = pe & BP_POS;
+        const FR cj = (pe & BP_CONJ) ? FR(-1) : FR(1);
+        const float2 m = mk[k * ms];
+        const FT mm = CT<FT>::make(m.x, m.y);
+#pragma unroll
+        for (int si = 0; si < CS; si++) {
+            if (si < ns) {
+                FT v = res[si * per + pos];
+                v.y *= cj;
+                y[si] = cadd(y[si], cmul(mm, v));
+            }
+        }
+    }
+    if (pm & BP_NOWRITE) return;
+    const int pos = pm & BP_POS;
+    const FR cj = (pm & BP_CONJ) ? FR(-1) : FR(1);
+    const double wgt = (pm & BP_W1) ? 1.0 : 2.0;
+    const bool skip = zd && (pm & BP_DC);
+#pragma unroll
+    for (int si = 0; si < CS; si++) {
+        if (si < ns) {
+            const FT yy = CT<FT>::make(sc * y[si].x, sc * cj * y[si].y);
+            C[si * per + pos] = make_float2((float)yy.x, (float)yy.y);
+            const FT rv = res[si * per + pos];
+            if (!skip) dot[si] = dfma(wgt, dfma(rv.x, yy.x, dmul(rv.y, yy.y, d32), d32), dot[si], d32);
+        }
+    }
+}
+
+// block_sum of N values at once (the same order for each as block_sum):
+// sh holds 5 * N doubles; 256-thread workgroups.
+template <int N>
+__device__ void block_sum_n(double *v, double *sh, int f32) {
+#pragma unroll
+    for (int n = 0; n < N; n++) {
+        if (f32) {
+            float f = (float)v[n];
+            for (int o = 32; o > 0; o >>= 1) f += __shfl_down(f, o, 64);
+            v[n] = f;
+        } else {
+            for (int o = 32; o > 0; o >>= 1) v[n] += __shfl_down(v[n], o, 64);
+        }
+    }
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int n = 0; n < N; n++) sh[n * 4 + wid] = v[n];
+    __syncthreads();
+    if (threadIdx.x < N) {
+        double s = 0;
+        float sf = 0.f;
+        for (int i = 0; i < 4; i++) {
+            s += sh[threadIdx.x * 4 + i];
+            sf += (float)sh[threadIdx.x * 4 + i];
+        }
+        sh[4 * N + threadIdx.x] = f32 ? (double)sf : s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int n = 0; n < N; n++) v[n] = sh[4 * N + n];
+    __syncthreads();
+}
+
 template <typename FT, int CS, int BS, int RM>
 __global__ __launch_bounds__(256) void k_col(KARGS) {
     KJOB;
@@ -916,7 +1207,7 @@ __global__ __launch_bounds__(256) void k_col(KARGS) {
     const FT *specf = (const FT *)PD.specf;
     float2 *speci = PD.speci;
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    __shared__ double sh[8];
+    __shared__ double sh[5 * CS];
     typedef typename CT<FT>::R FR;
     const int g = bx;
     const int s0 = J.s0 + bz * CS;
@@ -930,120 +1221,115 @@ __global__ __launch_bounds__(256) void k_col(KARGS) {
     FT *A = reinterpret_cast<FT *>(lds_raw), *B = A + cap;
     FT *twf = B + cap;
     float2 *twi = reinterpret_cast<float2 *>(twf + ny);
-    load_tw(twf, tw_of<FT>(G, true), ny);
-    load_tw(twi, G.twy, ny);
-    constexpr int U = 8;  // loads in flight per thread
-    for (int b0 = 0; b0 < tot; b0 += 256 * U) {
-        FT v[U];
+    int *coff = reinterpret_cast<int *>(twi + ny);  // per (column, layer): spectrum offset of the column
+    // Block rows of the group: with a compile-time block size, the first
+    // RP * 256 rows are loaded into registers here, so that their latency
+    // overlaps the spectrum loads and the forward FFT.
+    const int bs = BS > 0 ? BS : G.bs;
+    const int b0 = G.grp_blk[g], b1 = G.grp_blk[g + 1];
+    const int nrow = (b1 - b0) * bs;
+    constexpr int RP = BS > 0 ? 3 : 1;
+    constexpr int BR = BS > 0 ? BS : 1;
+    float2 mr[RP][BR];
+    int pr[RP][BR], pmr[RP];
+    if constexpr (BS > 0) {
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int i = b0 + threadIdx.x + 256 * u;
-            if (i < tot) {
-                const int si = CS == 1 ? 0 : (i >= per) + (i >= 2 * per) + (i >= 3 * per);
-                const int j = i - si * per;
-                const int f = fdiv(j, ldf(G.dfny)), ky = j - f * ny;
-                const int ci = fdiv(f, ldf(G.dnpsr)), l = f - ci * npsr;
-                v[u] = specf[(long)(s0 + si) * G.nspec + G.soff[l] + (long)G.colkx[c0 + ci] * ny + ky];
+        for (int u = 0; u < RP; u++) {
+            // unconditional (row clamped; a group has at least one block)
+            const int t = min((int)threadIdx.x + 256 * u, nrow - 1);
+            const int q = t / BS, m = t - q * BS;
+            const float2 *M = G.Minv + (long)(b0 + q) * BS * BS;
+            const int *bp = G.blkpos + (long)(b0 + q) * BS;
+#pragma unroll
+            for (int k = 0; k < BS; k++) {
+                mr[u][k] = M[m + k * BS];
+                pr[u][k] = bp[k];
             }
+            pmr[u] = bp[m];
+        }
+    }
+    // (uniform loop: scalar loads, which do not wait behind the vector loads
+    // above)
+    for (int ci = 0; ci < ncg; ci++) {
+        const int kx = ((const CONSTAS int *)G.colkx)[c0 + ci];
+        for (int l = 0; l < npsr; l++)
+            if (threadIdx.x == 0) coff[ci * npsr + l] = (int)G.soff[l] + kx * ny;
+    }
+    __syncthreads();
+    // Twiddles and spectrum columns: U loads in flight per thread,
+    // unconditional so that they issue back to back (past the end, the index
+    // is clamped to the last entry, which is then loaded and stored again
+    // with its own value); the twiddles go first and are stored with the
+    // first batch.
+    constexpr int U = 5;
+    {
+        const FT *twsf = tw_of<FT>(G, true);
+        const float2 *twsi = G.twy;
+        FT tf[(MAXN + 255) / 256];
+        float2 ti[(MAXN + 255) / 256];
+#pragma unroll
+        for (int u = 0; u < (MAXN + 255) / 256; u++) {
+            const int k = min((int)threadIdx.x + 256 * u, ny - 1);
+            tf[u] = twsf[k];
+            ti[u] = twsi[k];
         }
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int i = b0 + threadIdx.x + 256 * u;
-            if (i < tot) A[i] = v[u];
+        for (int u = 0; u < (MAXN + 255) / 256; u++) {
+            const int k = min((int)threadIdx.x + 256 * u, ny - 1);
+            twf[k] = tf[u];
+            twi[k] = ti[u];
         }
+    }
+    for (int i0 = 0; i0 < tot; i0 += 256 * U) {
+        FT v[U];
+        int ii[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int i = min(i0 + (int)threadIdx.x + 256 * u, tot - 1);
+            const int si = CS == 1 ? 0 : (i >= per) + (i >= 2 * per) + (i >= 3 * per);
+            const int j = i - si * per;
+            const int f = fdiv(j, ldf(G.dfny)), ky = j - f * ny;
+            ii[u] = i;
+            v[u] = specf[(long)(s0 + si) * G.nspec + coff[f] + ky];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) A[ii[u]] = v[u];
     }
     __syncthreads();
     FT *res = fft_run<-1, FT, RM>(A, B, ny, ns * ncg * npsr, G.nsty, G.fsy, twf);
-    if (G.scale) {
-        for (int i = threadIdx.x; i < tot; i += blockDim.x) {
-            res[i].x *= (FR)G.fscale;
-            res[i].y *= (FR)G.fscale;
-        }
-        __syncthreads();
-    }
-    // The per-layer piston is in the null space of the tomography operator
-    // (GP annihilates constants, the periodic L2'L2 stencil sums to zero) and
-    // the right-hand side is orthogonal to it, so in exact arithmetic every
-    // residual has a zero DC bin.  FDPCG's DC block is regularised only by
-    // max(psd)*sqrt(eps) (maos/fdpcg.c:396-401), so rounding left in that bin
-    // would be amplified ~1e8 and change alpha/beta.  Zeroing the DC input is
-    // therefore exact and keeps the iteration on the FP64 one.
-    if (G.zero_dc) {
-        for (int ci = 0; ci < ncg; ci++) {
-            if (G.colkx[c0 + ci] != 0) continue;
-            for (int t = threadIdx.x; t < ns * npsr; t += blockDim.x) {
-                const int si = t / npsr, l = t - si * npsr;
-                res[si * per + (ci * npsr + l) * ny] = CT<FT>::make(0, 0);
-            }
-        }
-        __syncthreads();
-    }
-    // Block multiply, and the FP64 partial of r.z taken on the spectrum:
-    // sum_xy r z = sc * sum_k conj(R_k) Z_k over the full spectrum (Parseval;
-    // sc = fscale when scaled, and R here already carries that fscale), i.e.
-    // the half-spectrum sum with weight 2 off the kx = 0 and kx = n/2 columns.
     float2 *C = reinterpret_cast<float2 *>(res == A ? B : A);
     float2 *D = reinterpret_cast<float2 *>(res);
-    // BS > 0: block size known at compile time (loads of a block row issued
-    // together); BS = 0: runtime G.bs.  Blocks are stored in group order.
-    const int bs = BS > 0 ? BS : G.bs;
-    const int b0 = G.grp_blk[g], b1 = G.grp_blk[g + 1];
+    const FR sc = G.scale ? (FR)G.fscale : FR(1);
+    const int zd = G.zero_dc, d32 = G.dot32;
     double dot[CS];
 #pragma unroll
     for (int si = 0; si < CS; si++) dot[si] = 0;
-    for (int t = threadIdx.x; t < (b1 - b0) * bs; t += blockDim.x) {
-        const int bi = b0 + t / bs;
-        const int m = t - (t / bs) * bs;
-        const float2 *M = G.Minv + (long)bi * bs * bs;
-        const int *bp = G.blkpos + (long)bi * bs;
-        FT y[CS];
+    if constexpr (BS > 0) {
 #pragma unroll
-        for (int si = 0; si < CS; si++) y[si] = CT<FT>::make(0, 0);
-#pragma unroll
-        for (int k = 0; k < bs; k++) {
-            const int pe = bp[k];
-            const int pos = pe & 0x1fffffff;
-            const FR cj = ((pe >> 30) & 1) ? FR(-1) : FR(1);
-            const float2 mk = M[m + k * bs];
-            const FT mm = CT<FT>::make(mk.x, mk.y);
-#pragma unroll
-            for (int si = 0; si < CS; si++) {
-                if (si < ns) {
-                    FT v = res[si * per + pos];
-                    v.y *= cj;
-                    y[si] = cadd(y[si], cmul(mm, v));
-                }
-            }
-        }
-        const int pm = bp[m];
-        if (!((pm >> 29) & 1)) {
-            const int pos = pm & 0x1fffffff;
-            const FR cj = ((pm >> 30) & 1) ? FR(-1) : FR(1);
-            const int kx = G.colkx[c0 + fdiv(fdiv(pos, ldf(G.dfny)), ldf(G.dnpsr))];
-            const double wgt = (kx == 0 || 2 * kx == G.fnx) ? 1.0 : 2.0;
-#pragma unroll
-            for (int si = 0; si < CS; si++) {
-                if (si < ns) {
-                    const FT yy = CT<FT>::make(y[si].x, cj * y[si].y);
-                    C[si * per + pos] = make_float2((float)yy.x, (float)yy.y);
-                    const FT rv = res[si * per + pos];
-                    dot[si] += wgt * ((double)rv.x * (double)yy.x + (double)rv.y * (double)yy.y);
-                }
-            }
+        for (int u = 0; u < RP; u++) {
+            const int t = threadIdx.x + 256 * u;
+            if (t < nrow) col_row<FT, CS, BS>(mr[u], 1, pr[u], pmr[u], BS, res, per, ns, C, dot, sc, zd, d32);
         }
     }
-#pragma unroll
-    for (int si = 0; si < CS; si++) {
-        const double d = block_sum(dot[si], sh);  // its barriers also order C before the inverse FFT
-        if (threadIdx.x == 0 && si < ns) SC.part_rz[(long)(s0 + si) * SC.maxpart + g] = d;
+    for (int t = threadIdx.x + 256 * (BS > 0 ? RP : 0); t < nrow; t += blockDim.x) {
+        const int q = t / bs, m = t - q * bs;
+        const float2 *M = G.Minv + (long)(b0 + q) * bs * bs;
+        const int *bp = G.blkpos + (long)(b0 + q) * bs;
+        col_row<FT, CS, BS>(M + m, bs, bp, bp[m], bs, res, per, ns, C, dot, sc, zd, d32);
     }
+    block_sum_n<CS>(dot, sh, d32);  // its barriers also order C before the inverse FFT
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int si = 0; si < CS; si++)
+            if (si < ns)
+                SC.part_rz[(long)(s0 + si) * SC.maxpart + g] =
+                    d32 ? (double)((float)dot[si] * (float)sc) : dot[si] * (double)sc;
     float2 *res2 = fft_run<1, float2, RM>(C, D, ny, ns * ncg * npsr, G.nsty, G.fsy, twi);
     for (int i = threadIdx.x; i < tot; i += blockDim.x) {
         const int si = CS == 1 ? 0 : (i >= per) + (i >= 2 * per) + (i >= 3 * per);
         const int j = i - si * per;
         const int f = fdiv(j, ldf(G.dfny)), ky = j - f * ny;
-        const int ci = fdiv(f, ldf(G.dnpsr)), l = f - ci * npsr;
-        speci[(long)(s0 + si) * G.nspec + G.soff[l] + (long)G.colkx[c0 + ci] * ny + ky] = res2[i];
+        speci[(long)(s0 + si) * G.nspec + coff[f] + ky] = res2[i];
     }
 }
 
@@ -1135,10 +1421,10 @@ __global__ __launch_bounds__(256) void k_irow(KARGS, int slot, int osel) {
     load_xp(0);
     double beta = 0, alpha = 0;
     if (MODE != IROW_Z) {
-        const double rz = reduce_partials(SC.part_rz + (long)s * SC.maxpart, SC.npart_col, sh);
+        const double rz = reduce_partials(SC.part_rz + (long)s * SC.maxpart, SC.npart_col, sh, G.dot32);
         double *hist = SC.rz + (long)s * (SC.maxit + 2);
         if (MODE == IROW_UPDATE) {
-            beta = rz / hist[slot];
+            beta = ddiv(rz, hist[slot], G.dot32);
             if (!isfinite(beta)) beta = 0;
             alpha = SC.alpha[s];
         }
@@ -1230,6 +1516,11 @@ static void launch_layer_rt(const LaunchTab &t, int matched, int cs, int psel_, 
                             hipStream_t st) {
     const dim3 grid(t.nwg), blk(256);
     if (!matched) hipLaunchKernelGGL((k_layer<LHS, RT>), grid, blk, 0, st, KT, psel_, osel, partsel);
+    else if (matched == 2) {  // 16 x 64 tiles
+        if (cs == 4) hipLaunchKernelGGL((k_layer_m4<LHS, RT, 4>), grid, blk, 0, st, KT, psel_, osel, partsel);
+        else if (cs == 2) hipLaunchKernelGGL((k_layer_m4<LHS, RT, 2>), grid, blk, 0, st, KT, psel_, osel, partsel);
+        else hipLaunchKernelGGL((k_layer_m4<LHS, RT, 1>), grid, blk, 0, st, KT, psel_, osel, partsel);
+    }
     else if (cs == 4) hipLaunchKernelGGL((k_layer_m<LHS, RT, 4>), grid, blk, 0, st, KT, psel_, osel, partsel);
     else if (cs == 2) hipLaunchKernelGGL((k_layer_m<LHS, RT, 2>), grid, blk, 0, st, KT, psel_, osel, partsel);
     else hipLaunchKernelGGL((k_layer_m<LHS, RT, 1>), grid, blk, 0, st, KT, psel_, osel, partsel);
@@ -1277,7 +1568,7 @@ void launch_row(const LaunchTab &t, int mode, int rt64, int spec64, int rm, int 
 // forward and inverse twiddles.
 size_t col_lds(int maxcol, int npsr, int fny, int f64, int cs) {
     return (f64 ? sizeof(double2) : sizeof(float2)) * (2 * (size_t)cs * maxcol * npsr * fny + fny) +
-           sizeof(float2) * fny;
+           sizeof(float2) * fny + sizeof(int) * (size_t)maxcol * npsr;
 }
 template <typename FT, int CS, int RM>
 static void launch_col_cs(const LaunchTab &t, int bs, size_t lds, hipStream_t st) {

@@ -26,6 +26,8 @@ constexpr int MAXST = 8;      // FFT stages
 constexpr int TILE = 16;      // output tile edge of the WFS / layer passes
 constexpr int TAPS = 4;       // max ploc samples landing on one layer node per axis
 constexpr int ROWS = 16;      // rows per row-FFT workgroup (8 complex FFTs)
+constexpr int LTY = 64;       // tile height of the 16 x 64 layer kernel (k_layer_m4)
+constexpr int MAXN = 1024;    // largest FFT length (layer grid edge)
 
 // Matched-sampling ray of one (WFS, layer) pair: ploc index i maps to layer
 // nodes i + c and i + c + 1 with a constant fraction f, for i in [a0, a1)
@@ -66,6 +68,7 @@ struct Geom {
     const int4 *hxt_x, *hxt_y;    // per (w,l) per layer index: up to 4 ploc indices (-1 pad)
     long hxt_xoff[MAXL + 1];      // offset of layer l in a (w,*) transposed x table
     int matched;                  // every (w,l) ray is matched sampling (fast path)
+    int lty;                      // layer tile height: TILE, or LTY (k_layer_m4)
     const MRay *mray;             // [w*MAXL + l]
     // subapertures
     const int *saidx;             // per WFS per ploc point: subaperture with that lower-left corner, -1
@@ -82,6 +85,7 @@ struct Geom {
     // FDPCG
     int precond, bs, scale;
     int zero_dc;                  // zero the DC bin of the residual spectrum (see k_col)
+    int dot32;                    // precision mode 3: dot products and alpha/beta in FP32
     int fnx, fny;                 // FFT sizes (all layers equal)
     long nspec;                   // complex entries per system (sum over layers (nx/2+1)*ny)
     long soff[MAXL + 1];
@@ -98,14 +102,20 @@ struct Geom {
     const int *grp_col;           // per group: first column index into colkx
     const int *colkx;             // column kx list (per group, contiguous)
     const int *grp_blk;           // per group: first block (group order), ngroup+1 entries
-    const int *blkpos;            // per block in group order (nbhalf*bs):
-                                  //   LDS position | (conj << 30) | (nowrite << 29)
+    const int *blkpos;            // per block in group order (nbhalf*bs): LDS position | BP_* flags
     int maxcol;                   // max columns in one group
     // row blocks of the row FFT kernels
     int nrowblk;
     const int2 *rowblk;           // (layer, first row), ROWS rows each
 };
 
+
+// Flags of a blkpos entry (the LDS position is the low BP_POS bits).
+constexpr int BP_CONJ = 1 << 30;      // the entry is the conjugate of the stored half-spectrum bin
+constexpr int BP_NOWRITE = 1 << 29;   // another block writes this bin
+constexpr int BP_DC = 1 << 28;        // the bin is a layer's DC bin
+constexpr int BP_W1 = 1 << 27;        // the bin is on the kx = 0 or kx = nx/2 column (Parseval weight 1)
+constexpr int BP_POS = (1 << 27) - 1;
 
 // Per-plan scalar state (device, FP64).
 struct Scal {
@@ -150,6 +160,7 @@ struct Job {
 struct WGI {
     unsigned short job, bx, by, bz;
 };
+static_assert(sizeof(WGI) == 8, "workgroup entries are read as one 8-byte word");
 
 // Buffer selectors (which state array of the plan a kernel reads/writes).
 enum { BUF_X = 0, BUF_P = 1, BUF_Z = 2, BUF_AP = 3, BUF_B = 4 };

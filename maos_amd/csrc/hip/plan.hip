@@ -135,6 +135,7 @@ struct gpu_tomo {
     DevBuf<double2> twx64, twy64;
     DevBuf<int> grp_col, colkx, grp_blk, blkpos;
     std::vector<int> blk_order;  // half-spectrum block id of each block in group order
+    std::vector<int> blkpos_host;  // blkpos, host copy
     DevBuf<int2> rowblk;
     // state
     DevBuf<float> x, z, p, g, tt;
@@ -358,6 +359,20 @@ std::vector<float2> half_blocks(const mt_recon_t *R, long nx0, long ny0) {
     }
     return out;
 }
+// The blocks as the column kernel reads them: group order, and with
+// zero_dc the columns of DC entries zeroed, which applies each block to the
+// residual spectrum with its DC bins set to zero (the kernel leaves the
+// input as it is and drops those bins from r.z).
+std::vector<float2> col_blocks(const mt_recon_t *R, const gpu_tomo *g) {
+    const long bs = R->fd_bs;
+    std::vector<float2> M = group_order(half_blocks(R, g->G.fnx, g->G.fny), g->blk_order, bs);
+    if (g->G.zero_dc)
+        for (size_t j = 0; j < g->blk_order.size(); j++)
+            for (long k = 0; k < bs; k++)
+                if (g->blkpos_host[j * bs + k] & BP_DC)
+                    for (long m = 0; m < bs; m++) M[j * bs * bs + m + k * bs] = make_float2(0.f, 0.f);
+    return M;
+}
 
 void build_plan(gpu_tomo *g, const mt_recon_t *R) {
     if (!R || R->npsr <= 0 || R->npsr > MAXL || R->nwfs <= 0) fail(MT_EINVAL, "invalid recon (npsr/nwfs)");
@@ -510,8 +525,15 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
                 matched = 0;
         }
     // the matched kernels wrap the L2 window once: layers of at least TILE + 2
+    // (16 x 16 tiles) or LTY + 2 rows (16 x 64 tiles)
+    G.lty = TILE;
     for (int l = 0; l < R->npsr; l++)
         if (R->xnx[l] < TILE + 2 || R->xny[l] < TILE + 2) matched = 0;
+    if (matched) {
+        G.lty = LTY;
+        for (int l = 0; l < R->npsr; l++)
+            if (R->xny[l] < LTY + 2) G.lty = TILE;
+    }
     G.matched = matched;
     g->mray.upload(hm);
     G.mray = g->mray.p;
@@ -628,6 +650,7 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
         if (R->xnx[l] != nx0 || R->xny[l] != ny0)
             fail(MT_EINVAL, "all layers must share one grid size (equal atmr.os)");
     if ((nx0 & 1) || (ny0 & 1)) fail(MT_EINVAL, "layer grids must have even size");
+    if (nx0 > MAXN || ny0 > MAXN) fail(MT_EINVAL, "layer grid larger than 1024");
     G.fnx = nx0;
     G.fny = ny0;
     {
@@ -733,7 +756,9 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
                     const Ent &e = ent[b * bs + m];
                     const int pos = (colslot[e.kx] * R->npsr + e.l) * ny0 + e.ky;
                     int &wr = written[(size_t)gg * maxcol * R->npsr * ny0 + pos];
-                    hbpos[b * bs + m] = pos | (e.cj << 30) | (wr ? (1 << 29) : 0);
+                    hbpos[b * bs + m] = pos | (e.cj ? BP_CONJ : 0) | (wr ? BP_NOWRITE : 0) |
+                                        (e.kx == 0 && e.ky == 0 ? BP_DC : 0) |
+                                        (e.kx == 0 || 2 * e.kx == nx0 ? BP_W1 : 0);
                     wr++;
                 }
             }
@@ -756,11 +781,13 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
             for (long m = 0; m < bs; m++) hbpos_g[j * bs + m] = hbpos[(long)hbid[j] * bs + m];
         g->grp_blk.upload(hgb);
         g->blkpos.upload(hbpos_g);
+        g->blkpos_host = hbpos_g;
         G.grp_col = g->grp_col.p;
         G.colkx = g->colkx.p;
         G.grp_blk = g->grp_blk.p;
         G.blkpos = g->blkpos.p;
-        g->Minv.upload(group_order(half_blocks(R, nx0, ny0), g->blk_order, bs));
+        if ((long)G.maxcol * R->npsr * ny0 > BP_POS) fail(MT_EINVAL, "FDPCG column group too large");
+        g->Minv.upload(col_blocks(R, g));
         G.Minv = g->Minv.p;
         if (col_lds(G.maxcol, G.npsr, G.fny, 1, 1) > 64 * 1024)
             fail(MT_EINVAL, "FDPCG column group too large for LDS");
@@ -772,7 +799,8 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
 // equal when FDPCG is used).
 int layer_tiles(const Geom &G) {
     int t = 1;
-    for (int l = 0; l < G.npsr; l++) t = std::max(t, ((G.xnx[l] + TILE - 1) / TILE) * ((G.xny[l] + TILE - 1) / TILE));
+    for (int l = 0; l < G.npsr; l++)
+        t = std::max(t, ((G.xnx[l] + TILE - 1) / TILE) * ((G.xny[l] + G.lty - 1) / G.lty));
     return t;
 }
 
@@ -969,7 +997,11 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
                                         : pick_cs(jw_of(js, [&](int j) { return (long)ptiles(j) * G(j).nwfs; }),
                                                   sb.pos == 1 ? 4 : 2, 512);
                 });
-    build_stage(b, b.layer, [&](int j, std::vector<int> &k) { k = {G(j).matched}; return true; },
+    build_stage(b, b.layer,
+                [&](int j, std::vector<int> &k) {
+                    k = {G(j).matched ? (G(j).lty == LTY ? 2 : 1) : 0};
+                    return true;
+                },
                 [&](const Sub &sb, int j) {
                     return JobGrid{layer_tiles(G(j)), G(j).npsr, sb.matched ? ceil_div(NS(j), sb.cs) : NS(j)};
                 },
@@ -1249,7 +1281,7 @@ int gpu_tomo_update_cn2(gpu_tomo_t *g, const mt_recon_t *recon) {
         if (g->G.precond) {
             const long nbh = recon->fd_nbhalf, bs = recon->fd_bs;
             if (bs != g->G.bs || (long)g->Minv.n != nbh * bs * bs) fail(MT_EINVAL, "FDPCG layout changed");
-            std::vector<float2> hM = group_order(half_blocks(recon, g->G.fnx, g->G.fny), g->blk_order, bs);
+            std::vector<float2> hM = col_blocks(recon, g);
             HIPCHK(hipMemcpy(g->Minv.p, hM.data(), sizeof(float2) * hM.size(), hipMemcpyHostToDevice));
         }
         // in-place update of the device plan: tables and captured graphs,
@@ -1567,13 +1599,15 @@ int gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice) {
 }
 
 int gpu_tomo_precision(gpu_tomo_t *g, int mode) {
-    if (!g || mode < 0 || mode > 2) return MT_EINVAL;
+    if (!g || mode < 0 || mode > 3) return MT_EINVAL;
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
         HIPCHK(hipStreamSynchronize(g->stream));
         HIPCHK(hipDeviceSynchronize());
-        g->rt64 = mode >= 1;
+        g->rt64 = mode == 1 || mode == 2;
         g->spec64 = mode == 1;
+        g->G.dot32 = mode == 3;
+        upload_plandev(g);
         drop_batches(g);
     });
 }

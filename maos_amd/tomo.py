@@ -155,7 +155,8 @@ class GpuTomo:
 
     def precision(self, mode):
         """0: FP32 residual and spectrum; 1: FP64 residual recurrence and
-        forward spectrum (default); 2: FP64 residual, FP32 spectrum."""
+        forward spectrum (default); 2: FP64 residual, FP32 spectrum; 3: as 0
+        with FP32 dot products and alpha/beta (FP32 everywhere)."""
         _abi.check(_abi.lib().gpu_tomo_precision(self._h, int(mode)), "gpu_tomo_precision")
 
     def info(self):
