@@ -323,6 +323,13 @@ void gpu_tomo_server_free(gpu_tomo_server_t *s);
  * counterpart. */
 int  gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice);
 
+/* Set the systems per workgroup of the WFS, layer and column kernels
+ * directly (choice[3]: each 1, 2 or 4, or 0 for the built-in heuristic),
+ * e.g. to replay an autotuned choice without timing.  Values above a
+ * kernel's maximum (WFS 2 when tomo.pos = 2, column 2 in precision modes 1
+ * and 2) are clamped when the launch is built.  No reference counterpart. */
+int  gpu_tomo_tune_set(gpu_tomo_t *g, const int *choice);
+
 /* Plan facts: matched-sampling ray path (1) or generic ray tables (0), FFT
  * sizes, FDPCG block size, largest FFT radix. */
 int  gpu_tomo_info(const gpu_tomo_t *g, int *matched, int *fnx, int *fny,
@@ -334,7 +341,11 @@ int  gpu_tomo_sizes(const gpu_tomo_t *g, long *nx, long *ng);
 /* Average device duration (ms) per launch of each kernel class over the
  * last gpu_tomo_solve_dev call made with profiling enabled (HIP events on the
  * stream the kernels run on), and the number of launches of that class.
- * names/ms/calls are parallel arrays of length *n (in: capacity). */
+ * names/ms/calls are parallel arrays of length *n (in: capacity).
+ * enable: 0 off, 1 on; n > 1 also launches each class whose kernels only
+ * read their inputs (TT, WFS, layer and column passes) n times back to back
+ * inside one event pair, which takes the event overhead out of the
+ * per-launch average (the solve's results are unchanged). */
 int  gpu_tomo_profile(gpu_tomo_t *g, int enable);
 int  gpu_tomo_profile_read(gpu_tomo_t *g, const char **names, double *ms,
                            double *calls, int *n);

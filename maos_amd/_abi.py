@@ -109,6 +109,7 @@ def lib():
     L.gpu_tomo_group_free.argtypes = [vp]
     L.gpu_tomo_group_free.restype = None
     L.gpu_tomo_autotune.argtypes = [vp, C.c_int, ip]
+    L.gpu_tomo_tune_set.argtypes = [vp, ip]
     L.gpu_tomo_info.argtypes = [vp, ip, ip, ip, ip, ip]
     L.gpu_tomo_server_create.argtypes = [C.c_int, C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(vp)]
     L.gpu_tomo_server_add_loop.argtypes = [vp, C.POINTER(MtRecon), C.c_int, ip]
@@ -143,7 +144,7 @@ EXPORTS = [
     "gpu_tomo_group_create", "gpu_tomo_group_solve", "gpu_tomo_group_free",
     "gpu_tomo_server_create", "gpu_tomo_server_add_loop", "gpu_tomo_server_remove_loop",
     "gpu_tomo_server_submit", "gpu_tomo_server_poll", "gpu_tomo_server_drain", "gpu_tomo_server_get_x",
-    "gpu_tomo_server_usage", "gpu_tomo_server_free", "gpu_tomo_autotune", "gpu_tomo_info", "gpu_tomo_residuals", "gpu_tomo_L", "gpu_tomo_R", "gpu_tomo_P",
+    "gpu_tomo_server_usage", "gpu_tomo_server_free", "gpu_tomo_autotune", "gpu_tomo_tune_set", "gpu_tomo_info", "gpu_tomo_residuals", "gpu_tomo_L", "gpu_tomo_R", "gpu_tomo_P",
     "gpu_tomo_sizes", "gpu_tomo_profile", "gpu_tomo_profile_read", "gpu_tomo_precision",
 ]
 

@@ -110,6 +110,12 @@ struct Batch {
 };
 }  // namespace
 
+// One profiled interval: n launches of kernel class kc between events a, b.
+struct ProfEv {
+    int kc, n;
+    hipEvent_t a, b;
+};
+
 struct gpu_tomo {
     int device = 0;
     int nbatch = 1, maxit = 30, warm = 0;
@@ -160,7 +166,7 @@ struct gpu_tomo {
     int tune_wfs = 0, tune_layer = 0, tune_col = 0;
     // profiling
     int prof = 0;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> evs;  // profiling events
+    std::vector<ProfEv> evs;  // profiling events
     double prof_ms[KC_N] = {0};
     int prof_cnt[KC_N] = {0};
 };
@@ -875,11 +881,19 @@ void alloc_state(gpu_tomo *g) {
 // Launch sequence.  Every launch goes through rec(), which brackets it with
 // HIP events on the plan's stream when profiling is on.
 // ---------------------------------------------------------------------------
-typedef std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> EvList;
+typedef std::vector<ProfEv> EvList;
+// Kernel classes whose launches only read their inputs (a repeated launch
+// writes the same outputs), so the profiler may launch them several times
+// back to back inside one event pair.
+bool kc_idempotent(int kc) {
+    return kc == KC_TTR || kc == KC_WFS || kc == KC_LAYER || kc == KC_COL || kc == KC_WFS_RHS ||
+           kc == KC_LAYER_RHS;
+}
 struct Rec {
     EvList *evs;
     hipStream_t st;
     bool on;
+    int reps = 1;  // launches per event pair of an idempotent class
     template <typename F>
     void operator()(int kc, F &&f) {
         if (on) {
@@ -887,9 +901,10 @@ struct Rec {
             HIPCHK(hipEventCreate(&a));
             HIPCHK(hipEventCreate(&b));
             HIPCHK(hipEventRecord(a, st));
-            f();
+            const int n = kc_idempotent(kc) ? std::max(reps, 1) : 1;
+            for (int i = 0; i < n; i++) f();
             HIPCHK(hipEventRecord(b, st));
-            evs->push_back({kc, {a, b}});
+            evs->push_back(ProfEv{kc, n, a, b});
         } else {
             f();
         }
@@ -902,9 +917,53 @@ struct Rec {
 struct JobGrid {
     int gx, gy, gz;
 };
+// Workgroup i of a launch runs on XCD i mod 8 (round-robin dispatch), and
+// each XCD has its own L2.  Reorder a table so that all workgroups of one
+// (job, system group) land on one XCD: the data they share (the layers read
+// by every WFS tile, the WFS fields read by every layer tile) then stays in
+// one L2 instead of being fetched by all eight.  Buckets go to the least
+// loaded XCD in order; positions whose XCD has run out take the longest
+// remaining queue, so a table with fewer buckets than XCDs keeps its order.
+// by_x: bucket by (job, x block) instead, for the column kernel, whose
+// shared data (the FDPCG blocks of a column group) follows x.
+constexpr int NXCD = 8;
+std::vector<WGI> xcd_order(std::vector<WGI> tab, bool by_x) {
+    if (by_x)
+        std::stable_sort(tab.begin(), tab.end(), [](const WGI &a, const WGI &b) {
+            return a.job != b.job ? a.job < b.job : a.bx < b.bx;
+        });
+    auto same = [&](const WGI &a, const WGI &b) {
+        return a.job == b.job && (by_x ? a.bx == b.bx : a.bz == b.bz);
+    };
+    std::vector<std::vector<WGI>> q(NXCD);
+    std::vector<size_t> load(NXCD, 0);
+    size_t i = 0;
+    while (i < tab.size()) {
+        size_t e = i;
+        while (e < tab.size() && same(tab[e], tab[i])) e++;
+        const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        q[k].insert(q[k].end(), tab.begin() + i, tab.begin() + e);
+        load[k] += e - i;
+        i = e;
+    }
+    std::vector<WGI> out;
+    out.reserve(tab.size());
+    std::vector<size_t> pos(NXCD, 0);
+    for (size_t n = 0; n < tab.size(); n++) {
+        int k = (int)(n % NXCD);
+        if (pos[k] == q[k].size()) {
+            k = 0;
+            for (int c = 1; c < NXCD; c++)
+                if (q[c].size() - pos[c] > q[k].size() - pos[k]) k = c;
+        }
+        out.push_back(q[k][pos[k]++]);
+    }
+    return out;
+}
+
 // Partition the jobs by a variant key and emit one Sub per partition.
 template <typename KeyF, typename GridF, typename SubF>
-void build_stage(Batch &B, Stage &out, KeyF key, GridF grid, SubF fill) {
+void build_stage(Batch &B, Stage &out, KeyF key, GridF grid, SubF fill, bool xcd_by_x = false) {
     std::map<std::vector<int>, std::vector<int>> parts;
     for (size_t j = 0; j < B.hjob.size(); j++) {
         std::vector<int> k;
@@ -923,6 +982,7 @@ void build_stage(Batch &B, Stage &out, KeyF key, GridF grid, SubF fill) {
                         tab.push_back(WGI{(unsigned short)j, (unsigned short)x, (unsigned short)y, (unsigned short)z});
         }
         if (tab.empty()) continue;
+        tab = xcd_order(std::move(tab), xcd_by_x);
         sub->nwg = (int)tab.size();
         sub->wgi.upload(tab);
         out.push_back(std::move(sub));
@@ -1044,7 +1104,8 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
                             if (col_lds(G(j).maxcol, G(j).npsr, G(j).fny, b.spec64, sb.cs) > 64 * 1024) sb.cs = 1;
                         for (int j : js)
                             sb.lds = std::max(sb.lds, col_lds(G(j).maxcol, G(j).npsr, G(j).fny, b.spec64, sb.cs));
-                    });
+                    },
+                    true);
     }
     build_stage(b, b.fin, [&](int, std::vector<int> &) { return true; },
                 [&](const Sub &, int j) { return JobGrid{1, 1, NS(j)}; },
@@ -1154,11 +1215,11 @@ void prof_collect(gpu_tomo *g) {
     }
     for (auto &e : g->evs) {
         float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, e.second.first, e.second.second));
-        g->prof_ms[e.first] += ms;
-        g->prof_cnt[e.first]++;
-        (void)hipEventDestroy(e.second.first);
-        (void)hipEventDestroy(e.second.second);
+        HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
+        g->prof_ms[e.kc] += ms;
+        g->prof_cnt[e.kc] += e.n;
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
     }
     g->evs.clear();
 }
@@ -1306,8 +1367,8 @@ void gpu_tomo_free(gpu_tomo_t *g) {
     (void)hipDeviceSynchronize();
     drop_batches(g);
     for (auto &e : g->evs) {
-        (void)hipEventDestroy(e.second.first);
-        (void)hipEventDestroy(e.second.second);
+        (void)hipEventDestroy(e.a);
+        (void)hipEventDestroy(e.b);
     }
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
@@ -1404,7 +1465,7 @@ int gpu_tomo_solve_dev(gpu_tomo_t *g, const float *d_grad, float *d_x, int nbatc
             }
             HIPCHK(hipGraphLaunch(g->gexec, st));
         } else {
-            Rec rec{&g->evs, st, g->prof != 0};
+            Rec rec{&g->evs, st, g->prof != 0, g->prof};
             enqueue_solve(B, rec);
             if (g->prof) {
                 if (st != g->stream) HIPCHK(hipStreamSynchronize(st));
@@ -1542,6 +1603,20 @@ void gpu_tomo_group_free(gpu_tomo_group_t *grp) {
     delete grp;
 }
 
+int gpu_tomo_tune_set(gpu_tomo_t *g, const int *choice) {
+    if (!g || !choice) return MT_EINVAL;
+    for (int i = 0; i < 3; i++)
+        if (choice[i] != 0 && choice[i] != 1 && choice[i] != 2 && choice[i] != 4) return MT_EINVAL;
+    return guarded([&] {
+        HIPCHK(hipSetDevice(g->device));
+        HIPCHK(hipDeviceSynchronize());
+        g->tune_wfs = choice[0];
+        g->tune_layer = choice[1];
+        g->tune_col = choice[2];
+        drop_batches(g);
+    });
+}
+
 int gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice) {
     if (!g || nsys <= 0 || nsys > g->nbatch) return MT_EINVAL;
     return guarded([&] {
@@ -1613,8 +1688,8 @@ int gpu_tomo_precision(gpu_tomo_t *g, int mode) {
 }
 
 int gpu_tomo_profile(gpu_tomo_t *g, int enable) {
-    if (!g) return MT_EINVAL;
-    g->prof = enable ? 1 : 0;
+    if (!g || enable < 0) return MT_EINVAL;
+    g->prof = enable;
     return MT_OK;
 }
 

@@ -172,6 +172,11 @@ class GpuTomo:
         _abi.check(_abi.lib().gpu_tomo_autotune(self._h, int(nsys or self.nbatch), ch), "gpu_tomo_autotune")
         return tuple(ch)
 
+    def tune_set(self, wfs, layer, col):
+        """gpu_tomo_tune_set: fix the systems per workgroup (0: heuristic)."""
+        ch = (C.c_int * 3)(int(wfs), int(layer), int(col))
+        _abi.check(_abi.lib().gpu_tomo_tune_set(self._h, ch), "gpu_tomo_tune_set")
+
     def close(self):
         if getattr(self, "_h", None):
             _abi.lib().gpu_tomo_free(self._h)
@@ -259,8 +264,10 @@ class GpuTomo:
         _abi.check(_abi.lib().gpu_tomo_residuals(self._h, _dp(out), nb), "gpu_tomo_residuals")
         return out
 
-    def profile(self, enable=True):
-        _abi.lib().gpu_tomo_profile(self._h, 1 if enable else 0)
+    def profile(self, enable=True, reps=1):
+        """Enable per-kernel-class timing; reps > 1 repeats the read-only
+        classes back to back inside one event pair."""
+        _abi.check(_abi.lib().gpu_tomo_profile(self._h, max(1, int(reps)) if enable else 0), "gpu_tomo_profile")
 
     def profile_read(self):
         n = C.c_int(16)
