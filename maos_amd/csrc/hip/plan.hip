@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <set>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -1030,6 +1031,21 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
         for (int j : js) v.push_back({per(j), NS(j)});
         return v;
     };
+    // A batch over several plans (uneven groups, the request loop) is mostly
+    // small systems, whose kernels are short enough that the number of
+    // launches sets the time.  There the FFT and column stages share one
+    // launch across plans, using the variant with the radix-5/7 butterflies
+    // compiled in and the runtime block size when the plans differ (per
+    // element the same arithmetic as the specialised variants).
+    const bool merge = std::set<const gpu_tomo *>(b.plan.begin(), b.plan.end()).size() > 1;
+    int rm_all = 3, bs_all = -1;
+    for (size_t j = 0; j < b.hjob.size(); j++) {
+        if (G((int)j).rmax > 4) rm_all = 7;
+        const int bs = G((int)j).bs;
+        const int bk = (bs == 2 || bs == 3 || bs == 4 || bs == 6 || bs == 8) ? bs : 0;
+        bs_all = bs_all < 0 || bs_all == bk ? bk : 0;
+    }
+    auto rm_key = [&](int j) { return merge ? rm_all : G(j).rmax > 4 ? 7 : 3; };
     // per-stage tables
     build_stage(b, b.zero, [&](int, std::vector<int> &) { return true; },
                 [&](const Sub &, int) { return JobGrid{1, 1, 1}; }, [&](Sub &, const std::vector<int> &, const std::vector<int> &) {});
@@ -1074,13 +1090,13 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
                                                   4, 512);
                 });
     if (b.precond) {
-        build_stage(b, b.row, [&](int j, std::vector<int> &k) { k = {G(j).rmax > 4 ? 7 : 3}; return true; },
+        build_stage(b, b.row, [&](int j, std::vector<int> &k) { k = {rm_key(j)}; return true; },
                     [&](const Sub &, int j) { return JobGrid{G(j).nrowblk, 1, NS(j)}; },
                     [&](Sub &sb, const std::vector<int> &k, const std::vector<int> &js) {
                         sb.rm = k[0];
                         for (int j : js) sb.lds = std::max(sb.lds, row_lds(G(j).fnx, b.spec64));
                     });
-        build_stage(b, b.irow, [&](int j, std::vector<int> &k) { k = {G(j).rmax > 4 ? 7 : 3}; return true; },
+        build_stage(b, b.irow, [&](int j, std::vector<int> &k) { k = {rm_key(j)}; return true; },
                     [&](const Sub &, int j) { return JobGrid{G(j).nrowblk, 1, NS(j)}; },
                     [&](Sub &sb, const std::vector<int> &k, const std::vector<int> &js) {
                         sb.rm = k[0];
@@ -1089,7 +1105,8 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
         build_stage(b, b.col,
                     [&](int j, std::vector<int> &k) {
                         const int bs = G(j).bs;
-                        k = {(bs == 2 || bs == 3 || bs == 4 || bs == 6 || bs == 8) ? bs : 0, G(j).rmax > 4 ? 7 : 3};
+                        const int bk = (bs == 2 || bs == 3 || bs == 4 || bs == 6 || bs == 8) ? bs : 0;
+                        k = {merge ? bs_all : bk, rm_key(j)};
                         return true;
                     },
                     [&](const Sub &sb, int j) { return JobGrid{G(j).ngroup, 1, ceil_div(NS(j), sb.cs)}; },
