@@ -1472,6 +1472,34 @@ __global__ __launch_bounds__(256) void k_final(KARGS) {
 }
 
 // Copy / convert helpers.
+// Request-loop staging (one copy each way per batch instead of one per
+// system): SB workgroups per job; jobs that are off exit at entry.
+constexpr int SB = 16;
+__global__ __launch_bounds__(256) void k_stage_in(const Job *__restrict__ jobs, const StageOff *__restrict__ off,
+                                                  const float *__restrict__ src) {
+    const int j = blockIdx.x / SB, part = blockIdx.x - j * SB;
+    const Job J = jobs[j];
+    if (!J.on) return;
+    CPD &PD = *(const CONSTAS PlanDev *)J.pd;
+    const long ng = PD.G.ngtot, n = (long)J.nsys * ng;
+    float *dst = PD.g + (long)J.s0 * ng;
+    const float *s = src + off[j].g;
+    for (long i = (long)part * 256 + threadIdx.x; i < n; i += (long)SB * 256) dst[i] = s[i];
+}
+__global__ __launch_bounds__(256) void k_stage_out(const Job *__restrict__ jobs, const StageOff *__restrict__ off,
+                                                   float *__restrict__ dx, double *__restrict__ dres) {
+    const int j = blockIdx.x / SB, part = blockIdx.x - j * SB;
+    const Job J = jobs[j];
+    if (!J.on) return;
+    CPD &PD = *(const CONSTAS PlanDev *)J.pd;
+    const long nx = PD.G.nxtot, n = (long)J.nsys * nx;
+    const float *s = PD.x + (long)J.s0 * nx;
+    float *d = dx + off[j].x;
+    for (long i = (long)part * 256 + threadIdx.x; i < n; i += (long)SB * 256) d[i] = s[i];
+    if (part == 0)
+        for (int k = threadIdx.x; k < J.nsys; k += 256) dres[off[j].r + k] = PD.SC.res[J.s0 + k];
+}
+
 __global__ void k_copy(const float *a, float *b, long n) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
 }
@@ -1614,6 +1642,13 @@ void launch_final(const LaunchTab &t, hipStream_t st) {
 }
 void launch_zero_stop(const LaunchTab &t, hipStream_t st) {
     hipLaunchKernelGGL(k_zero_stop, dim3(t.nwg), dim3(256), 0, st, KT);
+}
+void launch_stage_in(const Job *jobs, int njobs, const StageOff *off, const float *src, hipStream_t st) {
+    if (njobs > 0) hipLaunchKernelGGL(k_stage_in, dim3((unsigned)(njobs * SB)), dim3(256), 0, st, jobs, off, src);
+}
+void launch_stage_out(const Job *jobs, int njobs, const StageOff *off, float *dx, double *dres, hipStream_t st) {
+    if (njobs > 0)
+        hipLaunchKernelGGL(k_stage_out, dim3((unsigned)(njobs * SB)), dim3(256), 0, st, jobs, off, dx, dres);
 }
 void launch_copy(const float *a, float *b, long n, hipStream_t st) {
     const long nb = (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;

@@ -162,6 +162,12 @@ struct WGI {
 };
 static_assert(sizeof(WGI) == 8, "workgroup entries are read as one 8-byte word");
 
+// Offsets of one job's entries in packed staging buffers (gradients,
+// solutions, residuals), in elements.
+struct StageOff {
+    long g, x, r;
+};
+
 // Buffer selectors (which state array of the plan a kernel reads/writes).
 enum { BUF_X = 0, BUF_P = 1, BUF_Z = 2, BUF_AP = 3, BUF_B = 4 };
 enum { PART_NONE = 0, PART_PAP = 1, PART_BB = 2 };

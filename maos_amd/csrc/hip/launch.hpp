@@ -31,4 +31,8 @@ void launch_irow(const LaunchTab &t, int mode, int rm, int slot, int osel, size_
 void launch_final(const LaunchTab &t, hipStream_t st);
 void launch_zero_stop(const LaunchTab &t, hipStream_t st);
 void launch_copy(const float *a, float *b, long n, hipStream_t st);
+// Packed staging of the jobs that are on: gradients from src into each
+// plan (in), solutions and residuals from each plan into dx, dres (out).
+void launch_stage_in(const Job *jobs, int njobs, const StageOff *off, const float *src, hipStream_t st);
+void launch_stage_out(const Job *jobs, int njobs, const StageOff *off, float *dx, double *dres, hipStream_t st);
 }  // namespace mtk

@@ -1444,13 +1444,18 @@ int gpu_tomo_solve_batch(gpu_tomo_t *g, int nsys, const double *grad, double *x,
 }
 
 // Capture the solve of batch B on a private stream into (graph, exec).
-void capture_solve(Batch &B, hipGraph_t *graph, hipGraphExec_t *exec, EvList *evs) {
+// Capture pre(stream), the solve and post(stream) into one graph.
+extern "C++" {
+template <typename Pre, typename Post>
+void capture_solve_with(Batch &B, hipGraph_t *graph, hipGraphExec_t *exec, EvList *evs, Pre pre, Post post) {
     hipStream_t cs;
     HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     Rec rec{evs, cs, false};
     HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
     try {
+        pre(cs);
         enqueue_solve(B, rec);
+        post(cs);
     } catch (...) {
         hipGraph_t junk;
         (void)hipStreamEndCapture(cs, &junk);
@@ -1460,6 +1465,10 @@ void capture_solve(Batch &B, hipGraph_t *graph, hipGraphExec_t *exec, EvList *ev
     HIPCHK(hipStreamEndCapture(cs, graph));
     (void)hipStreamDestroy(cs);
     HIPCHK(hipGraphInstantiate(exec, *graph, nullptr, nullptr, 0));
+}
+}  // extern "C++"
+void capture_solve(Batch &B, hipGraph_t *graph, hipGraphExec_t *exec, EvList *evs) {
+    capture_solve_with(B, graph, exec, evs, [](hipStream_t) {}, [](hipStream_t) {});
 }
 
 int gpu_tomo_solve_dev(gpu_tomo_t *g, const float *d_grad, float *d_x, int nbatch, int mode, void *stream) {
@@ -1745,8 +1754,6 @@ struct SrvLoop {
     gpu_tomo *plan = nullptr;
     size_t bytes = 0;
     bool busy = false;                 // a request of this loop is on the GPU
-    float *hgrad = nullptr, *hx = nullptr;  // pinned staging
-    double *hres = nullptr;            // pinned
     std::vector<double> xlast;         // last completed solution (host)
     double reslast = NAN;
 };
@@ -1774,6 +1781,14 @@ struct Flight {
     hipGraphExec_t exec = nullptr;
     Job *hjobs = nullptr;        // pinned copy of the job table (on flags set per launch)
     std::vector<int> job_of;     // loop id -> job index (-1: not in the batch)
+    // packed staging of all loops of the batch: one copy each way per launch
+    std::vector<StageOff> hoff;  // per job
+    DevBuf<StageOff> doff;
+    float *hin = nullptr, *hout = nullptr;  // pinned: gradients, solutions
+    double *hres = nullptr;                 // pinned: residuals
+    DevBuf<float> din, dout;
+    DevBuf<double> dres;
+    long nin = 0, nout = 0, nres = 0;
 };
 size_t plan_bytes(const gpu_tomo *g) {
     size_t b = 0;
@@ -1814,6 +1829,16 @@ void flight_drop(Flight &f) {
     f.graph = nullptr;
     if (f.hjobs) (void)hipHostFree(f.hjobs);
     f.hjobs = nullptr;
+    if (f.hin) (void)hipHostFree(f.hin);
+    if (f.hout) (void)hipHostFree(f.hout);
+    if (f.hres) (void)hipHostFree(f.hres);
+    f.hin = f.hout = nullptr;
+    f.hres = nullptr;
+    f.din.free();
+    f.dout.free();
+    f.dres.free();
+    f.doff.free();
+    f.hoff.clear();
     f.B.reset();
 }
 // (Re)build an idle flight's batch over all admitted loops (lock held).
@@ -1830,7 +1855,30 @@ void flight_build(gpu_tomo_server *S, Flight &f) {
     if (pl.empty()) return;
     f.B = make_batch(pl, std::vector<int>(pl.size(), 0), std::vector<int>(pl.size(), 1));
     HIPCHK(hipHostMalloc((void **)&f.hjobs, sizeof(Job) * pl.size()));
-    if (S->use_graph) capture_solve(*f.B, &f.graph, &f.exec, &S->evs);
+    f.hoff.resize(pl.size());
+    f.nin = f.nout = f.nres = 0;
+    for (size_t j = 0; j < pl.size(); j++) {
+        f.hoff[j] = StageOff{f.nin, f.nout, f.nres};
+        f.nin += pl[j]->G.ngtot;
+        f.nout += pl[j]->G.nxtot;
+        f.nres += 1;
+    }
+    HIPCHK(hipHostMalloc((void **)&f.hin, sizeof(float) * f.nin));
+    HIPCHK(hipHostMalloc((void **)&f.hout, sizeof(float) * f.nout));
+    HIPCHK(hipHostMalloc((void **)&f.hres, sizeof(double) * f.nres));
+    std::memset(f.hin, 0, sizeof(float) * f.nin);
+    f.din.alloc(f.nin);
+    f.dout.alloc(f.nout);
+    f.dres.alloc(f.nres);
+    f.doff.upload(f.hoff);
+    if (S->use_graph) {
+        Flight *fp = &f;
+        const int nj = (int)pl.size();
+        capture_solve_with(
+            *f.B, &f.graph, &f.exec, &S->evs,
+            [&](hipStream_t cs) { launch_stage_in(fp->B->jobs.p, nj, fp->doff.p, fp->din.p, cs); },
+            [&](hipStream_t cs) { launch_stage_out(fp->B->jobs.p, nj, fp->doff.p, fp->dout.p, fp->dres.p, cs); });
+    }
 }
 
 // Launch the requests in f.reqs on f.st (worker thread; f.lp resolved and
@@ -1841,28 +1889,29 @@ void srv_launch(gpu_tomo_server *S, Flight &f) {
         f.hjobs[j] = B.hjob[j];
         f.hjobs[j].on = 0;
     }
-    for (auto &r : f.reqs) f.hjobs[f.job_of[r.loop]].on = 1;
-    HIPCHK(hipMemcpyAsync(B.jobs.p, f.hjobs, sizeof(Job) * B.hjob.size(), hipMemcpyHostToDevice, f.st));
-    for (size_t i = 0; i < f.reqs.size(); i++) {
-        SrvLoop &L = *f.lp[i];
-        std::memcpy(L.hgrad, f.reqs[i].grad.data(), sizeof(float) * f.reqs[i].grad.size());
-        HIPCHK(hipMemcpyAsync(L.plan->g.p, L.hgrad, sizeof(float) * f.reqs[i].grad.size(), hipMemcpyHostToDevice,
-                              f.st));
+    for (auto &r : f.reqs) {
+        const int j = f.job_of[r.loop];
+        f.hjobs[j].on = 1;
+        std::memcpy(f.hin + f.hoff[j].g, r.grad.data(), sizeof(float) * r.grad.size());
     }
+    const int nj = (int)B.hjob.size();
+    HIPCHK(hipMemcpyAsync(B.jobs.p, f.hjobs, sizeof(Job) * nj, hipMemcpyHostToDevice, f.st));
+    HIPCHK(hipMemcpyAsync(f.din.p, f.hin, sizeof(float) * f.nin, hipMemcpyHostToDevice, f.st));
     if (S->use_graph) {
         HIPCHK(hipGraphLaunch(f.exec, f.st));
     } else {
+        launch_stage_in(B.jobs.p, nj, f.doff.p, f.din.p, f.st);
         Rec rec{&S->evs, f.st, false};
         enqueue_solve(B, rec);
+        launch_stage_out(B.jobs.p, nj, f.doff.p, f.dout.p, f.dres.p, f.st);
     }
-    for (size_t i = 0; i < f.reqs.size(); i++) {
-        SrvLoop &L = *f.lp[i];
-        HIPCHK(hipMemcpyAsync(L.hx, L.plan->x.p, sizeof(float) * L.plan->G.nxtot, hipMemcpyDeviceToHost, f.st));
-        HIPCHK(hipMemcpyAsync(L.hres, L.plan->res.p, sizeof(double), hipMemcpyDeviceToHost, f.st));
-    }
+    HIPCHK(hipMemcpyAsync(f.hout, f.dout.p, sizeof(float) * f.nout, hipMemcpyDeviceToHost, f.st));
+    HIPCHK(hipMemcpyAsync(f.hres, f.dres.p, sizeof(double) * f.nres, hipMemcpyDeviceToHost, f.st));
     HIPCHK(hipEventRecord(f.done, f.st));
     f.active = true;
 }
+
+constexpr double HOLD_US = 250.0;
 
 void srv_worker(gpu_tomo_server *S) {
     (void)hipSetDevice(S->device);
@@ -1880,8 +1929,9 @@ void srv_worker(gpu_tomo_server *S) {
                 for (size_t i = 0; i < f.reqs.size(); i++) {
                     SrvLoop &L = *f.lp[i];
                     const SrvReq &r = f.reqs[i];
-                    L.xlast.assign(L.hx, L.hx + L.plan->G.nxtot);
-                    L.reslast = *L.hres;
+                    const StageOff &o = f.hoff[f.job_of[r.loop]];
+                    L.xlast.assign(f.hout + o.x, f.hout + o.x + L.plan->G.nxtot);
+                    L.reslast = f.hres[o.r];
                     L.busy = false;
                     S->done.push_back(SrvDone{r.loop, r.tag, L.reslast,
                                               std::chrono::duration<double, std::micro>(now - r.t_submit).count()});
@@ -1893,11 +1943,34 @@ void srv_worker(gpu_tomo_server *S) {
                 did = true;
                 S->cv_done.notify_all();
             }
-            // admit pending requests of idle loops into a free flight
+            // admit pending requests of idle loops into a free flight.  A
+            // launch costs about the same for few loops as for all (the
+            // small systems' kernels are launch-bound), so while another
+            // flight is on the GPU a flight that would carry fewer than half
+            // of the loops waits for more requests, up to HOLD_US after the
+            // oldest of them arrived.
+            bool anyactive = false;
+            for (auto &f : S->flights) anyactive |= f.active;
             for (auto &f : S->flights) {
                 if (f.active) continue;
                 {
                     std::lock_guard<std::mutex> lk(S->mu);
+                    if (anyactive) {
+                        int nl = 0, ready = 0;
+                        Clock::time_point oldest = Clock::now();
+                        for (auto &L : S->loops) nl += L->plan != nullptr;
+                        std::vector<int> seen;
+                        for (auto &r : S->pending) {
+                            if (S->loops[r.loop]->busy ||
+                                std::find(seen.begin(), seen.end(), r.loop) != seen.end())
+                                continue;
+                            seen.push_back(r.loop);
+                            ready++;
+                            oldest = std::min(oldest, r.t_submit);
+                        }
+                        const double waited = std::chrono::duration<double, std::micro>(Clock::now() - oldest).count();
+                        if (2 * ready < nl && waited < HOLD_US) break;
+                    }
                     std::vector<int> taken;
                     for (auto it = S->pending.begin(); it != S->pending.end() && (int)f.reqs.size() < S->max_batch;) {
                         SrvLoop &L = *S->loops[it->loop];
@@ -1916,11 +1989,12 @@ void srv_worker(gpu_tomo_server *S) {
                 }
                 if (f.reqs.empty()) break;
                 srv_launch(S, f);
+                anyactive = true;
                 did = true;
             }
             if (!did) {
                 std::unique_lock<std::mutex> lk(S->mu);
-                bool anyactive = false;
+                anyactive = false;
                 for (auto &f : S->flights) anyactive |= f.active;
                 if (anyactive) {
                     lk.unlock();
@@ -1993,10 +2067,6 @@ int gpu_tomo_server_add_loop(gpu_tomo_server_t *S, const mt_recon_t *recon, int 
         std::unique_ptr<SrvLoop> L(new SrvLoop());
         L->plan = g;
         L->bytes = bytes;
-        HIPCHK(hipHostMalloc((void **)&L->hgrad, sizeof(float) * g->G.ngtot));
-        HIPCHK(hipHostMalloc((void **)&L->hx, sizeof(float) * g->G.nxtot));
-        HIPCHK(hipHostMalloc((void **)&L->hres, sizeof(double)));
-        std::memset(L->hgrad, 0, sizeof(float) * g->G.ngtot);
         S->used += bytes;
         S->loops.push_back(std::move(L));
         *loop = (int)S->loops.size() - 1;
@@ -2017,9 +2087,6 @@ int gpu_tomo_server_remove_loop(gpu_tomo_server_t *S, int loop) {
     // flights still holding a batch with this loop switch it off (its job is
     // never on again) and rebuild before their next launch
     S->gen++;
-    (void)hipHostFree(L.hgrad);
-    (void)hipHostFree(L.hx);
-    (void)hipHostFree(L.hres);
     gpu_tomo_free(L.plan);
     S->used -= L.bytes;
     L = SrvLoop();
@@ -2114,9 +2181,6 @@ void gpu_tomo_server_free(gpu_tomo_server_t *S) {
     for (auto &f : S->flights) flight_drop(f);
     for (auto &L : S->loops) {
         if (!L->plan) continue;
-        (void)hipHostFree(L->hgrad);
-        (void)hipHostFree(L->hx);
-        (void)hipHostFree(L->hres);
         gpu_tomo_free(L->plan);
     }
     for (auto &f : S->flights) {
