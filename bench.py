@@ -150,6 +150,79 @@ def traffic_summary(d, batch):
                                                  e["traffic_bytes"] / 1e6))
 
 
+def copy_bandwidth():
+    """Achievable HBM bandwidth: an elementwise copy of 1 GiB (read + write),
+    timed with HIP events, best of 5."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.neg(a, out=b)
+    best = 1e30
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        torch.neg(a, out=b)
+        e1.record()
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1) * 1e-3)
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * 4 * n / best / 1e9
+
+
+def kernel_table(plan, ab, nb, d_g, d_x, stream):
+    """Per-kernel-class mean launch time (HIP events on the launch stream,
+    read-only classes 8 launches per event pair) and achieved algorithmic
+    GB/s for one solve of nb systems."""
+    plan.profile(True, reps=8)
+    plan.solve_dev(d_g, d_x, nbatch=nb, mode=0, stream=stream)
+    prof = plan.profile_read()
+    plan.profile(False)
+    out = {}
+    for k, (ms, calls) in prof.items():
+        per, shared = ab.get(k, (0, 0))
+        by = per * nb + shared
+        out[k] = dict(avg_us=1e3 * ms, calls=calls, total_ms=ms * calls, alg_MB=by / 1e6,
+                      GBs=(by / (ms * 1e-3) / 1e9) if ms > 0 else 0.0)
+    return out
+
+
+def bench_sweep(r, grads, maxit, sizes, copy_gbs):
+    """Batch sweep of the large30 solve: solves/s, p99 and per-kernel achieved
+    algorithmic bandwidth at several batch sizes (autotuned per size)."""
+    import torch
+    from maos_amd import tomo
+    ab = algorithmic_bytes(r)
+    out = {}
+    for B in sizes:
+        g = tomo.GpuTomo(r, nbatch=B, maxit=maxit)
+        tuned = g.autotune(B)
+        G = torch.tensor(np.stack([grads[i % len(grads)] for i in range(B)]), dtype=torch.float32).cuda()
+        X = torch.zeros((B, r.nx), dtype=torch.float32, device="cuda:0")
+        st = torch.cuda.current_stream().cuda_stream
+        for _ in range(3):
+            g.solve_dev(G.data_ptr(), X.data_ptr(), nbatch=B, mode=1, stream=st)
+        torch.cuda.synchronize()
+        lat = []
+        for _ in range(10):
+            t = time.perf_counter()
+            g.solve_dev(G.data_ptr(), X.data_ptr(), nbatch=B, mode=1, stream=st)
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - t)
+        kt = kernel_table(g, ab, B, G.data_ptr(), X.data_ptr(), st)
+        torch.cuda.synchronize()
+        g.close()
+        del G, X
+        out[str(B)] = dict(solves_per_s=B * len(lat) / sum(lat), p99_ms=1e3 * float(np.percentile(lat, 99)),
+                           systems_per_workgroup=dict(wfs=tuned[0], layer=tuned[1], col=tuned[2]),
+                           kernels={k: dict(avg_us=round(v["avg_us"], 2), GBs=round(v["GBs"], 1),
+                                            frac_of_peak=round(v["GBs"] / HBM_PEAK_GBS, 4),
+                                            frac_of_copy=round(v["GBs"] / copy_gbs, 4))
+                                    for k, v in kt.items() if k in ab})
+    return out
+
+
 def cpu_baseline(r, grads, maxit, seconds):
     from oracle.oracle import Oracle
     try:
@@ -171,7 +244,17 @@ def cpu_baseline(r, grads, maxit, seconds):
         if time.perf_counter() - t0 > seconds and n >= 3:
             break
     el = time.perf_counter() - t0
-    return dict(value=n / el, unit="solves/s", cores=cores, kind="port",
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return dict(value=n / el, unit="solves/s", cores=cores, kind="port", cpu_model=model,
+                p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
                 sample="%d cold 30-iteration FDPCG solves of the 30 m system (RHS + PCG, FP64, "
                        "OpenMP over WFS/layers/blocks), %.1f s; p50 %.1f ms p99 %.1f ms"
                        % (n, el, 1e3 * np.percentile(lat, 50), 1e3 * np.percentile(lat, 99))), x0
@@ -386,7 +469,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", type=int, default=None,
                     help="solve precision mode (0, 1, 2; default: the library default)")
-    ap.add_argument("--modes", default="plain,graph,batched,uneven,served,precision",
+    ap.add_argument("--modes", default="plain,graph,batched,uneven,served,precision,sweep",
                     help="comma list of modes to time (profiling runs use one)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of 1 GiB (known HBM traffic) for counter scaling")
@@ -474,23 +557,23 @@ def main():
     # per-kernel profile of one batched solve (plain launches, HIP events on
     # the launch stream; the read-only kernel classes launched 8 times back
     # to back per event pair, so the average excludes the event overhead)
-    batched.profile(True, reps=8)
-    batched.solve_dev(d_g.data_ptr(), d_x.data_ptr(), nbatch=B, mode=0, stream=sp)
-    prof = batched.profile_read()
-    batched.profile(False)
-    torch.cuda.synchronize()
     ab = algorithmic_bytes(r)
-    kernels = {}
-    for k, (ms, calls) in prof.items():
-        per, shared = ab.get(k, (0, 0))
-        by = per * B + shared
-        kernels[k] = dict(avg_us=1e3 * ms, calls=calls, total_ms=ms * calls,
-                          alg_MB=by / 1e6, GBs=(by / (ms * 1e-3) / 1e9) if ms > 0 else 0.0)
+    kernels = kernel_table(batched, ab, B, d_g.data_ptr(), d_x.data_ptr(), sp)
+    torch.cuda.synchronize()
     dom = max((k for k in kernels if k in ab), key=lambda k: kernels[k]["total_ms"])
     kd = kernels[dom]
+    copy_gbs = copy_bandwidth()
     roofline = dict(bound="hbm", achieved=round(kd["GBs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(kd["GBs"] / HBM_PEAK_GBS, 4), traffic=measured_traffic(dom, B), kernel=dom,
-                    avg_us=round(kd["avg_us"], 2), alg_bytes_per_launch=int(ab[dom][0] * B + ab[dom][1]))
+                    avg_us=round(kd["avg_us"], 2), alg_bytes_per_launch=int(ab[dom][0] * B + ab[dom][1]),
+                    copy_GBs=round(copy_gbs, 1), frac_of_copy=round(kd["GBs"] / copy_gbs, 4))
+
+    sweep = None
+    if "sweep" in modes:
+        try:
+            sweep = bench_sweep(r, grads, maxit, (64, 128, 256), copy_gbs)
+        except Exception as e:  # reported, not fatal for the headline line
+            sweep = {"error": repr(e)}
 
     uneven = None
     if "uneven" in modes:
@@ -547,6 +630,7 @@ def main():
         "uneven64": uneven,
         "served16": served,
         "precision_study": precision,
+        "batch_sweep": sweep,
     }
     print(json.dumps(out))
 
