@@ -104,6 +104,22 @@ __device__ __forceinline__ Scal ldsc(const CONSTAS Scal &c) {
     return r;
 }
 
+// Buffer loads: a 32-bit byte offset against a base held in scalar
+// registers (no per-lane 64-bit address arithmetic), and an offset at or
+// past the buffer's size reads 0 (BOOB marks a ray that is not valid, so
+// its term vanishes without a per-lane weight select).
+constexpr unsigned BOOB = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t brsrc(const void *base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T> __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <> __device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+template <> __device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+
 // ---------------------------------------------------------------------------
 // Deterministic FP64 reductions (fixed order: per-lane strided sums, then a
 // 64-lane shuffle tree, then wavefront partials summed by lane 0 in order).
@@ -874,75 +890,47 @@ __global__ __launch_bounds__(256) void k_layer_m4(KARGS, int psel_, int osel, in
 #pragma unroll
         for (int r = 0; r < R; r++) acc[si][r] = 0;
     const int tw = min(TILE, nxl - jx0), th = min(LTY, nyl - jy0);
-    const long qstride = (long)G.nwfs * G.nploc;
+    // One body per WFS covers every case: each of the four rays of a node
+    // (two per axis) is loaded through a buffer offset, and a ray outside the
+    // interval gets the out-of-range offset, which reads 0; the wrapped edge
+    // ray (the last ray's second node is node 0) is a per-lane select of the
+    // first ray's index for node 0.  Per node the arithmetic is that of the
+    // reference's bilinear transpose:
+    //   acc += fy (gx q[iy0, ix1] + fx q[iy0, ix0]) + gy (gx q[iy1, ix1] + fx q[iy1, ix0]),
+    // gx = 1 - fx, gy = 1 - fy.  Five sample rows serve the four nodes.
     if (nr > 0) {
+        const int pnx = G.pnx;
+        const unsigned nrec = (unsigned)G.nploc * (unsigned)sizeof(RT);
         for (int w = 0; w < G.nwfs; w++) {
             const MRay m = mray(G, w * MAXL + l);
             const bool edgex = m.wx && jx0 == 0, edgey = m.wy && jy0 == 0;
             if (!edgex && (jx0 + tw - 1 - m.cx < m.ax0 || jx0 - m.cx - 1 >= m.ax1)) continue;
             if (!edgey && (jy0 + th - 1 - m.cy < m.ay0 || jy0 - m.cy - 1 >= m.ay1)) continue;
-            const RT *qw = q + ((long)s0 * G.nwfs + w) * G.nploc;
-            const RT fx = m.fx, fy = m.fy;
-            const int ix0 = jx - m.cx - 1, ix1 = jx - m.cx;
-            if (edgex || edgey) {
-                // tiles holding the wrapped edge ray: node by node
-#pragma unroll
-                for (int r = 0; r < R; r++) {
-                    if (r >= nr) continue;
-                    const int jy = jyb + r;
-                    int iy0 = jy - m.cy - 1, jx0r = ix0;
-                    if (m.wx && jx == 0) jx0r = m.ax1 - 1;
-                    if (m.wy && jy == 0) iy0 = m.ay1 - 1;
-                    const int iy1 = jy - m.cy;
-                    const RT wx0 = (jx0r >= m.ax0 && jx0r < m.ax1) ? fx : RT(0);
-                    const RT wx1 = (ix1 >= m.ax0 && ix1 < m.ax1) ? RT(1) - fx : RT(0);
-                    const RT wy0 = (iy0 >= m.ay0 && iy0 < m.ay1) ? fy : RT(0);
-                    const RT wy1 = (iy1 >= m.ay0 && iy1 < m.ay1) ? RT(1) - fy : RT(0);
-                    const int cx0 = min(max(jx0r, 0), G.pnx - 1), cx1 = min(max(ix1, 0), G.pnx - 1);
-                    const int cy0 = min(max(iy0, 0), G.pny - 1), cy1 = min(max(iy1, 0), G.pny - 1);
-#pragma unroll
-                    for (int si = 0; si < CS; si++) {
-                        if (si < ns) {
-                            const RT *t = qw + si * qstride;
-                            acc[si][r] += wy0 * (wx1 * t[cy0 * G.pnx + cx1] + wx0 * t[cy0 * G.pnx + cx0]) +
-                                          wy1 * (wx1 * t[cy1 * G.pnx + cx1] + wx0 * t[cy1 * G.pnx + cx0]);
-                        }
-                    }
-                }
-                continue;
-            }
-            const bool fast = jx0 - m.cx - 1 >= m.ax0 && jx0 + tw - m.cx <= m.ax1 && jy0 - m.cy - 1 >= m.ay0 &&
-                              jy0 + th - m.cy <= m.ay1;
-            // sample rows k = 0..nr of this thread: iy = jyb - cy - 1 + k
-            RT wxa = fx, wxb = RT(1) - fx;
-            int ca = ix0, cb = ix1;
-            if (!fast) {
-                wxa = (ix0 >= m.ax0 && ix0 < m.ax1) ? fx : RT(0);
-                wxb = (ix1 >= m.ax0 && ix1 < m.ax1) ? RT(1) - fx : RT(0);
-                ca = min(max(ix0, 0), G.pnx - 1);
-                cb = min(max(ix1, 0), G.pnx - 1);
-            }
-            // Rows are clamped into the grid (a no-op on the plain path for
-            // rows up to nr; rows past it only feed nodes that are not
-            // stored) and systems past ns read system ns-1, so the loads
-            // are unconditional.
-            int rowo[R + 1];
-            RT my[R + 1];
+            const RT fx = m.fx, fy = m.fy, gx = RT(1) - fx, gy = RT(1) - fy;
+            // columns: node jx takes ray ix0 = jx - cx - 1 (weight fx; the last
+            // ray for node 0 on a wrapped axis) and ix1 = jx - cx (weight gx)
+            const int ix0 = (m.wx && jx == 0) ? m.ax1 - 1 : jx - m.cx - 1, ix1 = jx - m.cx;
+            const bool v0 = ix0 >= m.ax0 && ix0 < m.ax1, v1 = ix1 >= m.ax0 && ix1 < m.ax1;
+            // sample rows k = 0..R: iy = jyb - cy - 1 + k; row 0 serves node
+            // jyb only (its first ray, the last ray when jyb = 0 wraps)
+            unsigned o0[R + 1], o1[R + 1];
 #pragma unroll
             for (int k = 0; k <= R; k++) {
-                const int iy = jyb - m.cy - 1 + k;
-                my[k] = (fast || (iy >= m.ay0 && iy < m.ay1)) ? RT(1) : RT(0);
-                rowo[k] = min(max(iy, 0), G.pny - 1) * G.pnx;
+                const int iy = (k == 0 && m.wy && jyb == 0) ? m.ay1 - 1 : jyb - m.cy - 1 + k;
+                const bool vy = iy >= m.ay0 && iy < m.ay1;
+                const unsigned rb = (unsigned)(iy * pnx);
+                o0[k] = vy && v0 ? (rb + (unsigned)ix0) * (unsigned)sizeof(RT) : BOOB;
+                o1[k] = vy && v1 ? (rb + (unsigned)ix1) * (unsigned)sizeof(RT) : BOOB;
             }
+            // systems past ns read system ns-1 (in range; never stored)
 #pragma unroll
             for (int si = 0; si < CS; si++) {
-                const RT *t = qw + min(si, ns - 1) * qstride;
+                const auto rs = brsrc(q + ((long)(s0 + min(si, ns - 1)) * G.nwfs + w) * G.nploc, nrec);
                 RT h[R + 1];
 #pragma unroll
-                for (int k = 0; k <= R; k++) h[k] = wxb * t[rowo[k] + cb] + wxa * t[rowo[k] + ca];
+                for (int k = 0; k <= R; k++) h[k] = gx * bload<RT>(rs, o1[k]) + fx * bload<RT>(rs, o0[k]);
 #pragma unroll
-                for (int r = 0; r < R; r++)
-                    acc[si][r] += (my[r] * fy) * h[r] + (my[r + 1] * (RT(1) - fy)) * h[r + 1];
+                for (int r = 0; r < R; r++) acc[si][r] += fy * h[r] + gy * h[r + 1];
             }
         }
     }
