@@ -316,18 +316,23 @@ int  gpu_tomo_server_usage(gpu_tomo_server_t *s, double *used_gb,
                            double *budget_gb);
 void gpu_tomo_server_free(gpu_tomo_server_t *s);
 
-/* Autotuner: times the solve of nsys systems under each choice of systems
- * per workgroup (1, 2, 4) of the WFS, layer and column kernels and keeps the
- * fastest for this plan (choice[3] receives wfs, layer, col).  Overwrites
- * the gradients and solutions of slots [0, nsys).  No reference
- * counterpart. */
+/* Autotuner: times the solve of nsys systems under each layer tile shape
+ * (16 x 64 or 16 x 16 nodes, where both fit) and each choice of systems per
+ * workgroup (1, 2, 4) of the WFS, layer and column kernels, and keeps the
+ * fastest for this plan.  choice[4] receives wfs, layer, col and the layer
+ * tile rows (16 or 64).  Overwrites the gradients and solutions of slots
+ * [0, nsys).  Call it before the plan joins a group.  The tile shape sets
+ * the order of the p.Ap partial sums, so it can change results in the last
+ * bits; systems per workgroup never do.  No reference counterpart. */
 int  gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice);
 
-/* Set the systems per workgroup of the WFS, layer and column kernels
- * directly (choice[3]: each 1, 2 or 4, or 0 for the built-in heuristic),
- * e.g. to replay an autotuned choice without timing.  Values above a
- * kernel's maximum (WFS 2 when tomo.pos = 2, column 2 in precision modes 1
- * and 2) are clamped when the launch is built.  No reference counterpart. */
+/* Set the launch shape directly, e.g. to replay an autotuned choice without
+ * timing: choice[4] = systems per workgroup of the WFS, layer and column
+ * kernels (each 1, 2 or 4, or 0 for the built-in heuristic) and the layer
+ * tile rows (16, 64, or 0 for 64 where it fits).  Values above a kernel's
+ * maximum (WFS 2 when tomo.pos = 2, column 2 in precision modes 1 and 2) are
+ * clamped when the launch is built.  Call it before the plan joins a group.
+ * No reference counterpart. */
 int  gpu_tomo_tune_set(gpu_tomo_t *g, const int *choice);
 
 /* Plan facts: matched-sampling ray path (1) or generic ray tables (0), FFT

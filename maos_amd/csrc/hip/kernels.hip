@@ -10,6 +10,8 @@
 //                                                          k_layer/k_irow/k_row,
 //                                                          fixed-order reduction
 #include "kernels.hpp"
+#include <type_traits>
+
 #include "launch.hpp"
 
 namespace mtk {
@@ -1598,19 +1600,27 @@ static void launch_col_cs(const LaunchTab &t, int bs, size_t lds, hipStream_t st
         default: hipLaunchKernelGGL((k_col<FT, CS, 0, RM>), grid, blk, lds, st, KT); break;
     }
 }
-template <typename FT, int CSMAX>
-static void launch_col_t(const LaunchTab &t, int cs, int bs, int rm, size_t lds, hipStream_t st) {
-    if (rm > 4) {
-        if (cs > 1) launch_col_cs<FT, CSMAX, 7>(t, bs, lds, st);
-        else launch_col_cs<FT, 1, 7>(t, bs, lds, st);
-    } else {
-        if (cs > 1) launch_col_cs<FT, CSMAX, 3>(t, bs, lds, st);
-        else launch_col_cs<FT, 1, 3>(t, bs, lds, st);
+// The instantiation must be the systems per workgroup the grid and the LDS
+// size were built for (1, 2, or 4 with an FP32 spectrum).
+template <typename FT, int RM>
+static void launch_col_r(const LaunchTab &t, int cs, int bs, size_t lds, hipStream_t st) {
+    if constexpr (std::is_same<FT, float2>::value) {
+        if (cs == 4) {
+            launch_col_cs<FT, 4, RM>(t, bs, lds, st);
+            return;
+        }
     }
+    if (cs == 2) launch_col_cs<FT, 2, RM>(t, bs, lds, st);
+    else launch_col_cs<FT, 1, RM>(t, bs, lds, st);
 }
 void launch_col(const LaunchTab &t, int spec64, int cs, int bs, int rm, size_t lds, hipStream_t st) {
-    if (spec64) launch_col_t<double2, 2>(t, cs, bs, rm, lds, st);
-    else launch_col_t<float2, 4>(t, cs, bs, rm, lds, st);
+    if (spec64) {
+        if (rm > 4) launch_col_r<double2, 7>(t, cs, bs, lds, st);
+        else launch_col_r<double2, 3>(t, cs, bs, lds, st);
+    } else {
+        if (rm > 4) launch_col_r<float2, 7>(t, cs, bs, lds, st);
+        else launch_col_r<float2, 3>(t, cs, bs, lds, st);
+    }
 }
 template <int RM>
 static void launch_irow_r(const LaunchTab &t, int mode, int slot, int osel, size_t lds, hipStream_t st) {

@@ -165,6 +165,8 @@ struct gpu_tomo {
     std::map<std::pair<int, int>, std::unique_ptr<Batch>> own;
     // systems per workgroup chosen by the autotuner (0: heuristic)
     int tune_wfs = 0, tune_layer = 0, tune_col = 0;
+    int tune_lty = 0;     // layer tile rows: 0 = heuristic (LTY where it fits), TILE or LTY
+    bool lty_ok = false;  // LTY-row tiles fit every layer of this (matched) plan
     // profiling
     int prof = 0;
     std::vector<ProfEv> evs;  // profiling events
@@ -541,6 +543,7 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
         for (int l = 0; l < R->npsr; l++)
             if (R->xny[l] < LTY + 2) G.lty = TILE;
     }
+    g->lty_ok = G.lty == LTY;
     G.matched = matched;
     g->mray.upload(hm);
     G.mray = g->mray.p;
@@ -811,6 +814,20 @@ int layer_tiles(const Geom &G) {
     return t;
 }
 
+void upload_plandev(gpu_tomo *g);
+void drop_batches(gpu_tomo *g);
+// Apply the layer tile height choice (the plan's device record and
+// partial counts follow; launch tables are rebuilt on next use).
+void apply_lty(gpu_tomo *g) {
+    if (!g->G.matched) return;
+    const int lty = g->tune_lty == TILE ? TILE : g->lty_ok ? LTY : TILE;
+    if (lty == g->G.lty) return;
+    g->G.lty = lty;
+    g->SC.npart_layer = layer_tiles(g->G) * g->G.npsr;
+    upload_plandev(g);
+    drop_batches(g);
+}
+
 void upload_plandev(gpu_tomo *g) {
     PlanDev h{};
     h.G = g->G;
@@ -846,7 +863,9 @@ void alloc_state(gpu_tomo *g) {
     g->tt.alloc(ns * G.nwfs * 2);
     g->specf.alloc(ns * std::max<long>(G.nspec, 1));
     g->speci.alloc(ns * std::max<long>(G.nspec, 1));
-    const int maxpart = std::max({layer_tiles(G) * G.npsr, G.nrowblk, G.ngroup, 1});
+    Geom g16 = G;  // partials sized for the smaller layer tiles (either height may be chosen later)
+    g16.lty = TILE;
+    const int maxpart = std::max({layer_tiles(g16) * G.npsr, G.nrowblk, G.ngroup, 1});
     g->part_pap.alloc(ns * maxpart);
     g->part_rz.alloc(ns * maxpart);
     g->part_rr.alloc(ns * maxpart);
@@ -1633,13 +1652,16 @@ int gpu_tomo_tune_set(gpu_tomo_t *g, const int *choice) {
     if (!g || !choice) return MT_EINVAL;
     for (int i = 0; i < 3; i++)
         if (choice[i] != 0 && choice[i] != 1 && choice[i] != 2 && choice[i] != 4) return MT_EINVAL;
+    if (choice[3] != 0 && choice[3] != TILE && choice[3] != LTY) return MT_EINVAL;
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
         HIPCHK(hipDeviceSynchronize());
         g->tune_wfs = choice[0];
         g->tune_layer = choice[1];
         g->tune_col = choice[2];
+        g->tune_lty = choice[3];
         drop_batches(g);
+        apply_lty(g);
     });
 }
 
@@ -1674,6 +1696,22 @@ int gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice) {
             int *field, kc, cmax;
         };
         const int colmax = g->spec64 ? 2 : 4;
+        // layer tile shape first (16 x 64 or 16 x 16 tiles, where both fit)
+        if (g->G.matched && g->lty_ok) {
+            double best = 1e30;
+            int bestl = LTY;
+            for (int lty : {LTY, TILE}) {
+                g->tune_lty = lty;
+                apply_lty(g);
+                const double t = stage_ms(KC_LAYER);
+                if (t < best * 0.98) {
+                    best = t;
+                    bestl = lty;
+                }
+            }
+            g->tune_lty = bestl;
+            apply_lty(g);
+        }
         Knob knobs[3] = {{&g->tune_wfs, KC_WFS, g->G.pos == 1 ? 4 : 2}, {&g->tune_layer, KC_LAYER, 4},
                          {&g->tune_col, KC_COL, colmax}};
         for (auto &k : knobs) {
@@ -1695,6 +1733,7 @@ int gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice) {
             choice[0] = g->tune_wfs;
             choice[1] = g->tune_layer;
             choice[2] = g->tune_col;
+            choice[3] = g->G.lty;
         }
     });
 }

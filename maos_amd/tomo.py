@@ -166,15 +166,17 @@ class GpuTomo:
         return dict(zip(("matched", "fnx", "fny", "bs", "rmax"), [a.value for a in v]))
 
     def autotune(self, nsys=None):
-        """gpu_tomo_autotune: pick systems per workgroup of the WFS, layer
-        and column kernels by timing; returns (wfs, layer, col)."""
-        ch = (C.c_int * 3)()
+        """gpu_tomo_autotune: pick the layer tile shape and the systems per
+        workgroup of the WFS, layer and column kernels by timing; returns
+        (wfs, layer, col, layer tile rows)."""
+        ch = (C.c_int * 4)()
         _abi.check(_abi.lib().gpu_tomo_autotune(self._h, int(nsys or self.nbatch), ch), "gpu_tomo_autotune")
         return tuple(ch)
 
-    def tune_set(self, wfs, layer, col):
-        """gpu_tomo_tune_set: fix the systems per workgroup (0: heuristic)."""
-        ch = (C.c_int * 3)(int(wfs), int(layer), int(col))
+    def tune_set(self, wfs, layer, col, layer_rows=0):
+        """gpu_tomo_tune_set: fix the systems per workgroup (0: heuristic)
+        and the layer tile rows (16, 64; 0: 64 where it fits)."""
+        ch = (C.c_int * 4)(int(wfs), int(layer), int(col), int(layer_rows))
         _abi.check(_abi.lib().gpu_tomo_tune_set(self._h, ch), "gpu_tomo_tune_set")
 
     def close(self):

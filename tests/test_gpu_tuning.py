@@ -1,0 +1,110 @@
+"""GPU: launch-shape choices do not change results.
+
+Systems per workgroup (gpu_tomo_tune_set / gpu_tomo_autotune), the
+workgroup order and the profiler's repeated launches change how the work is
+laid out on the GPU, never what a system computes: every choice gives the
+same bits.  All calls go through the C ABI (maos_amd.tomo)."""
+import numpy as np
+import pytest
+
+from maos_amd import _abi, tomo, workload
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(r, cfg, n):
+    return np.stack([workload.make_inputs(cfg, r, 40 + s) for s in range(n)])
+
+
+@pytest.mark.parametrize("mode", [0, 1, 3])
+def test_systems_per_workgroup_bit_exact(large30, mode):
+    """Five systems (one partial workgroup for every choice above 1) solved
+    under each systems-per-workgroup choice and by the heuristic: identical
+    solutions and residuals."""
+    cfg, r = large30
+    grads = _batch(r, cfg, 5)
+    g = tomo.GpuTomo(r, nbatch=5, maxit=cfg["maxit"])
+    g.precision(mode)
+    ref = g.tomo_batch(grads)
+    ref_res = g.residuals(5)
+    # (column 2 with an FP32 spectrum once launched the 4-system kernel on a
+    # grid and LDS sized for 2)
+    for choice in [(1, 1, 1), (2, 2, 2), (4, 4, 2), (2, 1, 2), (1, 1, 2), (1, 1, 4)]:
+        g.tune_set(*choice)
+        x = g.tomo_batch(grads)
+        print("choice", choice, "max |dx|", float(np.abs(x - ref).max()), flush=True)
+        assert np.array_equal(x, ref), "systems per workgroup %s changed the solution" % (choice,)
+        assert np.array_equal(g.residuals(5), ref_res)
+    g.close()
+
+
+def test_tune_set_rejects_bad_choice(large30):
+    cfg, r = large30
+    g = tomo.GpuTomo(r, nbatch=1, maxit=5)
+    with pytest.raises(_abi.MtError):
+        g.tune_set(3, 1, 1)  # 3 is not a choice
+    with pytest.raises(_abi.MtError):
+        g.tune_set(1, 1, 1, 32)  # nor are 32-row tiles
+    g.close()
+
+
+def test_autotune_replay(large30):
+    """The autotuner overwrites slots and picks a choice; replaying that
+    choice with tune_set gives the same bits, and the result stays within
+    the rounding of the choice of layer tile shape (the order of the p.Ap
+    partial sums) of the solution before tuning."""
+    cfg, r = large30
+    grads = _batch(r, cfg, 3)
+    g = tomo.GpuTomo(r, nbatch=3, maxit=cfg["maxit"])
+    ref = g.tomo_batch(grads)
+    choice = g.autotune(3)
+    print("autotuned", choice, flush=True)
+    assert all(c in (1, 2, 4) for c in choice[:3]) and choice[3] in (16, 64)
+    x = g.tomo_batch(grads)
+    assert np.linalg.norm(x - ref) <= 1e-5 * np.linalg.norm(ref)
+    g2 = tomo.GpuTomo(r, nbatch=3, maxit=cfg["maxit"])
+    g2.tune_set(*choice)
+    assert np.array_equal(g2.tomo_batch(grads), x)
+    g.close()
+    g2.close()
+
+
+@pytest.mark.parametrize("rows", [16, 64])
+def test_layer_tile_shape_parity(large30, rows):
+    """Both layer tile shapes against the FP64 oracle."""
+    from oracle.oracle import Oracle
+    cfg, r = large30
+    grad = workload.make_inputs(cfg, r, 1)
+    g = tomo.GpuTomo(r, maxit=cfg["maxit"])
+    g.tune_set(0, 0, 0, rows)
+    xg, resg, rc = g.tomo(grad)
+    xo, reso = Oracle(r, nthread=8).solve(grad, cfg["maxit"])
+    e = float(np.linalg.norm(xg - xo) / np.linalg.norm(xo))
+    print("layer tile rows %d: rel L2 %.3e" % (rows, e), flush=True)
+    assert e <= 1e-4
+    g.close()
+
+
+def test_profile_repeats_keep_results(large30):
+    """Profiling with the read-only kernel classes launched 8 times per event
+    pair leaves the solution unchanged and counts every launch.  (The plan's
+    own device buffers: gradients by set_grad, solutions by get_x.)"""
+    cfg, r = large30
+    grads = _batch(r, cfg, 2)
+    g = tomo.GpuTomo(r, nbatch=2, maxit=cfg["maxit"])
+    for i in range(2):
+        g.set_grad(i, grads[i])
+    g.solve_dev(0, 0, nbatch=2, mode=0)
+    ref = [g.get_x(i)[0] for i in range(2)]
+    g.profile(True, reps=8)
+    g.solve_dev(0, 0, nbatch=2, mode=0)
+    prof = g.profile_read()
+    g.profile(False)
+    for i in range(2):
+        assert np.array_equal(g.get_x(i)[0], ref[i])
+    it = cfg["maxit"]
+    # read-only classes: 8 launches per event pair; k_row once per update
+    assert prof["k_col"][1] == 8 * it
+    assert prof["k_layer"][1] == 8 * it
+    assert prof["k_row"][1] == it - 1
+    g.close()
