@@ -1685,10 +1685,12 @@ int gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice) {
             Batch &B = own_batch(g, 0, nsys);
             double best = 1e30;
             for (int rep = 0; rep < 3; rep++) {
-                Rec rec{&g->evs, g->stream, true};
+                // the timed classes only read their inputs: 4 launches per
+                // event pair keep the event overhead out of the comparison
+                Rec rec{&g->evs, g->stream, true, 4};
                 enqueue_solve(B, rec);
                 prof_collect(g);
-                best = std::min(best, g->prof_ms[kc]);
+                best = std::min(best, g->prof_ms[kc] / std::max(g->prof_cnt[kc], 1));
             }
             return best;
         };
@@ -1696,24 +1698,30 @@ int gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice) {
             int *field, kc, cmax;
         };
         const int colmax = g->spec64 ? 2 : 4;
-        // layer tile shape first (16 x 64 or 16 x 16 tiles, where both fit)
-        if (g->G.matched && g->lty_ok) {
+        // the layer stage: tile shape (16 x 64 or 16 x 16, where both fit)
+        // and systems per workgroup together
+        {
             double best = 1e30;
-            int bestl = LTY;
+            int bestl = g->G.lty, bestc = 1;
             for (int lty : {LTY, TILE}) {
+                if (lty == LTY && !(g->G.matched && g->lty_ok)) continue;
                 g->tune_lty = lty;
                 apply_lty(g);
-                const double t = stage_ms(KC_LAYER);
-                if (t < best * 0.98) {
-                    best = t;
-                    bestl = lty;
+                for (int c = 1; c <= 4; c <<= 1) {
+                    g->tune_layer = c;
+                    const double t = stage_ms(KC_LAYER);
+                    if (t < best * 0.98) {  // prefer the earlier (larger tile, fewer systems) within 2%
+                        best = t;
+                        bestl = lty;
+                        bestc = c;
+                    }
                 }
             }
             g->tune_lty = bestl;
+            g->tune_layer = bestc;
             apply_lty(g);
         }
-        Knob knobs[3] = {{&g->tune_wfs, KC_WFS, g->G.pos == 1 ? 4 : 2}, {&g->tune_layer, KC_LAYER, 4},
-                         {&g->tune_col, KC_COL, colmax}};
+        Knob knobs[2] = {{&g->tune_wfs, KC_WFS, g->G.pos == 1 ? 4 : 2}, {&g->tune_col, KC_COL, colmax}};
         for (auto &k : knobs) {
             int bestc = 1;
             double bestt = 1e30;
