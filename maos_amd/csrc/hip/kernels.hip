@@ -1601,17 +1601,40 @@ static void launch_col_cs(const LaunchTab &t, int bs, size_t lds, hipStream_t st
     }
 }
 // The instantiation must be the systems per workgroup the grid and the LDS
-// size were built for (1, 2, or 4 with an FP32 spectrum).
+// size were built for (1, 2 or 4).
 template <typename FT, int RM>
 static void launch_col_r(const LaunchTab &t, int cs, int bs, size_t lds, hipStream_t st) {
-    if constexpr (std::is_same<FT, float2>::value) {
-        if (cs == 4) {
-            launch_col_cs<FT, 4, RM>(t, bs, lds, st);
-            return;
-        }
-    }
-    if (cs == 2) launch_col_cs<FT, 2, RM>(t, bs, lds, st);
+    if (cs == 4) launch_col_cs<FT, 4, RM>(t, bs, lds, st);
+    else if (cs == 2) launch_col_cs<FT, 2, RM>(t, bs, lds, st);
     else launch_col_cs<FT, 1, RM>(t, bs, lds, st);
+}
+template <typename FT, int CS, int RM>
+static void col_allow_bs(int bs, size_t lds) {
+    const void *f;
+    switch (bs) {
+        case 2: f = (const void *)k_col<FT, CS, 2, RM>; break;
+        case 3: f = (const void *)k_col<FT, CS, 3, RM>; break;
+        case 4: f = (const void *)k_col<FT, CS, 4, RM>; break;
+        case 6: f = (const void *)k_col<FT, CS, 6, RM>; break;
+        case 8: f = (const void *)k_col<FT, CS, 8, RM>; break;
+        default: f = (const void *)k_col<FT, CS, 0, RM>; break;
+    }
+    (void)hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+template <typename FT, int RM>
+static void col_allow_r(int cs, int bs, size_t lds) {
+    if (cs == 4) col_allow_bs<FT, 4, RM>(bs, lds);
+    else if (cs == 2) col_allow_bs<FT, 2, RM>(bs, lds);
+    else col_allow_bs<FT, 1, RM>(bs, lds);
+}
+void col_allow_lds(int spec64, int cs, int bs, int rm, size_t lds) {
+    if (spec64) {
+        if (rm > 4) col_allow_r<double2, 7>(cs, bs, lds);
+        else col_allow_r<double2, 3>(cs, bs, lds);
+    } else {
+        if (rm > 4) col_allow_r<float2, 7>(cs, bs, lds);
+        else col_allow_r<float2, 3>(cs, bs, lds);
+    }
 }
 void launch_col(const LaunchTab &t, int spec64, int cs, int bs, int rm, size_t lds, hipStream_t st) {
     if (spec64) {

@@ -27,6 +27,9 @@ void launch_row(const LaunchTab &t, int mode, int rt64, int spec64, int rm, int 
                 hipStream_t st);
 size_t col_lds(int maxcol, int npsr, int fny, int f64, int cs);
 void launch_col(const LaunchTab &t, int spec64, int cs, int bs, int rm, size_t lds, hipStream_t st);
+// Raise the dynamic-LDS limit of the column kernel instantiation that a
+// launch of more than 64 KB will use (before any launch or capture).
+void col_allow_lds(int spec64, int cs, int bs, int rm, size_t lds);
 void launch_irow(const LaunchTab &t, int mode, int rm, int slot, int osel, size_t lds, hipStream_t st);
 void launch_final(const LaunchTab &t, hipStream_t st);
 void launch_zero_stop(const LaunchTab &t, hipStream_t st);

@@ -149,6 +149,7 @@ struct gpu_tomo {
     DevBuf<double> r, Ap, b, q;  // residual side: double, or float in the first half when !rt64
     int rt64 = 1;             // 1: r, Ap, b in FP64 (default); 0: FP32
     int spec64 = 1;           // 1: forward FDPCG spectrum in FP64; 0: FP32
+    size_t lds_max = 64 * 1024;  // LDS one workgroup may use on this device
     DevBuf<double2> specf;  // forward spectrum of r (double2, or float2 in the first half when !rt64)
     DevBuf<float2> speci;   // block-multiplied spectrum, inverse transform input
     DevBuf<double> part_pap, part_rz, part_rr, part_bb, rz, alpha, res;
@@ -1132,14 +1133,23 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
                     [&](Sub &sb, const std::vector<int> &k, const std::vector<int> &js) {
                         sb.bs = k[0];
                         sb.rm = k[1];
-                        const int cmax = b.spec64 ? 2 : 4;
+                        // up to 4 systems per workgroup as far as the device's LDS
+                        // per workgroup allows (the heuristic stays at 2 with an
+                        // FP64 spectrum; the autotuner may choose 4)
+                        const size_t lmax = b.plan[js[0]]->lds_max;
                         const int tuned = b.plan[js[0]]->tune_col;
-                        sb.cs = tuned ? std::min(tuned, cmax)
-                                      : pick_cs(jw_of(js, [&](int j) { return (long)G(j).ngroup; }), cmax, 384);
-                        for (int j : js)
-                            if (col_lds(G(j).maxcol, G(j).npsr, G(j).fny, b.spec64, sb.cs) > 64 * 1024) sb.cs = 1;
+                        sb.cs = tuned ? std::min(tuned, 4)
+                                      : pick_cs(jw_of(js, [&](int j) { return (long)G(j).ngroup; }),
+                                                b.spec64 ? 2 : 4, 384);
+                        auto fits = [&](int c) {
+                            for (int j : js)
+                                if (col_lds(G(j).maxcol, G(j).npsr, G(j).fny, b.spec64, c) > lmax) return false;
+                            return true;
+                        };
+                        while (sb.cs > 1 && !fits(sb.cs)) sb.cs >>= 1;
                         for (int j : js)
                             sb.lds = std::max(sb.lds, col_lds(G(j).maxcol, G(j).npsr, G(j).fny, b.spec64, sb.cs));
+                        if (sb.lds > 64 * 1024) col_allow_lds(b.spec64, sb.cs, sb.bs, sb.rm, sb.lds);
                     },
                     true);
     }
@@ -1355,6 +1365,12 @@ int gpu_tomo_setup(const mt_recon_t *recon, int device, int nbatch, int maxit, i
         if (device < 0 || device >= ndev) fail(MT_EINVAL, "gpu_tomo_setup: no such device");
         g->device = device;
         HIPCHK(hipSetDevice(device));
+        {
+            int v = 0;
+            if (hipDeviceGetAttribute(&v, hipDeviceAttributeSharedMemPerBlockOptin, device) == hipSuccess &&
+                v > 64 * 1024)
+                g->lds_max = (size_t)v;
+        }
         g->nbatch = nbatch;
         g->maxit = maxit;
         g->warm = warm ? 1 : 0;
@@ -1697,7 +1713,7 @@ int gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice) {
         struct Knob {
             int *field, kc, cmax;
         };
-        const int colmax = g->spec64 ? 2 : 4;
+        const int colmax = 4;  // clamped by the LDS per workgroup when the launch is built
         // the layer stage: tile shape (16 x 64 or 16 x 16, where both fit)
         // and systems per workgroup together
         {
