@@ -317,22 +317,28 @@ int  gpu_tomo_server_usage(gpu_tomo_server_t *s, double *used_gb,
 void gpu_tomo_server_free(gpu_tomo_server_t *s);
 
 /* Autotuner: times the solve of nsys systems under each layer tile shape
- * (16 x 64 or 16 x 16 nodes, where both fit) and each choice of systems per
- * workgroup (1, 2, 4) of the WFS, layer and column kernels, and keeps the
- * fastest for this plan.  choice[4] receives wfs, layer, col and the layer
- * tile rows (16 or 64).  Overwrites the gradients and solutions of slots
- * [0, nsys).  Call it before the plan joins a group.  The tile shape sets
- * the order of the p.Ap partial sums, so it can change results in the last
- * bits; systems per workgroup never do.  No reference counterpart. */
+ * (16 x 64 or 16 x 16 nodes, where both fit), each column kernel (in
+ * registers, where every FDPCG block couples one frequency across the
+ * layers, at most 6, and the column length is 2^a x P with P in 2, 3, 4, 5,
+ * 7; or in LDS stages, always available) and each choice of systems per
+ * workgroup (1, 2, 4) of the WFS, layer and LDS column kernels, and keeps
+ * the fastest for this plan.  choice[5] receives wfs, layer, col (0 with
+ * the register column kernel), the layer tile rows (16 or 64) and the
+ * column kernel (1 registers, 2 LDS stages).  Overwrites the gradients and
+ * solutions of slots [0, nsys).  Call it before the plan joins a group.
+ * The tile shape and the column kernel set the order of sums, so they can
+ * change results in the last bits; systems per workgroup never do.  No
+ * reference counterpart. */
 int  gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice);
 
 /* Set the launch shape directly, e.g. to replay an autotuned choice without
- * timing: choice[4] = systems per workgroup of the WFS, layer and column
- * kernels (each 1, 2 or 4, or 0 for the built-in heuristic) and the layer
- * tile rows (16, 64, or 0 for 64 where it fits).  Values above a kernel's
- * maximum (WFS 2 when tomo.pos = 2, column 2 in precision modes 1 and 2) are
- * clamped when the launch is built.  Call it before the plan joins a group.
- * No reference counterpart. */
+ * timing: choice[5] = systems per workgroup of the WFS, layer and LDS
+ * column kernels (each 1, 2 or 4, or 0 for the built-in heuristic), the
+ * layer tile rows (16, 64, or 0 for 64 where it fits) and the column kernel
+ * (1 registers where it applies, 2 LDS stages, 0 as 1).  Values above a
+ * kernel's maximum (WFS 2 when tomo.pos = 2, column 2 in precision modes 1
+ * and 2) are clamped when the launch is built.  Call it before the plan
+ * joins a group.  No reference counterpart. */
 int  gpu_tomo_tune_set(gpu_tomo_t *g, const int *choice);
 
 /* Plan facts: matched-sampling ray path (1) or generic ray tables (0), FFT

@@ -290,6 +290,57 @@ __device__ C *fft_run(C *a, C *b, int n, int nf, int nst, const CONSTAS FStage *
     return in;
 }
 
+// ---------------------------------------------------------------------------
+// FFT of a column in registers: n = L * P points on L consecutive lanes (L a
+// power of two dividing 64), lane l holding x[l + L p] in register p.
+// Four-step form, with W_N = exp(-2 pi i / N):
+//   X[k1 + P k2] = sum_l W_L^(l k2) W_n^(l k1) sum_p x[l + L p] W_P^(p k1)
+// the forward transform: a radix-P DFT in each lane, the twiddles
+// W_n^(l k1), then log2(L) radix-2 stages across the lanes by xor shuffles
+// (decimation in frequency: no LDS, no barrier); on return lane l, register
+// k1 holds X[k1 + P * bitrev_L(l)].  The inverse takes that layout back: the
+// same stages in reverse order with conjugate twiddles (decimation in time),
+// the conjugate twiddles, then the radix-P inverse DFT, leaving n x the
+// inverse transform in natural order (lane l, register p: x[l + L p]).
+// tw: the n twiddles exp(-2 pi i k / n) (the twiddle of a stage is the same
+// on both lanes of a butterfly).
+// ---------------------------------------------------------------------------
+template <typename C> __device__ __forceinline__ C shfl_xor_c(C v, int m) {
+    return CT<C>::make(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
+}
+template <typename C> __device__ __forceinline__ C conj_c(C v) { return CT<C>::make(v.x, -v.y); }
+template <int P, typename C>
+__device__ __forceinline__ void fft_regs_fwd(C *y, int L, int l, const C *tw) {
+    if constexpr (P > 1) small_dft<P, -1, C>(y);
+#pragma unroll
+    for (int k1 = 1; k1 < P; k1++) y[k1] = cmul(y[k1], tw[l * k1]);
+    for (int h = L >> 1; h >= 1; h >>= 1) {
+        const bool up = (l & h) != 0;
+        const C w = tw[(l & (h - 1)) * (L / (2 * h)) * P];
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const C o = shfl_xor_c(y[p], h);
+            y[p] = up ? cmul(csub(o, y[p]), w) : cadd(y[p], o);
+        }
+    }
+}
+template <int P, typename C>
+__device__ __forceinline__ void fft_regs_inv(C *y, int L, int l, const C *tw) {
+    for (int h = 1; h < L; h <<= 1) {
+        const bool up = (l & h) != 0;
+        const C w = conj_c(tw[(l & (h - 1)) * (L / (2 * h)) * P]);
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const C t = up ? cmul(y[p], w) : y[p];
+            const C o = shfl_xor_c(t, h);
+            y[p] = up ? csub(o, t) : cadd(t, o);
+        }
+    }
+#pragma unroll
+    for (int k1 = 1; k1 < P; k1++) y[k1] = cmul(y[k1], conj_c(tw[l * k1]));
+    if constexpr (P > 1) small_dft<P, 1, C>(y);
+}
+
 // Copy of the n twiddles into LDS (before the caller's first barrier).
 template <typename C>
 __device__ __forceinline__ void load_tw(C *dst, const C *src, int n) {
@@ -1324,6 +1375,121 @@ __global__ __launch_bounds__(256) void k_col(KARGS) {
 }
 
 // ---------------------------------------------------------------------------
+// Column kernel with the column FFTs in registers, for FDPCG blocks that
+// each couple one frequency (kx, ky) of the half spectrum across the
+// bs = npsr layers (no aliasing partners: every column is a group of its
+// own).  A task is one (system, column) on bs x L threads: L lanes per layer
+// hold its ny = L * P points (lane l: ky = l + L p).  Each lane group runs
+// the forward FFT of its layer across its lanes (then lane l holds the bins
+// ky = k1 + P bitrev(l)), puts them in LDS, and after one barrier each lane
+// reads the same bins of the other layers, applies its row of each bin's
+// block, and runs the inverse FFT of its layer, again in registers.  The
+// block rows are stored per (column, k1, entry) with the L lanes innermost
+// (Mreg), conjugated where the block's frequency is the conjugate of its
+// half-spectrum bin and with the DC columns zeroed; they are loaded at the
+// start, under the spectrum loads and the forward FFT.  Same arithmetic per
+// bin as the LDS kernel: y_m = sum_k M[m + k bs] x_k in the spectrum type,
+// scaled, rounded to FP32 for the inverse; r.z adds w * Re(conj(x_m) y_m)
+// (w = 1 on the kx = 0 and Nyquist columns, else 2; DC skipped), reduced per
+// lane group, then over the layers in order, into part_rz[s][column].
+// Workgroup: T = max(1, 256 / (bs L)) tasks, systems bx * T + t of column
+// by; blockDim.x = T bs L rounded up to 64 (idle lanes repeat the last lane
+// group and write nothing).
+// ---------------------------------------------------------------------------
+constexpr int COLR_MAXBS = 8;
+__host__ __device__ inline int colr_tasks(int bs, int L) { return bs * L >= 256 ? 1 : 256 / (bs * L); }
+
+template <typename FT, int P>
+__global__ __launch_bounds__(512) void k_colr(KARGS) {
+    KJOB;
+    const Scal SC = ldsc(PD.SC);
+    typedef typename CT<FT>::R FR;
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    const FT *specf = (const FT *)PD.specf;
+    const int L = G.shL, ny = G.fny, bs = G.bs;
+    const int T = colr_tasks(bs, L), per = bs * L;
+    const int t = min((int)threadIdx.x, T * per - 1);  // idle lanes: the last lane group again
+    const int ti = t / per, q = t - ti * per, m = q / L, l = q - m * L;
+    const bool lane_live = (int)threadIdx.x < T * per;
+    const int si = bx * T + ti;
+    const bool live = lane_live && si < J.nsys;
+    const int s = J.s0 + min(si, J.nsys - 1);
+    const int c = by, kx = ((const CONSTAS int *)G.colkx)[c];
+    const long sidx = (long)s * G.nspec + G.soff[m] + (long)kx * ny + l;
+    FT x[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) x[p] = specf[sidx + L * p];
+    // row m of the blocks of this lane's bins
+    const float2 *Mm = G.Mreg + (long)c * P * bs * bs * L + m * L + l;
+    float2 mrow[P][COLR_MAXBS];
+#pragma unroll
+    for (int k1 = 0; k1 < P; k1++)
+#pragma unroll
+        for (int k = 0; k < COLR_MAXBS; k++)
+            mrow[k1][k] = Mm[((long)k1 * bs * bs + min(k, bs - 1) * bs) * L];
+    fft_regs_fwd<P>(x, L, l, tw_of<FT>(G, true));
+    // bins of every layer of the task: [task][layer][k1][lane]
+    FT *X = reinterpret_cast<FT *>(lds_raw);
+    double *dsum = reinterpret_cast<double *>(X + (long)T * per * P);
+    FT *Xt = X + (long)ti * per * P;
+    if (lane_live)
+#pragma unroll
+        for (int k1 = 0; k1 < P; k1++) Xt[(m * P + k1) * L + l] = x[k1];
+    __syncthreads();
+    const int kr = L > 1 ? (int)(__builtin_bitreverse32((unsigned)l) >> (32 - (31 - __builtin_clz(L)))) : 0;
+    const FR sc = G.scale ? (FR)G.fscale : FR(1);
+    const int zd = G.zero_dc, d32 = G.dot32;
+    const double wgt = (kx == 0 || 2 * kx == G.fnx) ? 1.0 : 2.0;
+    float2 z[P];
+    double dot = 0;
+#pragma unroll
+    for (int k1 = 0; k1 < P; k1++) {
+        FT y = CT<FT>::make(0, 0);
+#pragma unroll
+        for (int k = 0; k < COLR_MAXBS; k++) {
+            if (k < bs) {
+                const float2 mm = mrow[k1][k];
+                y = cadd(y, cmul(CT<FT>::make(mm.x, mm.y), Xt[(k * P + k1) * L + l]));
+            }
+        }
+        const FT yy = CT<FT>::make(sc * y.x, sc * y.y);
+        z[k1] = make_float2((float)yy.x, (float)yy.y);
+        if (!(zd && kx == 0 && k1 == 0 && kr == 0))
+            dot = dfma(wgt, dfma(x[k1].x, yy.x, dmul(x[k1].y, yy.y, d32), d32), dot, d32);
+    }
+    // r.z: over the lane group, then over the layers in order
+    if (d32) {
+        float f = (float)dot;
+        for (int h = L >> 1; h >= 1; h >>= 1) f += __shfl_xor(f, h, 64);
+        dot = f;
+    } else {
+        for (int h = L >> 1; h >= 1; h >>= 1) dot += __shfl_xor(dot, h, 64);
+    }
+    if (lane_live && l == 0) dsum[ti * bs + m] = dot;
+    fft_regs_inv<P>(z, L, l, G.twy);
+    if (live) {
+        float2 *dst = PD.speci + sidx;
+#pragma unroll
+        for (int p = 0; p < P; p++) dst[L * p] = z[p];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < T && bx * T + (int)threadIdx.x < J.nsys) {
+        const double *d = dsum + threadIdx.x * bs;
+        double v;
+        if (d32) {
+            float f = 0.f;
+            for (int k = 0; k < bs; k++) f += (float)d[k];
+            v = (double)(f * (float)sc);
+        } else {
+            v = 0;
+            for (int k = 0; k < bs; k++) v += d[k];
+            v *= (double)sc;
+        }
+        SC.part_rz[(long)(J.s0 + bx * T + threadIdx.x) * SC.maxpart + c] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Inverse row kernel: complex-to-real row FFT -> z (the imaginary parts of
 // the DC and Nyquist bins are ignored, as C2R does), then by mode
 //   IROW_Z       out = z                                  (P alone)
@@ -1644,6 +1810,31 @@ void launch_col(const LaunchTab &t, int spec64, int cs, int bs, int rm, size_t l
         if (rm > 4) launch_col_r<float2, 7>(t, cs, bs, lds, st);
         else launch_col_r<float2, 3>(t, cs, bs, lds, st);
     }
+}
+template <typename FT>
+static void launch_colr_t(const LaunchTab &t, int P, int nthr, size_t lds, hipStream_t st) {
+    const dim3 grid(t.nwg), blk(nthr);
+    switch (P) {
+        case 2: hipLaunchKernelGGL((k_colr<FT, 2>), grid, blk, lds, st, KT); break;
+        case 3: hipLaunchKernelGGL((k_colr<FT, 3>), grid, blk, lds, st, KT); break;
+        case 4: hipLaunchKernelGGL((k_colr<FT, 4>), grid, blk, lds, st, KT); break;
+        case 5: hipLaunchKernelGGL((k_colr<FT, 5>), grid, blk, lds, st, KT); break;
+        default: hipLaunchKernelGGL((k_colr<FT, 7>), grid, blk, lds, st, KT); break;
+    }
+}
+bool colr_supported(int P, int bs) { return (P == 2 || P == 3 || P == 4 || P == 5 || P == 7) && bs >= 1 && bs <= COLR_MAXBS; }
+int colr_shape(int bs, int L, int *tasks) {
+    const int T = colr_tasks(bs, L);
+    if (tasks) *tasks = T;
+    return (T * bs * L + 63) / 64 * 64;
+}
+size_t colr_lds(int bs, int L, int P, int f64) {
+    const int T = colr_tasks(bs, L);
+    return (size_t)T * bs * L * P * (f64 ? sizeof(double2) : sizeof(float2)) + sizeof(double) * (size_t)T * bs;
+}
+void launch_colr(const LaunchTab &t, int spec64, int P, int nthr, size_t lds, hipStream_t st) {
+    if (spec64) launch_colr_t<double2>(t, P, nthr, lds, st);
+    else launch_colr_t<float2>(t, P, nthr, lds, st);
 }
 template <int RM>
 static void launch_irow_r(const LaunchTab &t, int mode, int slot, int osel, size_t lds, hipStream_t st) {

@@ -93,11 +93,14 @@ struct Geom {
     int nstx, nsty;               // FFT stages
     FStage fsx[MAXST], fsy[MAXST];
     int rmax;                     // largest radix of either axis
+    int shL, shP;                 // column FFT in registers: ny = shL lanes x shP points (0: none)
     FDiv dfnx, dfny, dnpsr;       // division by fnx, fny, npsr
     FDiv dsmall[ROWS / 2 + 1];    // division by 1 .. ROWS/2 (complex rows per block)
     const float2 *twx, *twy;      // exp(-2 pi i k/n)
     const double2 *twx64, *twy64; // same in FP64 (forward transform of the FP64 residual)
     const float2 *Minv;           // nbhalf blocks bs*bs, column-major, group order
+    const float2 *Mreg;           // the same blocks for the register column kernel: per (column,
+                                  // k1, entry) the shL lanes (bin k1 + shP bitrev(lane)), or null
     int ngroup;                   // column groups of the block apply
     const int *grp_col;           // per group: first column index into colkx
     const int *colkx;             // column kx list (per group, contiguous)

@@ -30,6 +30,15 @@ void launch_col(const LaunchTab &t, int spec64, int cs, int bs, int rm, size_t l
 // Raise the dynamic-LDS limit of the column kernel instantiation that a
 // launch of more than 64 KB will use (before any launch or capture).
 void col_allow_lds(int spec64, int cs, int bs, int rm, size_t lds);
+// Column stage with the column FFTs in registers (blocks of one frequency
+// across the layers): a task is one (system, column) on bs x L threads,
+// P = ny / L points per lane.  colr_supported: an instantiation exists for P
+// and the block size.  colr_shape: threads per workgroup (tasks per
+// workgroup in *tasks); colr_lds: its dynamic LDS bytes.
+bool colr_supported(int P, int bs);
+int colr_shape(int bs, int L, int *tasks);
+size_t colr_lds(int bs, int L, int P, int f64);
+void launch_colr(const LaunchTab &t, int spec64, int P, int nthr, size_t lds, hipStream_t st);
 void launch_irow(const LaunchTab &t, int mode, int rm, int slot, int osel, size_t lds, hipStream_t st);
 void launch_final(const LaunchTab &t, hipStream_t st);
 void launch_zero_stop(const LaunchTab &t, hipStream_t st);

@@ -95,6 +95,8 @@ namespace {
 // One launch of a stage: a template variant over a subset of the jobs.
 struct Sub {
     int matched = 0, pos = 1, cs = 1, bs = 0, rm = 3;
+    int reg = 0;  // column stage: P of the register kernel (0: the LDS kernel)
+    int nthr = 0; // its threads per workgroup
     size_t lds = 0;
     DevBuf<WGI> wgi;
     int nwg = 0;
@@ -138,7 +140,7 @@ struct gpu_tomo {
     DevBuf<float> gpx, gpy, ptt;
     DevBuf<float4> neai, nead;
     DevBuf<float2> gpd;
-    DevBuf<float2> twx, twy, Minv;
+    DevBuf<float2> twx, twy, Minv, Mreg;
     DevBuf<double2> twx64, twy64;
     DevBuf<int> grp_col, colkx, grp_blk, blkpos;
     std::vector<int> blk_order;  // half-spectrum block id of each block in group order
@@ -167,6 +169,8 @@ struct gpu_tomo {
     // systems per workgroup chosen by the autotuner (0: heuristic)
     int tune_wfs = 0, tune_layer = 0, tune_col = 0;
     int tune_lty = 0;     // layer tile rows: 0 = heuristic (LTY where it fits), TILE or LTY
+    int tune_colfft = 0;  // column kernel: 0 = registers where available, 1 = registers, 2 = LDS stages
+    bool colr_ok = false; // the register column kernel applies (per-frequency blocks, ny = shL x shP)
     bool lty_ok = false;  // LTY-row tiles fit every layer of this (matched) plan
     // profiling
     int prof = 0;
@@ -382,6 +386,62 @@ std::vector<float2> col_blocks(const mt_recon_t *R, const gpu_tomo *g) {
                 if (g->blkpos_host[j * bs + k] & BP_DC)
                     for (long m = 0; m < bs; m++) M[j * bs * bs + m + k * bs] = make_float2(0.f, 0.f);
     return M;
+}
+
+// Blocks for the register column kernel, which applies where every block
+// couples one half-spectrum bin (kx, ky) across the layers (member m on layer
+// m, one conjugation flag) and every column is a group of its own: per
+// (column c, k1, entry e) the L lanes, lane l holding the block of the bin
+// ky = k1 + P bitrev_L(l), conjugated where the block's frequency is the
+// conjugate of its bin (conj(M) x = conj(M conj(x))).  Otherwise the plan
+// keeps the LDS column kernel only (colr_ok false, shL 0).
+void build_mreg(gpu_tomo *g, const std::vector<float2> &hM, const std::vector<int> &hgb) {
+    Geom &G = g->G;
+    g->colr_ok = false;
+    G.Mreg = nullptr;
+    const int L = G.shL, P = G.shP, ny = G.fny, bs = G.bs, ngr = G.ngroup;
+    auto none = [&] { G.shL = G.shP = 0; };
+    if (!L || G.maxcol != 1 || bs != G.npsr) return none();
+    std::vector<int> jof((size_t)ngr * ny, -1);  // block of (column, ky)
+    std::vector<char> cjof((size_t)ngr * ny, 0);
+    for (int c = 0; c < ngr; c++) {
+        for (int j = hgb[c]; j < hgb[c + 1]; j++) {
+            const int p0 = g->blkpos_host[(size_t)j * bs];
+            const int ky = (p0 & BP_POS) % ny;
+            for (int m = 0; m < bs; m++) {
+                const int pe = g->blkpos_host[(size_t)j * bs + m];
+                if ((pe & BP_POS) != m * ny + ky || (pe & BP_NOWRITE) || ((pe ^ p0) & BP_CONJ)) return none();
+            }
+            if (jof[(size_t)c * ny + ky] >= 0) return none();
+            jof[(size_t)c * ny + ky] = j;
+            cjof[(size_t)c * ny + ky] = (p0 & BP_CONJ) != 0;
+        }
+        for (int ky = 0; ky < ny; ky++)
+            if (jof[(size_t)c * ny + ky] < 0) return none();
+    }
+    int lb = 0;
+    while ((1 << lb) < L) lb++;
+    auto bitrev = [&](int l) {
+        int r = 0;
+        for (int b = 0; b < lb; b++) r |= ((l >> b) & 1) << (lb - 1 - b);
+        return r;
+    };
+    const size_t nb2 = (size_t)bs * bs;
+    std::vector<float2> Mr((size_t)ngr * ny * nb2);
+    for (int c = 0; c < ngr; c++)
+        for (int k1 = 0; k1 < P; k1++)
+            for (int l = 0; l < L; l++) {
+                const size_t q = (size_t)c * ny + k1 + (size_t)P * bitrev(l);
+                const float2 *blk = hM.data() + (size_t)jof[q] * nb2;
+                for (size_t e = 0; e < nb2; e++) {
+                    float2 v = blk[e];
+                    if (cjof[q]) v.y = -v.y;
+                    Mr[(((size_t)c * P + k1) * nb2 + e) * L + l] = v;
+                }
+            }
+    g->Mreg.upload(Mr);
+    G.Mreg = g->Mreg.p;
+    g->colr_ok = true;
 }
 
 void build_plan(gpu_tomo *g, const mt_recon_t *R) {
@@ -691,6 +751,17 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
         G.dfny = make_fdiv((unsigned)ny0);
         G.dnpsr = make_fdiv((unsigned)R->npsr);
         for (int d = 0; d <= ROWS / 2; d++) G.dsmall[d] = make_fdiv((unsigned)std::max(d, 1));
+        // register layout of the column FFT: ny = L lanes x P points, L a
+        // power of two dividing 64 (the widest first), P an in-lane radix
+        G.shL = G.shP = 0;
+        for (int L : {32, 64, 16, 8, 4}) {
+            const int P = ny0 % L ? 0 : ny0 / L;
+            if (colr_supported(P, R->fd_bs)) {
+                G.shL = L;
+                G.shP = P;
+                break;
+            }
+        }
         G.rmax = 2;
         for (int r : rx) G.rmax = std::max(G.rmax, r);
         for (int r : ry) G.rmax = std::max(G.rmax, r);
@@ -798,8 +869,10 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
         G.grp_blk = g->grp_blk.p;
         G.blkpos = g->blkpos.p;
         if ((long)G.maxcol * R->npsr * ny0 > BP_POS) fail(MT_EINVAL, "FDPCG column group too large");
-        g->Minv.upload(col_blocks(R, g));
+        const std::vector<float2> hM = col_blocks(R, g);
+        g->Minv.upload(hM);
         G.Minv = g->Minv.p;
+        build_mreg(g, hM, hgb);
         if (col_lds(G.maxcol, G.npsr, G.fny, 1, 1) > 64 * 1024)
             fail(MT_EINVAL, "FDPCG column group too large for LDS");
     }
@@ -945,17 +1018,18 @@ struct JobGrid {
 // one L2 instead of being fetched by all eight.  Buckets go to the least
 // loaded XCD in order; positions whose XCD has run out take the longest
 // remaining queue, so a table with fewer buckets than XCDs keeps its order.
-// by_x: bucket by (job, x block) instead, for the column kernel, whose
-// shared data (the FDPCG blocks of a column group) follows x.
+// by: the bucket's block index, XCD_BY_Z (system group) by default;
+// XCD_BY_X / XCD_BY_Y for the column kernels, whose shared data (the FDPCG
+// blocks of a column group) follows x (LDS kernel) or y (register kernel).
 constexpr int NXCD = 8;
-std::vector<WGI> xcd_order(std::vector<WGI> tab, bool by_x) {
-    if (by_x)
-        std::stable_sort(tab.begin(), tab.end(), [](const WGI &a, const WGI &b) {
-            return a.job != b.job ? a.job < b.job : a.bx < b.bx;
+enum { XCD_BY_Z = 0, XCD_BY_X = 1, XCD_BY_Y = 2 };
+std::vector<WGI> xcd_order(std::vector<WGI> tab, int by) {
+    auto bkey = [&](const WGI &a) { return by == XCD_BY_X ? a.bx : by == XCD_BY_Y ? a.by : a.bz; };
+    if (by != XCD_BY_Z)
+        std::stable_sort(tab.begin(), tab.end(), [&](const WGI &a, const WGI &b) {
+            return a.job != b.job ? a.job < b.job : bkey(a) < bkey(b);
         });
-    auto same = [&](const WGI &a, const WGI &b) {
-        return a.job == b.job && (by_x ? a.bx == b.bx : a.bz == b.bz);
-    };
+    auto same = [&](const WGI &a, const WGI &b) { return a.job == b.job && bkey(a) == bkey(b); };
     std::vector<std::vector<WGI>> q(NXCD);
     std::vector<size_t> load(NXCD, 0);
     size_t i = 0;
@@ -984,7 +1058,8 @@ std::vector<WGI> xcd_order(std::vector<WGI> tab, bool by_x) {
 
 // Partition the jobs by a variant key and emit one Sub per partition.
 template <typename KeyF, typename GridF, typename SubF>
-void build_stage(Batch &B, Stage &out, KeyF key, GridF grid, SubF fill, bool xcd_by_x = false) {
+void build_stage(Batch &B, Stage &out, KeyF key, GridF grid, SubF fill,
+                 std::function<int(const Sub &)> xcd_by = nullptr) {
     std::map<std::vector<int>, std::vector<int>> parts;
     for (size_t j = 0; j < B.hjob.size(); j++) {
         std::vector<int> k;
@@ -1003,7 +1078,7 @@ void build_stage(Batch &B, Stage &out, KeyF key, GridF grid, SubF fill, bool xcd
                         tab.push_back(WGI{(unsigned short)j, (unsigned short)x, (unsigned short)y, (unsigned short)z});
         }
         if (tab.empty()) continue;
-        tab = xcd_order(std::move(tab), xcd_by_x);
+        tab = xcd_order(std::move(tab), xcd_by ? xcd_by(*sub) : XCD_BY_Z);
         sub->nwg = (int)tab.size();
         sub->wgi.upload(tab);
         out.push_back(std::move(sub));
@@ -1122,15 +1197,37 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
                         sb.rm = k[0];
                         for (int j : js) sb.lds = std::max(sb.lds, row_lds(G(j).fnx, 0));
                     });
+        // register kernel where the plan allows it and the tuning does not
+        // ask for the LDS kernel; its jobs share a launch per (P, threads)
+        auto colreg = [&](int j) { return b.plan[j]->colr_ok && b.plan[j]->tune_colfft != 2 ? G(j).shP : 0; };
+        auto colr_tasks_of = [&](int j) {
+            int T = 1;
+            colr_shape(G(j).bs, G(j).shL, &T);
+            return T;
+        };
         build_stage(b, b.col,
                     [&](int j, std::vector<int> &k) {
+                        if (colreg(j)) {
+                            k = {-1, colreg(j), colr_shape(G(j).bs, G(j).shL, nullptr)};
+                            return true;
+                        }
                         const int bs = G(j).bs;
                         const int bk = (bs == 2 || bs == 3 || bs == 4 || bs == 6 || bs == 8) ? bs : 0;
                         k = {merge ? bs_all : bk, rm_key(j)};
                         return true;
                     },
-                    [&](const Sub &sb, int j) { return JobGrid{G(j).ngroup, 1, ceil_div(NS(j), sb.cs)}; },
+                    [&](const Sub &sb, int j) {
+                        return sb.reg ? JobGrid{ceil_div(NS(j), colr_tasks_of(j)), G(j).ngroup, 1}
+                                      : JobGrid{G(j).ngroup, 1, ceil_div(NS(j), sb.cs)};
+                    },
                     [&](Sub &sb, const std::vector<int> &k, const std::vector<int> &js) {
+                        if (k[0] == -1) {
+                            sb.reg = k[1];
+                            sb.nthr = k[2];
+                            for (int j : js)
+                                sb.lds = std::max(sb.lds, colr_lds(G(j).bs, G(j).shL, sb.reg, b.spec64));
+                            return;
+                        }
                         sb.bs = k[0];
                         sb.rm = k[1];
                         // up to 4 systems per workgroup as far as the device's LDS
@@ -1151,7 +1248,7 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
                             sb.lds = std::max(sb.lds, col_lds(G(j).maxcol, G(j).npsr, G(j).fny, b.spec64, sb.cs));
                         if (sb.lds > 64 * 1024) col_allow_lds(b.spec64, sb.cs, sb.bs, sb.rm, sb.lds);
                     },
-                    true);
+                    [](const Sub &sb) { return sb.reg ? XCD_BY_Y : XCD_BY_X; });
     }
     build_stage(b, b.fin, [&](int, std::vector<int> &) { return true; },
                 [&](const Sub &, int j) { return JobGrid{1, 1, NS(j)}; },
@@ -1210,7 +1307,10 @@ void enqueue_row(Batch &B, int kc, int mode, int iter, int xsel, Rec &rec) {
 // the spectral r.z partials, then the inverse row pass in the given mode.
 void enqueue_P_tail(Batch &B, int irow_mode, int slot, int osel, Rec &rec) {
     rec(KC_COL, [&] {
-        for (auto &s : B.col) launch_col(tab(B, *s), B.spec64, s->cs, s->bs, s->rm, s->lds, rec.st);
+        for (auto &s : B.col) {
+            if (s->reg) launch_colr(tab(B, *s), B.spec64, s->reg, s->nthr, s->lds, rec.st);
+            else launch_col(tab(B, *s), B.spec64, s->cs, s->bs, s->rm, s->lds, rec.st);
+        }
     });
     rec(irow_mode == IROW_MODE_UPDATE ? KC_IROW : KC_IROW_START, [&] {
         for (auto &s : B.irow) launch_irow(tab(B, *s), irow_mode, s->rm, slot, osel, s->lds, rec.st);
@@ -1669,6 +1769,7 @@ int gpu_tomo_tune_set(gpu_tomo_t *g, const int *choice) {
     for (int i = 0; i < 3; i++)
         if (choice[i] != 0 && choice[i] != 1 && choice[i] != 2 && choice[i] != 4) return MT_EINVAL;
     if (choice[3] != 0 && choice[3] != TILE && choice[3] != LTY) return MT_EINVAL;
+    if (choice[4] < 0 || choice[4] > 2) return MT_EINVAL;
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
         HIPCHK(hipDeviceSynchronize());
@@ -1676,6 +1777,7 @@ int gpu_tomo_tune_set(gpu_tomo_t *g, const int *choice) {
         g->tune_layer = choice[1];
         g->tune_col = choice[2];
         g->tune_lty = choice[3];
+        g->tune_colfft = choice[4];
         drop_batches(g);
         apply_lty(g);
     });
@@ -1737,7 +1839,29 @@ int gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice) {
             g->tune_layer = bestc;
             apply_lty(g);
         }
-        Knob knobs[2] = {{&g->tune_wfs, KC_WFS, g->G.pos == 1 ? 4 : 2}, {&g->tune_col, KC_COL, colmax}};
+        // the column stage: register kernel (where it applies; one system
+        // per lane group, no systems-per-workgroup choice) or the LDS kernel
+        // with 1, 2 or 4 systems per workgroup
+        {
+            double best = 1e30;
+            int bestf = 2, bestc = 1;
+            for (int f : {1, 2}) {
+                if (f == 1 && !g->colr_ok) continue;
+                g->tune_colfft = f;
+                for (int c = 1; c <= (f == 1 ? 1 : colmax); c <<= 1) {
+                    g->tune_col = c;
+                    const double t = stage_ms(KC_COL);
+                    if (t < best * 0.98) {
+                        best = t;
+                        bestf = f;
+                        bestc = c;
+                    }
+                }
+            }
+            g->tune_colfft = bestf;
+            g->tune_col = bestf == 1 ? 0 : bestc;
+        }
+        Knob knobs[1] = {{&g->tune_wfs, KC_WFS, g->G.pos == 1 ? 4 : 2}};
         for (auto &k : knobs) {
             int bestc = 1;
             double bestt = 1e30;
@@ -1758,6 +1882,7 @@ int gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice) {
             choice[1] = g->tune_layer;
             choice[2] = g->tune_col;
             choice[3] = g->G.lty;
+            choice[4] = g->tune_colfft;
         }
     });
 }

@@ -16,21 +16,24 @@ def _batch(r, cfg, n):
     return np.stack([workload.make_inputs(cfg, r, 40 + s) for s in range(n)])
 
 
+@pytest.mark.parametrize("colk", [1, 2])
 @pytest.mark.parametrize("mode", [0, 1, 3])
-def test_systems_per_workgroup_bit_exact(large30, mode):
+def test_systems_per_workgroup_bit_exact(large30, mode, colk):
     """Five systems (one partial workgroup for every choice above 1) solved
-    under each systems-per-workgroup choice and by the heuristic: identical
-    solutions and residuals."""
+    under each systems-per-workgroup choice and by the heuristic, with the
+    register column kernel (1, which has no such choice) and the LDS column
+    kernel (2): identical solutions and residuals."""
     cfg, r = large30
     grads = _batch(r, cfg, 5)
     g = tomo.GpuTomo(r, nbatch=5, maxit=cfg["maxit"])
     g.precision(mode)
+    g.tune_set(0, 0, 0, 0, colk)
     ref = g.tomo_batch(grads)
     ref_res = g.residuals(5)
     # (column 2 with an FP32 spectrum once launched the 4-system kernel on a
     # grid and LDS sized for 2)
     for choice in [(1, 1, 1), (2, 2, 2), (4, 4, 2), (2, 1, 2), (1, 1, 2), (1, 1, 4)]:
-        g.tune_set(*choice)
+        g.tune_set(*choice, 0, colk)
         x = g.tomo_batch(grads)
         print("choice", choice, "max |dx|", float(np.abs(x - ref).max()), flush=True)
         assert np.array_equal(x, ref), "systems per workgroup %s changed the solution" % (choice,)
@@ -45,6 +48,8 @@ def test_tune_set_rejects_bad_choice(large30):
         g.tune_set(3, 1, 1)  # 3 is not a choice
     with pytest.raises(_abi.MtError):
         g.tune_set(1, 1, 1, 32)  # nor are 32-row tiles
+    with pytest.raises(_abi.MtError):
+        g.tune_set(1, 1, 1, 0, 3)  # nor a third column FFT path
     g.close()
 
 
@@ -59,7 +64,8 @@ def test_autotune_replay(large30):
     ref = g.tomo_batch(grads)
     choice = g.autotune(3)
     print("autotuned", choice, flush=True)
-    assert all(c in (1, 2, 4) for c in choice[:3]) and choice[3] in (16, 64)
+    assert all(c in (1, 2, 4) for c in choice[:2]) and choice[3] in (16, 64) and choice[4] in (1, 2)
+    assert choice[2] in ((0,) if choice[4] == 1 else (1, 2, 4))
     x = g.tomo_batch(grads)
     assert np.linalg.norm(x - ref) <= 1e-5 * np.linalg.norm(ref)
     g2 = tomo.GpuTomo(r, nbatch=3, maxit=cfg["maxit"])
@@ -107,4 +113,40 @@ def test_profile_repeats_keep_results(large30):
     assert prof["k_col"][1] == 8 * it
     assert prof["k_layer"][1] == 8 * it
     assert prof["k_row"][1] == it - 1
+    g.close()
+
+
+@pytest.mark.parametrize("cfgname,path", [("large30", 1), ("large30", 2), ("small8", 1), ("small8", 2),
+                                           ("b8_w2_l4", 1), ("b8_w2_l4", 2), ("b8_w5_l5", 1),
+                                           ("b10_w3_l2", 1)])
+def test_column_fft_paths_parity(cfgname, path):
+    """The register column kernel (path 1: column FFTs across lanes, blocks
+    and inverse FFTs in registers) and the LDS column kernel (path 2), against
+    the FP64 oracle: one preconditioner apply and the 30-iteration solve.
+    The register kernel applies to large30 (96 = 32 x 3 points, 6 layers) and
+    the 8 m members (28 = 4 x 7, 4 and 5 layers); small8 (blocks coupling
+    aliased frequencies) takes the LDS kernel either way."""
+    from maos_amd import configs
+    from oracle.oracle import Oracle
+    cfg = {"large30": configs.large30, "small8": configs.small8,
+           "b8_w2_l4": lambda: configs.batch_member(8.0, 2, 4),
+           "b8_w5_l5": lambda: configs.batch_member(8.0, 5, 5),
+           "b10_w3_l2": lambda: configs.batch_member(10.0, 3, 2)}[cfgname]()
+    r = tomo.Recon(cfg)
+    o = Oracle(r, nthread=8)
+    g = tomo.GpuTomo(r, maxit=cfg["maxit"])
+    g.tune_set(0, 0, 0, 0, path)
+    print(cfgname, "info", g.info(), flush=True)
+    x = np.random.default_rng(6).standard_normal(r.nx)
+    off = r.layer_offsets()
+    for l in range(r.npsr):
+        x[off[l]:off[l + 1]] -= x[off[l]:off[l + 1]].mean()
+    ep = float(np.linalg.norm(g.P(x) - o.fdpcg(x)) / np.linalg.norm(o.fdpcg(x)))
+    grad = workload.make_inputs(cfg, r, 1)
+    xg, resg, rc = g.tomo(grad)
+    xo, reso = o.solve(grad, cfg["maxit"])
+    e = float(np.linalg.norm(xg - xo) / np.linalg.norm(xo))
+    print("%s column fft %d: P rel %.3e, solve rel %.3e" % (cfgname, path, ep, e), flush=True)
+    assert ep <= 2e-5
+    assert e <= (1e-4 if cfgname in ("large30", "small8") else 1e-3)
     g.close()
