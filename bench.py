@@ -217,6 +217,7 @@ def bench_sweep(r, grads, maxit, sizes, copy_gbs):
         out[str(B)] = dict(solves_per_s=B * len(lat) / sum(lat), p99_ms=1e3 * float(np.percentile(lat, 99)),
                            systems_per_workgroup=dict(wfs=tuned[0], layer=tuned[1], col=tuned[2]),
                            layer_tile_rows=tuned[3], column_fft={1: "registers", 2: "lds"}.get(tuned[4]),
+                           flights=tuned[5],
                            kernels={k: dict(avg_us=round(v["avg_us"], 2), GBs=round(v["GBs"], 1),
                                             frac_of_peak=round(v["GBs"] / HBM_PEAK_GBS, 4),
                                             frac_of_copy=round(v["GBs"] / copy_gbs, 4))
@@ -474,7 +475,7 @@ def main():
                     help="comma list of modes to time (profiling runs use one)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of 1 GiB (known HBM traffic) for counter scaling")
-    ap.add_argument("--systems-per-workgroup", metavar="W,L,C[,ROWS[,FFT]]", default=None,
+    ap.add_argument("--systems-per-workgroup", metavar="W,L,C[,ROWS[,COLK[,FLIGHTS]]]", default=None,
                     help="fix the systems per workgroup of the WFS, layer and column kernels (and the layer "
                          "tile rows) instead of autotuning (profiling runs, whose timing the counters distort)")
     ap.add_argument("--traffic-summary", metavar="DIR", default=None,
@@ -621,7 +622,8 @@ def main():
                    "batch": B, "global_batch": B, "parallelism": "batch of independent solves",
                    "autotuned_systems_per_workgroup": {"wfs": tuned[0], "layer": tuned[1], "col": tuned[2]},
                    "layer_tile_rows": tuned[3] if len(tuned) > 3 else None,
-                   "column_fft": {1: "registers", 2: "lds"}.get(tuned[4]) if len(tuned) > 4 else None},
+                   "column_fft": {1: "registers", 2: "lds"}.get(tuned[4]) if len(tuned) > 4 else None,
+                   "flights": tuned[5] if len(tuned) > 5 else None},
         "p50_ms": round(res_batch["p50_ms"], 4),
         "p99_ms": round(res_batch["p99_ms"], 4),
         "modes": {"plain": res_plain, "graph": res_graph, "batched": res_batch},

@@ -233,7 +233,12 @@ int  gpu_tomo_solve_batch(gpu_tomo_t *g, int nsys, const double *grad,
  * d_grad = nbatch stacked gradient vectors, d_x = nbatch stacked layer
  * vectors (in: warm start when warm=1; out: solution).  Enqueued on
  * `stream` (a hipStream_t, 0 = the plan's own stream); no host sync.
- * mode: 0 = one launch per kernel (plain), 1 = replay a captured hipGraph. */
+ * mode: 0 = one launch per kernel (plain), 1 = replay captured hipGraphs:
+ * the nbatch systems as k flights over consecutive system ranges (k from
+ * gpu_tomo_tune_set / gpu_tomo_autotune, default 1), each graph on a stream
+ * of the plan, forked from and joined back to `stream` by events, so their
+ * kernels can run concurrently (when the streams' hardware queues differ);
+ * results do not depend on k. */
 int  gpu_tomo_solve_dev(gpu_tomo_t *g, const float *d_grad, float *d_x,
                         int nbatch, int mode, void *stream);
 /* Residual sqrt(r'r/b'b) of each system after the last solve (host copy). */
@@ -322,20 +327,22 @@ void gpu_tomo_server_free(gpu_tomo_server_t *s);
  * layers, at most 6, and the column length is 2^a x P with P in 2, 3, 4, 5,
  * 7; or in LDS stages, always available) and each choice of systems per
  * workgroup (1, 2, 4) of the WFS, layer and LDS column kernels, and keeps
- * the fastest for this plan.  choice[5] receives wfs, layer, col (0 with
- * the register column kernel), the layer tile rows (16 or 64) and the
- * column kernel (1 registers, 2 LDS stages).  Overwrites the gradients and
- * solutions of slots [0, nsys).  Call it before the plan joins a group.
+ * the fastest for this plan; then times whole graph solves of the nsys
+ * systems as 1, 2 and 4 flights (gpu_tomo_solve_dev mode 1).  choice[6]
+ * receives wfs, layer, col (0 with the register column kernel), the layer
+ * tile rows (16 or 64), the column kernel (1 registers, 2 LDS stages) and
+ * the flights.  Overwrites the gradients and solutions of slots [0, nsys).  Call it before the plan joins a group.
  * The tile shape and the column kernel set the order of sums, so they can
  * change results in the last bits; systems per workgroup never do.  No
  * reference counterpart. */
 int  gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice);
 
 /* Set the launch shape directly, e.g. to replay an autotuned choice without
- * timing: choice[5] = systems per workgroup of the WFS, layer and LDS
+ * timing: choice[6] = systems per workgroup of the WFS, layer and LDS
  * column kernels (each 1, 2 or 4, or 0 for the built-in heuristic), the
- * layer tile rows (16, 64, or 0 for 64 where it fits) and the column kernel
- * (1 registers where it applies, 2 LDS stages, 0 as 1).  Values above a
+ * layer tile rows (16, 64, or 0 for 64 where it fits), the column kernel
+ * (1 registers where it applies, 2 LDS stages, 0 as 1) and the flights of a
+ * graph solve (1, 2, 4, or 0 for 1).  Values above a
  * kernel's maximum (WFS 2 when tomo.pos = 2, column 2 in precision modes 1
  * and 2) are clamped when the launch is built.  Call it before the plan
  * joins a group.  No reference counterpart. */

@@ -309,31 +309,42 @@ template <typename C> __device__ __forceinline__ C shfl_xor_c(C v, int m) {
     return CT<C>::make(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
 }
 template <typename C> __device__ __forceinline__ C conj_c(C v) { return CT<C>::make(v.x, -v.y); }
+// A butterfly is written without a select between its two results: the
+// lane adds s * (own value) to its partner's value (s = +1 on the lower lane
+// of the pair, -1 on the upper, so the pair forms a + b and a - b exactly)
+// and multiplies by w' (the stage twiddle on the lane that takes it, else
+// 1, an exact product).  q tracks L / (2h) by shifts.
 template <int P, typename C>
 __device__ __forceinline__ void fft_regs_fwd(C *y, int L, int l, const C *tw) {
+    typedef typename CT<C>::R F;
     if constexpr (P > 1) small_dft<P, -1, C>(y);
 #pragma unroll
     for (int k1 = 1; k1 < P; k1++) y[k1] = cmul(y[k1], tw[l * k1]);
-    for (int h = L >> 1; h >= 1; h >>= 1) {
+    for (int h = L >> 1, q = 1; h >= 1; h >>= 1, q <<= 1) {
         const bool up = (l & h) != 0;
-        const C w = tw[(l & (h - 1)) * (L / (2 * h)) * P];
+        const C wt = tw[(l & (h - 1)) * q * P];
+        const C w = up ? wt : CT<C>::make(1, 0);
+        const F sg = up ? F(-1) : F(1);
 #pragma unroll
         for (int p = 0; p < P; p++) {
             const C o = shfl_xor_c(y[p], h);
-            y[p] = up ? cmul(csub(o, y[p]), w) : cadd(y[p], o);
+            y[p] = cmul(CT<C>::make(o.x + sg * y[p].x, o.y + sg * y[p].y), w);
         }
     }
 }
 template <int P, typename C>
 __device__ __forceinline__ void fft_regs_inv(C *y, int L, int l, const C *tw) {
-    for (int h = 1; h < L; h <<= 1) {
+    typedef typename CT<C>::R F;
+    for (int h = 1, q = L >> 1; h < L; h <<= 1, q >>= 1) {
         const bool up = (l & h) != 0;
-        const C w = conj_c(tw[(l & (h - 1)) * (L / (2 * h)) * P]);
+        const C wt = conj_c(tw[(l & (h - 1)) * q * P]);
+        const C w = up ? wt : CT<C>::make(1, 0);
+        const F sg = up ? F(-1) : F(1);
 #pragma unroll
         for (int p = 0; p < P; p++) {
-            const C t = up ? cmul(y[p], w) : y[p];
+            const C t = cmul(y[p], w);
             const C o = shfl_xor_c(t, h);
-            y[p] = up ? csub(o, t) : cadd(t, o);
+            y[p] = CT<C>::make(o.x + sg * t.x, o.y + sg * t.y);
         }
     }
 #pragma unroll
@@ -1407,9 +1418,9 @@ __global__ __launch_bounds__(512) void k_colr(KARGS) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const FT *specf = (const FT *)PD.specf;
     const int L = G.shL, ny = G.fny, bs = G.bs;
-    const int T = colr_tasks(bs, L), per = bs * L;
+    const int T = colr_tasks(bs, L), per = bs * L, lb = 31 - __builtin_clz(L);
     const int t = min((int)threadIdx.x, T * per - 1);  // idle lanes: the last lane group again
-    const int ti = t / per, q = t - ti * per, m = q / L, l = q - m * L;
+    const int ti = t / per, q = t - ti * per, m = q >> lb, l = q & (L - 1);
     const bool lane_live = (int)threadIdx.x < T * per;
     const int si = bx * T + ti;
     const bool live = lane_live && si < J.nsys;
@@ -1436,7 +1447,7 @@ __global__ __launch_bounds__(512) void k_colr(KARGS) {
 #pragma unroll
         for (int k1 = 0; k1 < P; k1++) Xt[(m * P + k1) * L + l] = x[k1];
     __syncthreads();
-    const int kr = L > 1 ? (int)(__builtin_bitreverse32((unsigned)l) >> (32 - (31 - __builtin_clz(L)))) : 0;
+    const int kr = L > 1 ? (int)(__builtin_bitreverse32((unsigned)l) >> (32 - lb)) : 0;
     const FR sc = G.scale ? (FR)G.fscale : FR(1);
     const int zd = G.zero_dc, d32 = G.dot32;
     const double wgt = (kx == 0 || 2 * kx == G.fnx) ? 1.0 : 2.0;

@@ -162,6 +162,20 @@ struct gpu_tomo {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_nsys = 0;
+    // flights: a graph solve of n systems as k graphs over consecutive
+    // system ranges, each on its own stream, forked from and joined to the
+    // caller's stream (the kernels of one flight fill the ramps and tails of
+    // the others'); graphs per (n, k), streams and events kept
+    struct Flight {
+        hipStream_t st = nullptr;
+        hipEvent_t done = nullptr;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::vector<Flight> fl;
+    int fl_nsys = 0, fl_k = 0;
+    hipEvent_t fork = nullptr;
+    int tune_flights = 0;  // 1, 2 or 4 (0: 1)
     // device copy of geometry + buffers, and the launch tables of this plan's
     // own system ranges (key: first system, count)
     DevBuf<PlanDev> pdev;
@@ -1272,6 +1286,13 @@ void drop_batches(gpu_tomo *g) {
     if (g->graph) (void)hipGraphDestroy(g->graph);
     g->gexec = nullptr;
     g->graph = nullptr;
+    for (auto &f : g->fl) {
+        if (f.exec) (void)hipGraphExecDestroy(f.exec);
+        if (f.graph) (void)hipGraphDestroy(f.graph);
+        f.exec = nullptr;
+        f.graph = nullptr;
+    }
+    g->fl_nsys = g->fl_k = 0;
 }
 
 LaunchTab tab(const Batch &B, const Sub &s) { return LaunchTab{B.jobs.p, s.wgi.p, s.nwg}; }
@@ -1522,6 +1543,11 @@ void gpu_tomo_free(gpu_tomo_t *g) {
         (void)hipEventDestroy(e.a);
         (void)hipEventDestroy(e.b);
     }
+    for (auto &f : g->fl) {
+        if (f.st) (void)hipStreamDestroy(f.st);
+        if (f.done) (void)hipEventDestroy(f.done);
+    }
+    if (g->fork) (void)hipEventDestroy(g->fork);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
 }
@@ -1606,6 +1632,60 @@ void capture_solve(Batch &B, hipGraph_t *graph, hipGraphExec_t *exec, EvList *ev
     capture_solve_with(B, graph, exec, evs, [](hipStream_t) {}, [](hipStream_t) {});
 }
 
+// Flights of a graph solve of n systems: the tuned count (default 1),
+// never more than n.  Whether flights overlap depends on the hardware queues
+// their streams land on: streams share a small pool of queues (least used
+// first), so two flights can share one and run one after the other,
+// depending on the other streams the process holds.  Hence no default above
+// 1: the autotuner times 1, 2 and 4 and keeps more only where they gain.
+int flights_for(const gpu_tomo *g, int n) { return std::min(std::max(g->tune_flights, 1), n); }
+// The graph solve of systems [0, n) on stream st: one graph, or k flights
+// over consecutive ranges (the first n % k ranges one system longer).
+void graph_solve(gpu_tomo *g, int n, hipStream_t st) {
+    const int k = flights_for(g, n);
+    if (k == 1) {
+        if (!g->gexec || g->graph_nsys != n) {
+            if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+            if (g->graph) (void)hipGraphDestroy(g->graph);
+            g->gexec = nullptr;
+            g->graph = nullptr;
+            capture_solve(own_batch(g, 0, n), &g->graph, &g->gexec, &g->evs);
+            g->graph_nsys = n;
+        }
+        HIPCHK(hipGraphLaunch(g->gexec, st));
+        return;
+    }
+    if (!g->fork) HIPCHK(hipEventCreateWithFlags(&g->fork, hipEventDisableTiming));
+    if ((int)g->fl.size() < k) g->fl.resize(k);
+    if (g->fl_nsys != n || g->fl_k != k) {
+        for (auto &f : g->fl) {
+            if (f.exec) (void)hipGraphExecDestroy(f.exec);
+            if (f.graph) (void)hipGraphDestroy(f.graph);
+            f.exec = nullptr;
+            f.graph = nullptr;
+        }
+        g->fl_nsys = g->fl_k = 0;
+        for (int i = 0, s0 = 0; i < k; i++) {
+            const int ni = n / k + (i < n % k);
+            auto &f = g->fl[i];
+            if (!f.st) HIPCHK(hipStreamCreateWithFlags(&f.st, hipStreamNonBlocking));
+            if (!f.done) HIPCHK(hipEventCreateWithFlags(&f.done, hipEventDisableTiming));
+            capture_solve(own_batch(g, s0, ni), &f.graph, &f.exec, &g->evs);
+            s0 += ni;
+        }
+        g->fl_nsys = n;
+        g->fl_k = k;
+    }
+    HIPCHK(hipEventRecord(g->fork, st));
+    for (int i = 0; i < k; i++) {
+        auto &f = g->fl[i];
+        HIPCHK(hipStreamWaitEvent(f.st, g->fork, 0));
+        HIPCHK(hipGraphLaunch(f.exec, f.st));
+        HIPCHK(hipEventRecord(f.done, f.st));
+    }
+    for (int i = 0; i < k; i++) HIPCHK(hipStreamWaitEvent(st, g->fl[i].done, 0));
+}
+
 int gpu_tomo_solve_dev(gpu_tomo_t *g, const float *d_grad, float *d_x, int nbatch, int mode, void *stream) {
     if (!g || nbatch <= 0 || nbatch > g->nbatch) return MT_EINVAL;
     return guarded([&] {
@@ -1614,20 +1694,11 @@ int gpu_tomo_solve_dev(gpu_tomo_t *g, const float *d_grad, float *d_x, int nbatc
         const Geom &G = g->G;
         if (d_grad && d_grad != g->g.p) launch_copy(d_grad, g->g.p, (long)nbatch * G.ngtot, st);
         if (d_x && d_x != g->x.p && g->warm) launch_copy(d_x, g->x.p, (long)nbatch * G.nxtot, st);
-        Batch &B = own_batch(g, 0, nbatch);
         if (mode == 1 && !g->prof) {
-            if (!g->gexec || g->graph_nsys != nbatch) {
-                if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
-                if (g->graph) (void)hipGraphDestroy(g->graph);
-                g->gexec = nullptr;
-                g->graph = nullptr;
-                capture_solve(B, &g->graph, &g->gexec, &g->evs);
-                g->graph_nsys = nbatch;
-            }
-            HIPCHK(hipGraphLaunch(g->gexec, st));
+            graph_solve(g, nbatch, st);
         } else {
             Rec rec{&g->evs, st, g->prof != 0, g->prof};
-            enqueue_solve(B, rec);
+            enqueue_solve(own_batch(g, 0, nbatch), rec);
             if (g->prof) {
                 if (st != g->stream) HIPCHK(hipStreamSynchronize(st));
                 prof_collect(g);
@@ -1770,6 +1841,7 @@ int gpu_tomo_tune_set(gpu_tomo_t *g, const int *choice) {
         if (choice[i] != 0 && choice[i] != 1 && choice[i] != 2 && choice[i] != 4) return MT_EINVAL;
     if (choice[3] != 0 && choice[3] != TILE && choice[3] != LTY) return MT_EINVAL;
     if (choice[4] < 0 || choice[4] > 2) return MT_EINVAL;
+    if (choice[5] != 0 && choice[5] != 1 && choice[5] != 2 && choice[5] != 4) return MT_EINVAL;
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
         HIPCHK(hipDeviceSynchronize());
@@ -1778,6 +1850,7 @@ int gpu_tomo_tune_set(gpu_tomo_t *g, const int *choice) {
         g->tune_col = choice[2];
         g->tune_lty = choice[3];
         g->tune_colfft = choice[4];
+        g->tune_flights = choice[5];
         drop_batches(g);
         apply_lty(g);
     });
@@ -1877,12 +1950,49 @@ int gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice) {
         }
         drop_batches(g);
         g->prof = save_prof;
+        // flights: whole graph solves of the nsys systems, timed on the
+        // plan's stream (median of 5 after 2 warm-up solves)
+        {
+            const int save_pr = g->prof;
+            g->prof = 0;
+            hipEvent_t e0, e1;
+            HIPCHK(hipEventCreate(&e0));
+            HIPCHK(hipEventCreate(&e1));
+            double best = 1e30;
+            int bestk = 1;
+            for (int k : {1, 2, 4}) {
+                if (k > nsys) break;
+                g->tune_flights = k;
+                std::vector<float> ts;
+                for (int rep = 0; rep < 7; rep++) {
+                    HIPCHK(hipEventRecord(e0, g->stream));
+                    graph_solve(g, nsys, g->stream);
+                    HIPCHK(hipEventRecord(e1, g->stream));
+                    HIPCHK(hipEventSynchronize(e1));
+                    float ms = 0;
+                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                    if (rep >= 2) ts.push_back(ms);
+                }
+                std::sort(ts.begin(), ts.end());
+                const double t = ts[ts.size() / 2];
+                if (t < best * 0.98) {  // prefer fewer flights within 2%
+                    best = t;
+                    bestk = k;
+                }
+            }
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            g->tune_flights = bestk;
+            g->prof = save_pr;
+            drop_batches(g);
+        }
         if (choice) {
             choice[0] = g->tune_wfs;
             choice[1] = g->tune_layer;
             choice[2] = g->tune_col;
             choice[3] = g->G.lty;
             choice[4] = g->tune_colfft;
+            choice[5] = g->tune_flights;
         }
     });
 }

@@ -168,17 +168,18 @@ class GpuTomo:
     def autotune(self, nsys=None):
         """gpu_tomo_autotune: pick the layer tile shape, the column kernel
         and the systems per workgroup of the WFS, layer and column kernels by
-        timing; returns (wfs, layer, col, layer tile rows, column kernel:
-        1 registers, 2 LDS stages)."""
-        ch = (C.c_int * 5)()
+        timing, then the flights of a graph solve; returns (wfs, layer, col,
+        layer tile rows, column kernel: 1 registers, 2 LDS stages, flights)."""
+        ch = (C.c_int * 6)()
         _abi.check(_abi.lib().gpu_tomo_autotune(self._h, int(nsys or self.nbatch), ch), "gpu_tomo_autotune")
         return tuple(ch)
 
-    def tune_set(self, wfs, layer, col, layer_rows=0, col_fft=0):
+    def tune_set(self, wfs, layer, col, layer_rows=0, col_fft=0, flights=0):
         """gpu_tomo_tune_set: fix the systems per workgroup (0: heuristic),
         the layer tile rows (16, 64; 0: 64 where it fits) and the column
-        kernel (1 registers where it applies, 2 LDS stages; 0: as 1)."""
-        ch = (C.c_int * 5)(int(wfs), int(layer), int(col), int(layer_rows), int(col_fft))
+        kernel (1 registers where it applies, 2 LDS stages; 0: as 1) and the
+        flights of a graph solve (1, 2, 4; 0: 1)."""
+        ch = (C.c_int * 6)(int(wfs), int(layer), int(col), int(layer_rows), int(col_fft), int(flights))
         _abi.check(_abi.lib().gpu_tomo_tune_set(self._h, ch), "gpu_tomo_tune_set")
 
     def close(self):

@@ -49,7 +49,9 @@ def test_tune_set_rejects_bad_choice(large30):
     with pytest.raises(_abi.MtError):
         g.tune_set(1, 1, 1, 32)  # nor are 32-row tiles
     with pytest.raises(_abi.MtError):
-        g.tune_set(1, 1, 1, 0, 3)  # nor a third column FFT path
+        g.tune_set(1, 1, 1, 0, 3)  # nor a third column kernel
+    with pytest.raises(_abi.MtError):
+        g.tune_set(1, 1, 1, 0, 0, 3)  # nor 3 flights
     g.close()
 
 
@@ -65,7 +67,7 @@ def test_autotune_replay(large30):
     choice = g.autotune(3)
     print("autotuned", choice, flush=True)
     assert all(c in (1, 2, 4) for c in choice[:2]) and choice[3] in (16, 64) and choice[4] in (1, 2)
-    assert choice[2] in ((0,) if choice[4] == 1 else (1, 2, 4))
+    assert choice[2] in ((0,) if choice[4] == 1 else (1, 2, 4)) and choice[5] in (1, 2)
     x = g.tomo_batch(grads)
     assert np.linalg.norm(x - ref) <= 1e-5 * np.linalg.norm(ref)
     g2 = tomo.GpuTomo(r, nbatch=3, maxit=cfg["maxit"])
@@ -149,4 +151,26 @@ def test_column_fft_paths_parity(cfgname, path):
     print("%s column fft %d: P rel %.3e, solve rel %.3e" % (cfgname, path, ep, e), flush=True)
     assert ep <= 2e-5
     assert e <= (1e-4 if cfgname in ("large30", "small8") else 1e-3)
+    g.close()
+
+
+@pytest.mark.parametrize("flights", [2, 4])
+def test_flights_bit_exact(large30, flights):
+    """A graph solve (gpu_tomo_solve_dev mode 1) of 9 systems as 1 flight and
+    as 2 or 4 concurrent flights (ranges 5 + 4, 3 + 2 + 2 + 2): identical
+    solutions and residuals.  The plan's own buffers (set_grad / get_x)."""
+    cfg, r = large30
+    n = 9
+    grads = _batch(r, cfg, n)
+    g = tomo.GpuTomo(r, nbatch=n, maxit=cfg["maxit"])
+    out = {}
+    for k in (1, flights):
+        for i in range(n):
+            g.set_grad(i, grads[i])
+        g.tune_set(0, 0, 0, 0, 0, k)
+        g.solve_dev(0, 0, nbatch=n, mode=1)
+        out[k] = [g.get_x(i) for i in range(n)]
+    for i in range(n):
+        assert np.array_equal(out[1][i][0], out[flights][i][0]), "system %d differs" % i
+        assert out[1][i][1] == out[flights][i][1]
     g.close()
