@@ -215,18 +215,31 @@ __device__ __forceinline__ void small_dft(C *v) {
         v[1] = cadd(a1, a3);
         v[3] = csub(a1, a3);
     } else {
+        // odd R: pair v_r with v_(R-r); with W^(kr) = c + i s (s carrying
+        // SIGN), W^(kr) v_r + W^(-kr) v_(R-r) = c (v_r + v_(R-r)) + i s (v_r - v_(R-r))
+        static_assert(R % 2 == 1, "odd radix");
         typedef typename CT<C>::R F;
+        constexpr int H = (R - 1) / 2;
+        C a[H], d[H];
+#pragma unroll
+        for (int r = 1; r <= H; r++) {
+            a[r - 1] = cadd(v[r], v[R - r]);
+            d[r - 1] = csub(v[r], v[R - r]);
+        }
         C o[R];
+        o[0] = v[0];
 #pragma unroll
-        for (int k = 0; k < R; k++) {
-            C acc = v[0];
+        for (int r = 0; r < H; r++) o[0] = cadd(o[0], a[r]);
 #pragma unroll
-            for (int r = 1; r < R; r++) {
-                // exp(SIGN * 2 pi i k r / R)
-                const C w = CT<C>::make((F)Roots<R>::c[(k * r) % R], (F)(SIGN * Roots<R>::s[(k * r) % R]));
-                acc = cadd(acc, cmul(v[r], w));
+        for (int k = 1; k < R; k++) {
+            F re = v[0].x, im = v[0].y;
+#pragma unroll
+            for (int r = 1; r <= H; r++) {
+                const F c = (F)Roots<R>::c[(k * r) % R], sn = (F)(SIGN * Roots<R>::s[(k * r) % R]);
+                re += c * a[r - 1].x - sn * d[r - 1].y;
+                im += c * a[r - 1].y + sn * d[r - 1].x;
             }
-            o[k] = acc;
+            o[k] = CT<C>::make(re, im);
         }
 #pragma unroll
         for (int k = 0; k < R; k++) v[k] = o[k];
@@ -309,6 +322,22 @@ template <typename C> __device__ __forceinline__ C shfl_xor_c(C v, int m) {
     return CT<C>::make(__shfl_xor(v.x, m, 64), __shfl_xor(v.y, m, 64));
 }
 template <typename C> __device__ __forceinline__ C conj_c(C v) { return CT<C>::make(v.x, -v.y); }
+// Exchange with lane ^ H for a compile-time H below 32, per 32-bit word:
+// DPP quad permutes for H = 1, 2 (a VALU move), ds_swizzle in bit-mask mode
+// (lane ^ H within each 32 lanes; no address register) for H = 4 .. 16.
+template <int H> __device__ __forceinline__ int xchg_i(int v) {
+    static_assert(H >= 1 && H < 32 && (H & (H - 1)) == 0, "power of two below 32");
+    if constexpr (H == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    else if constexpr (H == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    else return __builtin_amdgcn_ds_swizzle(v, (H << 10) | 0x1F);
+}
+template <int H> __device__ __forceinline__ float xchg(float v) { return __int_as_float(xchg_i<H>(__float_as_int(v))); }
+template <int H> __device__ __forceinline__ double xchg(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = xchg_i<H>((int)(b & 0xffffffffLL)), hi = xchg_i<H>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int H, typename C> __device__ __forceinline__ C xchg_c(C v) { return CT<C>::make(xchg<H>(v.x), xchg<H>(v.y)); }
 // A butterfly is written without a select between its two results: the
 // lane adds s * (own value) to its partner's value (s = +1 on the lower lane
 // of the pair, -1 on the upper, so the pair forms a + b and a - b exactly)
@@ -350,6 +379,64 @@ __device__ __forceinline__ void fft_regs_inv(C *y, int L, int l, const C *tw) {
 #pragma unroll
     for (int k1 = 1; k1 < P; k1++) y[k1] = cmul(y[k1], conj_c(tw[l * k1]));
     if constexpr (P > 1) small_dft<P, 1, C>(y);
+}
+
+// The same transforms with L fixed at compile time: stages unrolled,
+// exchanges by xchg_c (L at most 32).
+template <int P, int L, int H, typename C>
+__device__ __forceinline__ void fwd_stages(C *y, int l, const C *tw) {
+    if constexpr (H >= 1) {
+        typedef typename CT<C>::R F;
+        const bool up = (l & H) != 0;
+        const C wt = tw[(l & (H - 1)) * (L / (2 * H)) * P];
+        const C w = up ? wt : CT<C>::make(1, 0);
+        const F sg = up ? F(-1) : F(1);
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const C o = xchg_c<H>(y[p]);
+            y[p] = cmul(CT<C>::make(o.x + sg * y[p].x, o.y + sg * y[p].y), w);
+        }
+        fwd_stages<P, L, H / 2, C>(y, l, tw);
+    }
+}
+template <int P, int L, int H, typename C>
+__device__ __forceinline__ void inv_stages(C *y, int l, const C *tw) {
+    if constexpr (H < L) {
+        typedef typename CT<C>::R F;
+        const bool up = (l & H) != 0;
+        const C wt = conj_c(tw[(l & (H - 1)) * (L / (2 * H)) * P]);
+        const C w = up ? wt : CT<C>::make(1, 0);
+        const F sg = up ? F(-1) : F(1);
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+            const C t = cmul(y[p], w);
+            const C o = xchg_c<H>(t);
+            y[p] = CT<C>::make(o.x + sg * t.x, o.y + sg * t.y);
+        }
+        inv_stages<P, L, H * 2, C>(y, l, tw);
+    }
+}
+template <int P, int L, typename C>
+__device__ __forceinline__ void fft_fwd_l(C *y, int l, const C *tw) {
+    if constexpr (P > 1) small_dft<P, -1, C>(y);
+#pragma unroll
+    for (int k1 = 1; k1 < P; k1++) y[k1] = cmul(y[k1], tw[l * k1]);
+    fwd_stages<P, L, L / 2, C>(y, l, tw);
+}
+template <int P, int L, typename C>
+__device__ __forceinline__ void fft_inv_l(C *y, int l, const C *tw) {
+    inv_stages<P, L, 1, C>(y, l, tw);
+#pragma unroll
+    for (int k1 = 1; k1 < P; k1++) y[k1] = cmul(y[k1], conj_c(tw[l * k1]));
+    if constexpr (P > 1) small_dft<P, 1, C>(y);
+}
+template <int L, typename T> __device__ __forceinline__ T lane_sum(T v) {
+    if constexpr (L > 1) {
+        v += xchg<L / 2>(v);
+        return lane_sum<L / 2>(v);
+    } else {
+        return v;
+    }
 }
 
 // Copy of the n twiddles into LDS (before the caller's first barrier).
@@ -1398,26 +1485,29 @@ __global__ __launch_bounds__(256) void k_col(KARGS) {
 // block rows are stored per (column, k1, entry) with the L lanes innermost
 // (Mreg), conjugated where the block's frequency is the conjugate of its
 // half-spectrum bin and with the DC columns zeroed; they are loaded at the
-// start, under the spectrum loads and the forward FFT.  Same arithmetic per
-// bin as the LDS kernel: y_m = sum_k M[m + k bs] x_k in the spectrum type,
-// scaled, rounded to FP32 for the inverse; r.z adds w * Re(conj(x_m) y_m)
-// (w = 1 on the kx = 0 and Nyquist columns, else 2; DC skipped), reduced per
-// lane group, then over the layers in order, into part_rz[s][column].
-// Workgroup: T = max(1, 256 / (bs L)) tasks, systems bx * T + t of column
-// by; blockDim.x = T bs L rounded up to 64 (idle lanes repeat the last lane
-// group and write nothing).
+// start, under the spectrum loads and the forward FFT.  Per bin, as the LDS
+// kernel: y_m = sum_k M[m + k bs] x_k in the spectrum type (here with fused
+// multiply-adds), scaled, rounded to FP32 for the inverse; r.z adds
+// w * Re(conj(x_m) y_m) (w = 1 on the kx = 0 and Nyquist columns, else 2;
+// DC skipped), reduced per lane group, then over the layers in order, into
+// part_rz[s][column].  Workgroup: T = max(1, 256 / (bs L)) tasks, systems
+// bx * T + t of column by; blockDim.x = T bs L rounded up to 64 (idle lanes
+// repeat the last lane group and write nothing).  LT: L at compile time
+// (32: unrolled stages, exchanges without LDS addresses), or 0 (runtime);
+// BS: the block size at compile time, or 0 (runtime, at most COLR_MAXBS).
 // ---------------------------------------------------------------------------
 constexpr int COLR_MAXBS = 8;
 __host__ __device__ inline int colr_tasks(int bs, int L) { return bs * L >= 256 ? 1 : 256 / (bs * L); }
 
-template <typename FT, int P>
+template <typename FT, int P, int LT, int BS>
 __global__ __launch_bounds__(512) void k_colr(KARGS) {
     KJOB;
     const Scal SC = ldsc(PD.SC);
     typedef typename CT<FT>::R FR;
+    constexpr int NB = BS ? BS : COLR_MAXBS;
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const FT *specf = (const FT *)PD.specf;
-    const int L = G.shL, ny = G.fny, bs = G.bs;
+    const int L = LT ? LT : G.shL, ny = G.fny, bs = BS ? BS : G.bs;
     const int T = colr_tasks(bs, L), per = bs * L, lb = 31 - __builtin_clz(L);
     const int t = min((int)threadIdx.x, T * per - 1);  // idle lanes: the last lane group again
     const int ti = t / per, q = t - ti * per, m = q >> lb, l = q & (L - 1);
@@ -1430,15 +1520,18 @@ __global__ __launch_bounds__(512) void k_colr(KARGS) {
     FT x[P];
 #pragma unroll
     for (int p = 0; p < P; p++) x[p] = specf[sidx + L * p];
-    // row m of the blocks of this lane's bins
-    const float2 *Mm = G.Mreg + (long)c * P * bs * bs * L + m * L + l;
-    float2 mrow[P][COLR_MAXBS];
+    // row m of the blocks of this lane's bins (a uniform base per entry,
+    // the lane's offset m L + l)
+    const float2 *Mcol = G.Mreg + (long)c * P * bs * bs * L;
+    const unsigned moff = (unsigned)(m * L + l);
+    float2 mrow[P][NB];
 #pragma unroll
     for (int k1 = 0; k1 < P; k1++)
 #pragma unroll
-        for (int k = 0; k < COLR_MAXBS; k++)
-            mrow[k1][k] = Mm[((long)k1 * bs * bs + min(k, bs - 1) * bs) * L];
-    fft_regs_fwd<P>(x, L, l, tw_of<FT>(G, true));
+        for (int k = 0; k < NB; k++)
+            if (k < bs) mrow[k1][k] = (Mcol + (k1 * bs * bs + k * bs) * L)[moff];
+    if constexpr (LT > 0) fft_fwd_l<P, LT>(x, l, tw_of<FT>(G, true));
+    else fft_regs_fwd<P>(x, L, l, tw_of<FT>(G, true));
     // bins of every layer of the task: [task][layer][k1][lane]
     FT *X = reinterpret_cast<FT *>(lds_raw);
     double *dsum = reinterpret_cast<double *>(X + (long)T * per * P);
@@ -1455,14 +1548,19 @@ __global__ __launch_bounds__(512) void k_colr(KARGS) {
     double dot = 0;
 #pragma unroll
     for (int k1 = 0; k1 < P; k1++) {
-        FT y = CT<FT>::make(0, 0);
+        FR yr = 0, yi = 0;
 #pragma unroll
-        for (int k = 0; k < COLR_MAXBS; k++) {
+        for (int k = 0; k < NB; k++) {
             if (k < bs) {
-                const float2 mm = mrow[k1][k];
-                y = cadd(y, cmul(CT<FT>::make(mm.x, mm.y), Xt[(k * P + k1) * L + l]));
+                const FR mr = mrow[k1][k].x, mi = mrow[k1][k].y;
+                const FT v = Xt[(k * P + k1) * L + l];
+                yr = fma(mr, v.x, yr);
+                yr = fma(-mi, v.y, yr);
+                yi = fma(mr, v.y, yi);
+                yi = fma(mi, v.x, yi);
             }
         }
+        const FT y = CT<FT>::make(yr, yi);
         const FT yy = CT<FT>::make(sc * y.x, sc * y.y);
         z[k1] = make_float2((float)yy.x, (float)yy.y);
         if (!(zd && kx == 0 && k1 == 0 && kr == 0))
@@ -1471,13 +1569,18 @@ __global__ __launch_bounds__(512) void k_colr(KARGS) {
     // r.z: over the lane group, then over the layers in order
     if (d32) {
         float f = (float)dot;
-        for (int h = L >> 1; h >= 1; h >>= 1) f += __shfl_xor(f, h, 64);
+        if constexpr (LT > 0) f = lane_sum<LT>(f);
+        else
+            for (int h = L >> 1; h >= 1; h >>= 1) f += __shfl_xor(f, h, 64);
         dot = f;
     } else {
-        for (int h = L >> 1; h >= 1; h >>= 1) dot += __shfl_xor(dot, h, 64);
+        if constexpr (LT > 0) dot = lane_sum<LT>(dot);
+        else
+            for (int h = L >> 1; h >= 1; h >>= 1) dot += __shfl_xor(dot, h, 64);
     }
     if (lane_live && l == 0) dsum[ti * bs + m] = dot;
-    fft_regs_inv<P>(z, L, l, G.twy);
+    if constexpr (LT > 0) fft_inv_l<P, LT>(z, l, G.twy);
+    else fft_regs_inv<P>(z, L, l, G.twy);
     if (live) {
         float2 *dst = PD.speci + sidx;
 #pragma unroll
@@ -1823,14 +1926,17 @@ void launch_col(const LaunchTab &t, int spec64, int cs, int bs, int rm, size_t l
     }
 }
 template <typename FT>
-static void launch_colr_t(const LaunchTab &t, int P, int nthr, size_t lds, hipStream_t st) {
+static void launch_colr_t(const LaunchTab &t, int P, int L, int bs, int nthr, size_t lds, hipStream_t st) {
     const dim3 grid(t.nwg), blk(nthr);
-    switch (P) {
-        case 2: hipLaunchKernelGGL((k_colr<FT, 2>), grid, blk, lds, st, KT); break;
-        case 3: hipLaunchKernelGGL((k_colr<FT, 3>), grid, blk, lds, st, KT); break;
-        case 4: hipLaunchKernelGGL((k_colr<FT, 4>), grid, blk, lds, st, KT); break;
-        case 5: hipLaunchKernelGGL((k_colr<FT, 5>), grid, blk, lds, st, KT); break;
-        default: hipLaunchKernelGGL((k_colr<FT, 7>), grid, blk, lds, st, KT); break;
+    if (L == 32 && P == 3 && bs == 6) hipLaunchKernelGGL((k_colr<FT, 3, 32, 6>), grid, blk, lds, st, KT);
+    else if (L == 32 && P == 3) hipLaunchKernelGGL((k_colr<FT, 3, 32, 0>), grid, blk, lds, st, KT);
+    else if (L == 32 && P == 2) hipLaunchKernelGGL((k_colr<FT, 2, 32, 0>), grid, blk, lds, st, KT);
+    else switch (P) {
+        case 2: hipLaunchKernelGGL((k_colr<FT, 2, 0, 0>), grid, blk, lds, st, KT); break;
+        case 3: hipLaunchKernelGGL((k_colr<FT, 3, 0, 0>), grid, blk, lds, st, KT); break;
+        case 4: hipLaunchKernelGGL((k_colr<FT, 4, 0, 0>), grid, blk, lds, st, KT); break;
+        case 5: hipLaunchKernelGGL((k_colr<FT, 5, 0, 0>), grid, blk, lds, st, KT); break;
+        default: hipLaunchKernelGGL((k_colr<FT, 7, 0, 0>), grid, blk, lds, st, KT); break;
     }
 }
 bool colr_supported(int P, int bs) { return (P == 2 || P == 3 || P == 4 || P == 5 || P == 7) && bs >= 1 && bs <= COLR_MAXBS; }
@@ -1843,9 +1949,9 @@ size_t colr_lds(int bs, int L, int P, int f64) {
     const int T = colr_tasks(bs, L);
     return (size_t)T * bs * L * P * (f64 ? sizeof(double2) : sizeof(float2)) + sizeof(double) * (size_t)T * bs;
 }
-void launch_colr(const LaunchTab &t, int spec64, int P, int nthr, size_t lds, hipStream_t st) {
-    if (spec64) launch_colr_t<double2>(t, P, nthr, lds, st);
-    else launch_colr_t<float2>(t, P, nthr, lds, st);
+void launch_colr(const LaunchTab &t, int spec64, int P, int L, int bs, int nthr, size_t lds, hipStream_t st) {
+    if (spec64) launch_colr_t<double2>(t, P, L, bs, nthr, lds, st);
+    else launch_colr_t<float2>(t, P, L, bs, nthr, lds, st);
 }
 template <int RM>
 static void launch_irow_r(const LaunchTab &t, int mode, int slot, int osel, size_t lds, hipStream_t st) {

@@ -34,11 +34,12 @@ void col_allow_lds(int spec64, int cs, int bs, int rm, size_t lds);
 // across the layers): a task is one (system, column) on bs x L threads,
 // P = ny / L points per lane.  colr_supported: an instantiation exists for P
 // and the block size.  colr_shape: threads per workgroup (tasks per
-// workgroup in *tasks); colr_lds: its dynamic LDS bytes.
+// workgroup in *tasks); colr_lds: its dynamic LDS bytes.  Every job of a
+// launch has the same P, L (shL), block size and threads per workgroup.
 bool colr_supported(int P, int bs);
 int colr_shape(int bs, int L, int *tasks);
 size_t colr_lds(int bs, int L, int P, int f64);
-void launch_colr(const LaunchTab &t, int spec64, int P, int nthr, size_t lds, hipStream_t st);
+void launch_colr(const LaunchTab &t, int spec64, int P, int L, int bs, int nthr, size_t lds, hipStream_t st);
 void launch_irow(const LaunchTab &t, int mode, int rm, int slot, int osel, size_t lds, hipStream_t st);
 void launch_final(const LaunchTab &t, hipStream_t st);
 void launch_zero_stop(const LaunchTab &t, hipStream_t st);

@@ -97,6 +97,7 @@ struct Sub {
     int matched = 0, pos = 1, cs = 1, bs = 0, rm = 3;
     int reg = 0;  // column stage: P of the register kernel (0: the LDS kernel)
     int nthr = 0; // its threads per workgroup
+    int regL = 0; // its lanes per column FFT (shL of every job)
     size_t lds = 0;
     DevBuf<WGI> wgi;
     int nwg = 0;
@@ -1222,7 +1223,7 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
         build_stage(b, b.col,
                     [&](int j, std::vector<int> &k) {
                         if (colreg(j)) {
-                            k = {-1, colreg(j), colr_shape(G(j).bs, G(j).shL, nullptr)};
+                            k = {-1, colreg(j), colr_shape(G(j).bs, G(j).shL, nullptr), G(j).shL, G(j).bs};
                             return true;
                         }
                         const int bs = G(j).bs;
@@ -1238,6 +1239,8 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
                         if (k[0] == -1) {
                             sb.reg = k[1];
                             sb.nthr = k[2];
+                            sb.regL = k[3];
+                            sb.bs = k[4];
                             for (int j : js)
                                 sb.lds = std::max(sb.lds, colr_lds(G(j).bs, G(j).shL, sb.reg, b.spec64));
                             return;
@@ -1329,7 +1332,7 @@ void enqueue_row(Batch &B, int kc, int mode, int iter, int xsel, Rec &rec) {
 void enqueue_P_tail(Batch &B, int irow_mode, int slot, int osel, Rec &rec) {
     rec(KC_COL, [&] {
         for (auto &s : B.col) {
-            if (s->reg) launch_colr(tab(B, *s), B.spec64, s->reg, s->nthr, s->lds, rec.st);
+            if (s->reg) launch_colr(tab(B, *s), B.spec64, s->reg, s->regL, s->bs, s->nthr, s->lds, rec.st);
             else launch_col(tab(B, *s), B.spec64, s->cs, s->bs, s->rm, s->lds, rec.st);
         }
     });
