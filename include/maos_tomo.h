@@ -341,7 +341,9 @@ int  gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice);
  * timing: choice[6] = systems per workgroup of the WFS, layer and LDS
  * column kernels (each 1, 2 or 4, or 0 for the built-in heuristic), the
  * layer tile rows (16, 64, or 0 for 64 where it fits), the column kernel
- * (1 registers where it applies, 2 LDS stages, 0 as 1) and the flights of a
+ * (1 registers where it applies, 2 LDS stages, 0 as 1; 2 also keeps the row
+ * FFTs in LDS stages, which otherwise run in registers for single-plan
+ * batches with layers of 64 or 96 points) and the flights of a
  * graph solve (1, 2, 4, or 0 for 1).  Values above a
  * kernel's maximum (WFS 2 when tomo.pos = 2, column 2 in precision modes 1
  * and 2) are clamped when the launch is built.  Call it before the plan

@@ -1122,11 +1122,13 @@ __global__ __launch_bounds__(256) void k_layer_m4(KARGS, int psel_, int osel, in
 
 // ---------------------------------------------------------------------------
 // Row kernel: PCG vector update fused with the forward real-to-complex row
-// FFT of r.  Two real rows are packed into one complex FFT.
+// FFT of r.  Two real rows are packed into one complex FFT.  RP = 0: the FFT
+// in LDS stages; RP = 2 or 3 (rows of n = 32 RP): in registers, 32 lanes per
+// row pair (lane l: columns l + 32 p), written to LDS in bin order.
 // ---------------------------------------------------------------------------
 enum { ROW_INIT_COLD = 0, ROW_INIT_WARM = 1, ROW_UPDATE = 2, ROW_LAST = 3 };
 
-template <int MODE, typename RT, typename FT, int RM>
+template <int MODE, typename RT, typename FT, int RM, int RP>
 __global__ __launch_bounds__(256) void k_row(KARGS, int iter, int xsel) {
     KJOB;
     const Scal SC = ldsc(PD.SC);
@@ -1147,9 +1149,29 @@ __global__ __launch_bounds__(256) void k_row(KARGS, int iter, int xsel) {
     const int nrows = min(ROWS, nyl - row0);
     const int nf = nrows / 2;
     FT *twl = lds + ROWS * n;
-    if (MODE != ROW_LAST) load_tw(twl, tw_of<FT>(G, false), n);
+    if (MODE != ROW_LAST && RP == 0) load_tw(twl, tw_of<FT>(G, false), n);
     const long base = (long)s * G.nxtot + G.xoff[l] + (long)row0 * n;
     const int tot = nrows * n;
+    // register path: lane group f takes rows 2f, 2f + 1 (groups past the
+    // last pair repeat it and store nothing), lane lr columns lr + 32 p
+    constexpr int RPL = RP > 0 ? RP : 1;
+    const int fg = threadIdx.x >> 5, lr = threadIdx.x & 31;
+    const int fc = min(fg, nf - 1);
+    RT w0[2][RPL], w1[2][RPL];
+    if constexpr (RP > 0) {
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+#pragma unroll
+            for (int q = 0; q < RP; q++) {
+                const long idx = base + (long)(2 * fc + h) * n + lr + 32 * q;
+                if (MODE == ROW_INIT_COLD) {
+                    w0[h][q] = b[idx];
+                } else {
+                    w0[h][q] = MODE == ROW_INIT_WARM ? b[idx] : r[idx];
+                    w1[h][q] = Ap[idx];
+                }
+            }
+    }
     // First chunk of the vector loads, issued before alpha is reduced (they
     // do not depend on it); further chunks (rows longer than 96) in the loop.
     constexpr int U = 6;
@@ -1177,7 +1199,7 @@ __global__ __launch_bounds__(256) void k_row(KARGS, int iter, int xsel) {
             }
         }
     };
-    load(0);
+    if constexpr (RP == 0) load(0);
     double alpha = 0.;
     if (MODE >= ROW_UPDATE) {
         const double pap = reduce_partials(SC.part_pap + (long)s * SC.maxpart, SC.npart_layer, sh, G.dot32);
@@ -1197,6 +1219,30 @@ __global__ __launch_bounds__(256) void k_row(KARGS, int iter, int xsel) {
     if (MODE == ROW_UPDATE && bx == 0 && threadIdx.x == 0) SC.alpha[s] = alpha;
     FR *bufA = reinterpret_cast<FR *>(lds);
     double dot = 0;
+    if constexpr (RP > 0 && MODE != ROW_LAST) {
+        FT zr[RP];
+#pragma unroll
+        for (int q = 0; q < RP; q++) {
+            RT rv[2];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const long idx = base + (long)(2 * fc + h) * n + lr + 32 * q;
+                if (MODE == ROW_INIT_COLD) rv[h] = w0[h][q];
+                else if (MODE == ROW_INIT_WARM) rv[h] = w0[h][q] - w1[h][q];
+                else rv[h] = w0[h][q] - (RT)alpha * w1[h][q];
+                if (fg < nf) {
+                    r[idx] = rv[h];
+                    if (MODE == ROW_INIT_COLD) x[idx] = 0.f;
+                }
+            }
+            zr[q] = CT<FT>::make((FR)rv[0], (FR)rv[1]);
+        }
+        fft_fwd_l<RP, 32>(zr, lr, tw_of<FT>(G, false));
+        const int kr = (int)(__builtin_bitreverse32((unsigned)lr) >> 27);
+        if (fg < nf)
+#pragma unroll
+            for (int q = 0; q < RP; q++) lds[fc * n + q + RP * kr] = zr[q];
+    } else {
     for (int c0 = 0; c0 < tot; c0 += 256 * U) {
         if (c0 > 0) load(c0);
 #pragma unroll
@@ -1221,6 +1267,7 @@ __global__ __launch_bounds__(256) void k_row(KARGS, int iter, int xsel) {
             }
         }
     }
+    }
     if (MODE == ROW_LAST) {
         dot = block_sum(dot, sh, G.dot32);
         if (threadIdx.x == 0) SC.part_rr[(long)s * SC.maxpart + bx] = dot;
@@ -1228,7 +1275,7 @@ __global__ __launch_bounds__(256) void k_row(KARGS, int iter, int xsel) {
     }
     __syncthreads();
     FT *A = lds, *B = lds + ROWS / 2 * n;
-    FT *res = fft_run<-1, FT, RM>(A, B, n, nf, G.nstx, G.fsx, twl);
+    FT *res = RP > 0 ? A : fft_run<-1, FT, RM>(A, B, n, nf, G.nstx, G.fsx, twl);
     const int nh = n / 2 + 1;
     FT *S = spec + (long)s * G.nspec + G.soff[l];
     const FDiv dnf = ldf(G.dsmall[nf]);
@@ -1615,7 +1662,7 @@ __global__ __launch_bounds__(512) void k_colr(KARGS) {
 // ---------------------------------------------------------------------------
 enum { IROW_Z = 0, IROW_FIRST = 1, IROW_UPDATE = 2 };
 
-template <int MODE, int RM>
+template <int MODE, int RM, int RP>
 __global__ __launch_bounds__(256) void k_irow(KARGS, int slot, int osel) {
     KJOB;
     const Scal SC = ldsc(PD.SC);
@@ -1634,7 +1681,7 @@ __global__ __launch_bounds__(256) void k_irow(KARGS, int slot, int osel) {
     const float2 *S = spec + (long)s * G.nspec + G.soff[l];
     float2 *A = lds, *B = lds + ROWS / 2 * n;
     float2 *twl = lds + ROWS * n;
-    load_tw(twl, G.twx, n);
+    if (RP == 0) load_tw(twl, G.twx, n);
     // Loads first (spectrum, and x, p for the update), then the r.z
     // reduction while they are in flight.
     const int tots = nf * n;
@@ -1688,7 +1735,25 @@ __global__ __launch_bounds__(256) void k_irow(KARGS, int slot, int osel) {
         }
     };
     load_spec(0);
-    load_xp(0);
+    // register path (RP > 0): lane group f takes rows 2f, 2f + 1, lane lr
+    // columns lr + 32 p; its x and p are loaded here
+    constexpr int RPL = RP > 0 ? RP : 1;
+    const int fg = threadIdx.x >> 5, lr = threadIdx.x & 31;
+    const int fc = min(fg, nf - 1);
+    float px[2][RPL], xx[2][RPL];
+    if constexpr (RP > 0) {
+        if (MODE == IROW_UPDATE)
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+#pragma unroll
+                for (int q = 0; q < RP; q++) {
+                    const long idx = base + (long)(2 * fc + h) * n + lr + 32 * q;
+                    px[h][q] = out[idx];
+                    xx[h][q] = x[idx];
+                }
+    } else {
+        load_xp(0);
+    }
     double beta = 0, alpha = 0;
     if (MODE != IROW_Z) {
         const double rz = reduce_partials(SC.part_rz + (long)s * SC.maxpart, SC.npart_col, sh, G.dot32);
@@ -1705,9 +1770,31 @@ __global__ __launch_bounds__(256) void k_irow(KARGS, int slot, int osel) {
         store_spec(c0);
     }
     __syncthreads();
-    float2 *res = fft_run<1, float2, RM>(A, B, n, nf, G.nstx, G.fsx, twl);
     const float sc = G.scale ? G.fscale : 1.f;
     const float fa = (float)alpha, fb = (float)beta;
+    if constexpr (RP > 0) {
+        const int kr = (int)(__builtin_bitreverse32((unsigned)lr) >> 27);
+        float2 y[RP];
+#pragma unroll
+        for (int q = 0; q < RP; q++) y[q] = A[fc * n + q + RP * kr];
+        fft_inv_l<RP, 32>(y, lr, G.twx);
+        if (fg < nf)
+#pragma unroll
+            for (int q = 0; q < RP; q++)
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const long idx = base + (long)(2 * fc + h) * n + lr + 32 * q;
+                    const float zv = (h ? y[q].y : y[q].x) * sc;
+                    if (MODE == IROW_UPDATE) {
+                        x[idx] = xx[h][q] + fa * px[h][q];
+                        out[idx] = zv + fb * px[h][q];
+                    } else {
+                        out[idx] = zv;
+                    }
+                }
+        return;
+    }
+    float2 *res = fft_run<1, float2, RM>(A, B, n, nf, G.nstx, G.fsx, twl);
     for (int c0 = 0; c0 < tot; c0 += 256 * U) {
         if (c0 > 0) load_xp(c0);
 #pragma unroll
@@ -1835,32 +1922,35 @@ void launch_layer(const LaunchTab &t, bool lhs, int matched, int cs, int rt64, i
 }
 // LDS of the row kernels: ROWS/2 complex rows twice (ping-pong) + twiddles.
 size_t row_lds(int fnx, int f64) { return (f64 ? sizeof(double2) : sizeof(float2)) * (ROWS + 1) * (size_t)fnx; }
-template <typename RT, typename FT, int RM>
+template <typename RT, typename FT, int RM, int RP>
 static void launch_row_r(const LaunchTab &t, int mode, int iter, int xsel, size_t lds, hipStream_t st) {
     const dim3 grid(t.nwg), blk(256);
     switch (mode) {
         case ROW_INIT_COLD:
-            hipLaunchKernelGGL((k_row<ROW_INIT_COLD, RT, FT, RM>), grid, blk, lds, st, KT, iter, xsel);
+            hipLaunchKernelGGL((k_row<ROW_INIT_COLD, RT, FT, RM, RP>), grid, blk, lds, st, KT, iter, xsel);
             break;
         case ROW_INIT_WARM:
-            hipLaunchKernelGGL((k_row<ROW_INIT_WARM, RT, FT, RM>), grid, blk, lds, st, KT, iter, xsel);
+            hipLaunchKernelGGL((k_row<ROW_INIT_WARM, RT, FT, RM, RP>), grid, blk, lds, st, KT, iter, xsel);
             break;
         case ROW_UPDATE:
-            hipLaunchKernelGGL((k_row<ROW_UPDATE, RT, FT, RM>), grid, blk, lds, st, KT, iter, xsel);
+            hipLaunchKernelGGL((k_row<ROW_UPDATE, RT, FT, RM, RP>), grid, blk, lds, st, KT, iter, xsel);
             break;
-        default: hipLaunchKernelGGL((k_row<ROW_LAST, RT, FT, RM>), grid, blk, lds, st, KT, iter, xsel); break;
+        default: hipLaunchKernelGGL((k_row<ROW_LAST, RT, FT, RM, 0>), grid, blk, lds, st, KT, iter, xsel); break;
     }
 }
 template <typename RT, typename FT>
-static void launch_row_t(const LaunchTab &t, int mode, int rm, int iter, int xsel, size_t lds, hipStream_t st) {
-    if (rm > 4) launch_row_r<RT, FT, 7>(t, mode, iter, xsel, lds, st);
-    else launch_row_r<RT, FT, 3>(t, mode, iter, xsel, lds, st);
+static void launch_row_t(const LaunchTab &t, int mode, int rm, int rp, int iter, int xsel, size_t lds,
+                         hipStream_t st) {
+    if (rm > 4) launch_row_r<RT, FT, 7, 0>(t, mode, iter, xsel, lds, st);
+    else if (rp == 3) launch_row_r<RT, FT, 3, 3>(t, mode, iter, xsel, lds, st);
+    else if (rp == 2) launch_row_r<RT, FT, 3, 2>(t, mode, iter, xsel, lds, st);
+    else launch_row_r<RT, FT, 3, 0>(t, mode, iter, xsel, lds, st);
 }
-void launch_row(const LaunchTab &t, int mode, int rt64, int spec64, int rm, int iter, int xsel, size_t lds,
+void launch_row(const LaunchTab &t, int mode, int rt64, int spec64, int rm, int rp, int iter, int xsel, size_t lds,
                 hipStream_t st) {
-    if (rt64 && spec64) launch_row_t<double, double2>(t, mode, rm, iter, xsel, lds, st);
-    else if (rt64) launch_row_t<double, float2>(t, mode, rm, iter, xsel, lds, st);
-    else launch_row_t<float, float2>(t, mode, rm, iter, xsel, lds, st);
+    if (rt64 && spec64) launch_row_t<double, double2>(t, mode, rm, rp, iter, xsel, lds, st);
+    else if (rt64) launch_row_t<double, float2>(t, mode, rm, rp, iter, xsel, lds, st);
+    else launch_row_t<float, float2>(t, mode, rm, rp, iter, xsel, lds, st);
 }
 // LDS of the column kernel: two buffers of cs systems' column groups + the
 // forward and inverse twiddles.
@@ -1953,18 +2043,20 @@ void launch_colr(const LaunchTab &t, int spec64, int P, int L, int bs, int nthr,
     if (spec64) launch_colr_t<double2>(t, P, L, bs, nthr, lds, st);
     else launch_colr_t<float2>(t, P, L, bs, nthr, lds, st);
 }
-template <int RM>
+template <int RM, int RP>
 static void launch_irow_r(const LaunchTab &t, int mode, int slot, int osel, size_t lds, hipStream_t st) {
     const dim3 grid(t.nwg), blk(256);
     switch (mode) {
-        case IROW_Z: hipLaunchKernelGGL((k_irow<IROW_Z, RM>), grid, blk, lds, st, KT, slot, osel); break;
-        case IROW_FIRST: hipLaunchKernelGGL((k_irow<IROW_FIRST, RM>), grid, blk, lds, st, KT, slot, osel); break;
-        default: hipLaunchKernelGGL((k_irow<IROW_UPDATE, RM>), grid, blk, lds, st, KT, slot, osel); break;
+        case IROW_Z: hipLaunchKernelGGL((k_irow<IROW_Z, RM, RP>), grid, blk, lds, st, KT, slot, osel); break;
+        case IROW_FIRST: hipLaunchKernelGGL((k_irow<IROW_FIRST, RM, RP>), grid, blk, lds, st, KT, slot, osel); break;
+        default: hipLaunchKernelGGL((k_irow<IROW_UPDATE, RM, RP>), grid, blk, lds, st, KT, slot, osel); break;
     }
 }
-void launch_irow(const LaunchTab &t, int mode, int rm, int slot, int osel, size_t lds, hipStream_t st) {
-    if (rm > 4) launch_irow_r<7>(t, mode, slot, osel, lds, st);
-    else launch_irow_r<3>(t, mode, slot, osel, lds, st);
+void launch_irow(const LaunchTab &t, int mode, int rm, int rp, int slot, int osel, size_t lds, hipStream_t st) {
+    if (rm > 4) launch_irow_r<7, 0>(t, mode, slot, osel, lds, st);
+    else if (rp == 3) launch_irow_r<3, 3>(t, mode, slot, osel, lds, st);
+    else if (rp == 2) launch_irow_r<3, 2>(t, mode, slot, osel, lds, st);
+    else launch_irow_r<3, 0>(t, mode, slot, osel, lds, st);
 }
 void launch_final(const LaunchTab &t, hipStream_t st) {
     hipLaunchKernelGGL(k_final, dim3(t.nwg), dim3(256), 0, st, KT);

@@ -23,7 +23,9 @@ void launch_layer(const LaunchTab &t, bool lhs, int matched, int cs, int rt64, i
                   hipStream_t st);
 size_t row_lds(int fnx, int f64);
 // spec64: the forward row/column spectrum is double2 (requires rt64).
-void launch_row(const LaunchTab &t, int mode, int rt64, int spec64, int rm, int iter, int xsel, size_t lds,
+// rp: 2 or 3 for the row FFT in registers (rows of 32 rp points, radices up
+// to 4), else 0 (LDS stages).
+void launch_row(const LaunchTab &t, int mode, int rt64, int spec64, int rm, int rp, int iter, int xsel, size_t lds,
                 hipStream_t st);
 size_t col_lds(int maxcol, int npsr, int fny, int f64, int cs);
 void launch_col(const LaunchTab &t, int spec64, int cs, int bs, int rm, size_t lds, hipStream_t st);
@@ -40,7 +42,7 @@ bool colr_supported(int P, int bs);
 int colr_shape(int bs, int L, int *tasks);
 size_t colr_lds(int bs, int L, int P, int f64);
 void launch_colr(const LaunchTab &t, int spec64, int P, int L, int bs, int nthr, size_t lds, hipStream_t st);
-void launch_irow(const LaunchTab &t, int mode, int rm, int slot, int osel, size_t lds, hipStream_t st);
+void launch_irow(const LaunchTab &t, int mode, int rm, int rp, int slot, int osel, size_t lds, hipStream_t st);
 void launch_final(const LaunchTab &t, hipStream_t st);
 void launch_zero_stop(const LaunchTab &t, hipStream_t st);
 void launch_copy(const float *a, float *b, long n, hipStream_t st);

@@ -96,6 +96,7 @@ namespace {
 struct Sub {
     int matched = 0, pos = 1, cs = 1, bs = 0, rm = 3;
     int reg = 0;  // column stage: P of the register kernel (0: the LDS kernel)
+    int rp = 0;   // row stages: P of the register row FFT (rows of 32 P points), or 0
     int nthr = 0; // its threads per workgroup
     int regL = 0; // its lanes per column FFT (shL of every job)
     size_t lds = 0;
@@ -1156,6 +1157,17 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
         bs_all = bs_all < 0 || bs_all == bk ? bk : 0;
     }
     auto rm_key = [&](int j) { return merge ? rm_all : G(j).rmax > 4 ? 7 : 3; };
+    // row FFTs in registers for rows of 64 or 96 points (32 lanes x 2 or 3),
+    // where every layer is fnx wide and a whole number of 16-row blocks tall
+    // (each workgroup then has its 8 row pairs), unless the plan asks for
+    // LDS stages; a merged batch keeps one launch per row stage (LDS stages)
+    auto rp_key = [&](int j) {
+        const Geom &g = G(j);
+        if (merge || b.plan[j]->tune_colfft == 2 || rm_key(j) != 3 || (g.fnx != 96 && g.fnx != 64)) return 0;
+        for (int l = 0; l < g.npsr; l++)
+            if (g.xnx[l] != g.fnx || g.xny[l] % ROWS != 0) return 0;
+        return g.fnx / 32;
+    };
     // per-stage tables
     build_stage(b, b.zero, [&](int, std::vector<int> &) { return true; },
                 [&](const Sub &, int) { return JobGrid{1, 1, 1}; }, [&](Sub &, const std::vector<int> &, const std::vector<int> &) {});
@@ -1200,16 +1212,18 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
                                                   4, 512);
                 });
     if (b.precond) {
-        build_stage(b, b.row, [&](int j, std::vector<int> &k) { k = {rm_key(j)}; return true; },
+        build_stage(b, b.row, [&](int j, std::vector<int> &k) { k = {rm_key(j), rp_key(j)}; return true; },
                     [&](const Sub &, int j) { return JobGrid{G(j).nrowblk, 1, NS(j)}; },
                     [&](Sub &sb, const std::vector<int> &k, const std::vector<int> &js) {
                         sb.rm = k[0];
+                        sb.rp = k[1];
                         for (int j : js) sb.lds = std::max(sb.lds, row_lds(G(j).fnx, b.spec64));
                     });
-        build_stage(b, b.irow, [&](int j, std::vector<int> &k) { k = {rm_key(j)}; return true; },
+        build_stage(b, b.irow, [&](int j, std::vector<int> &k) { k = {rm_key(j), rp_key(j)}; return true; },
                     [&](const Sub &, int j) { return JobGrid{G(j).nrowblk, 1, NS(j)}; },
                     [&](Sub &sb, const std::vector<int> &k, const std::vector<int> &js) {
                         sb.rm = k[0];
+                        sb.rp = k[1];
                         for (int j : js) sb.lds = std::max(sb.lds, row_lds(G(j).fnx, 0));
                     });
         // register kernel where the plan allows it and the tuning does not
@@ -1324,7 +1338,8 @@ void enqueue_R(Batch &B, Rec &rec) {
 }
 void enqueue_row(Batch &B, int kc, int mode, int iter, int xsel, Rec &rec) {
     rec(kc, [&] {
-        for (auto &s : B.row) launch_row(tab(B, *s), mode, B.rt64, B.spec64, s->rm, iter, xsel, s->lds, rec.st);
+        for (auto &s : B.row)
+            launch_row(tab(B, *s), mode, B.rt64, B.spec64, s->rm, s->rp, iter, xsel, s->lds, rec.st);
     });
 }
 // Preconditioner on the row-transformed residual (specf): column pass with
@@ -1337,7 +1352,7 @@ void enqueue_P_tail(Batch &B, int irow_mode, int slot, int osel, Rec &rec) {
         }
     });
     rec(irow_mode == IROW_MODE_UPDATE ? KC_IROW : KC_IROW_START, [&] {
-        for (auto &s : B.irow) launch_irow(tab(B, *s), irow_mode, s->rm, slot, osel, s->lds, rec.st);
+        for (auto &s : B.irow) launch_irow(tab(B, *s), irow_mode, s->rm, s->rp, slot, osel, s->lds, rec.st);
     });
 }
 

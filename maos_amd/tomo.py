@@ -176,8 +176,9 @@ class GpuTomo:
 
     def tune_set(self, wfs, layer, col, layer_rows=0, col_fft=0, flights=0):
         """gpu_tomo_tune_set: fix the systems per workgroup (0: heuristic),
-        the layer tile rows (16, 64; 0: 64 where it fits) and the column
-        kernel (1 registers where it applies, 2 LDS stages; 0: as 1) and the
+        the layer tile rows (16, 64; 0: 64 where it fits), the FFT kernels
+        (1 registers where they apply, 2 LDS stages for columns and rows;
+        0: as 1) and the
         flights of a graph solve (1, 2, 4; 0: 1)."""
         ch = (C.c_int * 6)(int(wfs), int(layer), int(col), int(layer_rows), int(col_fft), int(flights))
         _abi.check(_abi.lib().gpu_tomo_tune_set(self._h, ch), "gpu_tomo_tune_set")
