@@ -95,6 +95,8 @@ def _pmc_rows(d, counter):
 
 def _kernel_class(name):
     base = name.split("(")[0].split("<")[0].replace("void ", "").strip().split("::")[-1]
+    if base == "k_colr":  # the column stage in registers
+        return "k_col"
     for k in sorted(KERNEL_CLASSES, key=len, reverse=True):
         if base == k or base.startswith(k + "_"):
             return k
@@ -278,6 +280,9 @@ def bench_uneven(steps, check=3):
         cfg = cfgs[idx[0]]
         r = tomo.Recon(cfg)
         g = tomo.GpuTomo(r, nbatch=len(idx), maxit=cfg["maxit"])
+        # FFTs in LDS stages: one launch per FFT stage for all 47 plans (the
+        # register kernels would add a launch per point count; launch-bound here)
+        g.tune_set(0, 0, 0, 0, 2)
         plans[name] = (cfg, r, g)
         members.append((g, 0, len(idx)))
         for s, i in enumerate(idx):

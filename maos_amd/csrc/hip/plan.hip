@@ -1160,14 +1160,17 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
     // row FFTs in registers for rows of 64 or 96 points (32 lanes x 2 or 3),
     // where every layer is fnx wide and a whole number of 16-row blocks tall
     // (each workgroup then has its 8 row pairs), unless the plan asks for
-    // LDS stages; a merged batch keeps one launch per row stage (LDS stages)
+    // LDS stages.  A property of the plan, not of the batch, so that a system
+    // computes the same bits alone and in any mix; a merged batch issues one
+    // launch per register length beside its shared LDS-stage launch.
     auto rp_key = [&](int j) {
         const Geom &g = G(j);
-        if (merge || b.plan[j]->tune_colfft == 2 || rm_key(j) != 3 || (g.fnx != 96 && g.fnx != 64)) return 0;
+        if (b.plan[j]->tune_colfft == 2 || g.rmax > 4 || (g.fnx != 96 && g.fnx != 64)) return 0;
         for (int l = 0; l < g.npsr; l++)
             if (g.xnx[l] != g.fnx || g.xny[l] % ROWS != 0) return 0;
         return g.fnx / 32;
     };
+    auto row_key = [&](int j) { return rp_key(j) ? std::vector<int>{3, rp_key(j)} : std::vector<int>{rm_key(j), 0}; };
     // per-stage tables
     build_stage(b, b.zero, [&](int, std::vector<int> &) { return true; },
                 [&](const Sub &, int) { return JobGrid{1, 1, 1}; }, [&](Sub &, const std::vector<int> &, const std::vector<int> &) {});
@@ -1212,14 +1215,14 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
                                                   4, 512);
                 });
     if (b.precond) {
-        build_stage(b, b.row, [&](int j, std::vector<int> &k) { k = {rm_key(j), rp_key(j)}; return true; },
+        build_stage(b, b.row, [&](int j, std::vector<int> &k) { k = row_key(j); return true; },
                     [&](const Sub &, int j) { return JobGrid{G(j).nrowblk, 1, NS(j)}; },
                     [&](Sub &sb, const std::vector<int> &k, const std::vector<int> &js) {
                         sb.rm = k[0];
                         sb.rp = k[1];
                         for (int j : js) sb.lds = std::max(sb.lds, row_lds(G(j).fnx, b.spec64));
                     });
-        build_stage(b, b.irow, [&](int j, std::vector<int> &k) { k = {rm_key(j), rp_key(j)}; return true; },
+        build_stage(b, b.irow, [&](int j, std::vector<int> &k) { k = row_key(j); return true; },
                     [&](const Sub &, int j) { return JobGrid{G(j).nrowblk, 1, NS(j)}; },
                     [&](Sub &sb, const std::vector<int> &k, const std::vector<int> &js) {
                         sb.rm = k[0];
@@ -1227,8 +1230,18 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
                         for (int j : js) sb.lds = std::max(sb.lds, row_lds(G(j).fnx, 0));
                     });
         // register kernel where the plan allows it and the tuning does not
-        // ask for the LDS kernel; its jobs share a launch per (P, threads)
+        // ask for the LDS kernel (a property of the plan, as for the rows).
+        // One plan: a launch per (P, lanes, block size), specialised where an
+        // instantiation exists.  Several plans: one launch per P, the
+        // runtime lanes / block size variant (the same operations in the
+        // same order, so the same bits) with the largest workgroup.
         auto colreg = [&](int j) { return b.plan[j]->colr_ok && b.plan[j]->tune_colfft != 2 ? G(j).shP : 0; };
+        std::map<int, int> nthr_of_p;  // merged: largest workgroup per P
+        for (size_t j = 0; j < b.hjob.size(); j++)
+            if (colreg((int)j)) {
+                int &v = nthr_of_p[colreg((int)j)];
+                v = std::max(v, colr_shape(G((int)j).bs, G((int)j).shL, nullptr));
+            }
         auto colr_tasks_of = [&](int j) {
             int T = 1;
             colr_shape(G(j).bs, G(j).shL, &T);
@@ -1237,7 +1250,8 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
         build_stage(b, b.col,
                     [&](int j, std::vector<int> &k) {
                         if (colreg(j)) {
-                            k = {-1, colreg(j), colr_shape(G(j).bs, G(j).shL, nullptr), G(j).shL, G(j).bs};
+                            if (merge) k = {-1, colreg(j), nthr_of_p[colreg(j)], 0, 0};
+                            else k = {-1, colreg(j), colr_shape(G(j).bs, G(j).shL, nullptr), G(j).shL, G(j).bs};
                             return true;
                         }
                         const int bs = G(j).bs;
@@ -2372,6 +2386,10 @@ int gpu_tomo_server_add_loop(gpu_tomo_server_t *S, const mt_recon_t *recon, int 
     gpu_tomo_t *g = nullptr;
     int rc = gpu_tomo_setup(recon, S->device, 1, maxit, 1, &g);
     if (rc != MT_OK) return rc;
+    // the server always solves its loops in batches over several plans:
+    // FFTs in LDS stages, so each FFT stage stays one launch for all loops
+    // (a loop's path is the same alone and in any mix, so its bits are too)
+    g->tune_colfft = 2;
     const size_t bytes = plan_bytes(g);
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->used + bytes > S->budget) {

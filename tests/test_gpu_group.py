@@ -73,3 +73,31 @@ def test_group_vs_oracle(plans):
         assert e <= TOL[k]
         assert abs(res - reso) <= 0.02 * reso
     grp.close()
+
+
+def test_group_register_kernels_equal_alone():
+    """large30 alone takes the column kernel specialised for 32 lanes and
+    6 layers and the register row FFTs; in a group with other plans its
+    column stage runs in the shared runtime-shape variant of the register
+    kernel (one launch per point count), beside an LDS-stage plan (small8)
+    and another register plan (28 = 4 x 7 points).  Same bits as alone."""
+    members = [configs.large30, configs.small8, lambda: configs.batch_member(8.0, 2, 4)]
+    plans = []
+    for k, mk in enumerate(members):
+        cfg = mk()
+        r = tomo.Recon(cfg)
+        g = tomo.GpuTomo(r, nbatch=1, maxit=30)
+        grad = workload.make_inputs(cfg, r, 300 + k)
+        plans.append((cfg, g, grad, g.tomo(grad)[0]))
+        print("member", k, cfg["name"], "info", g.info(), flush=True)
+    for cfg, g, grad, x1 in plans:
+        g.set_grad(0, grad)
+    grp = tomo.GpuGroup([(g, 0, 1) for cfg, g, grad, x1 in plans])
+    for mode in (0, 1):
+        grp.solve(mode=mode)
+        for k, (cfg, g, grad, x1) in enumerate(plans):
+            x, res = g.get_x(0)
+            assert np.array_equal(x, x1), "member %s (mode %d)" % (cfg["name"], mode)
+    grp.close()
+    for cfg, g, grad, x1 in plans:
+        g.close()
