@@ -1051,7 +1051,7 @@ __global__ __launch_bounds__(256) void k_layer_m4(KARGS, int psel_, int osel, in
     // gx = 1 - fx, gy = 1 - fy.  Five sample rows serve the four nodes.
     if (nr > 0) {
         const int pnx = G.pnx;
-        const unsigned nrec = (unsigned)G.nploc * (unsigned)sizeof(RT);
+        const unsigned rstep = (unsigned)pnx * (unsigned)sizeof(RT);
         for (int w = 0; w < G.nwfs; w++) {
             const MRay m = mray(G, w * MAXL + l);
             const bool edgex = m.wx && jx0 == 0, edgey = m.wy && jy0 == 0;
@@ -1063,20 +1063,30 @@ __global__ __launch_bounds__(256) void k_layer_m4(KARGS, int psel_, int osel, in
             const int ix0 = (m.wx && jx == 0) ? m.ax1 - 1 : jx - m.cx - 1, ix1 = jx - m.cx;
             const bool v0 = ix0 >= m.ax0 && ix0 < m.ax1, v1 = ix1 >= m.ax0 && ix1 < m.ax1;
             // sample rows k = 0..R: iy = jyb - cy - 1 + k; row 0 serves node
-            // jyb only (its first ray, the last ray when jyb = 0 wraps)
+            // jyb only (its first ray, the last ray when jyb = 0 wraps).  The
+            // buffer of this WFS starts at the first valid row ay0 and holds
+            // the valid rows [ay0, ay1) only, so a sample row outside them
+            // falls outside the buffer (rows above wrap to offsets past its
+            // end) and reads 0 with no test per row; a column outside its
+            // interval starts from BOOB, past the end for every row.
+            const unsigned c0 = v0 ? (unsigned)ix0 * (unsigned)sizeof(RT) : BOOB;
+            const unsigned c1 = v1 ? (unsigned)ix1 * (unsigned)sizeof(RT) : BOOB;
+            const unsigned rw = (m.wy && jyb == 0) ? (unsigned)(m.ay1 - 1 - m.ay0) * rstep
+                                                   : (unsigned)(jyb - m.cy - 1 - m.ay0) * rstep;
+            const unsigned r1 = (unsigned)(jyb - m.cy - m.ay0) * rstep;
             unsigned o0[R + 1], o1[R + 1];
+            o0[0] = c0 + rw;
+            o1[0] = c1 + rw;
 #pragma unroll
-            for (int k = 0; k <= R; k++) {
-                const int iy = (k == 0 && m.wy && jyb == 0) ? m.ay1 - 1 : jyb - m.cy - 1 + k;
-                const bool vy = iy >= m.ay0 && iy < m.ay1;
-                const unsigned rb = (unsigned)(iy * pnx);
-                o0[k] = vy && v0 ? (rb + (unsigned)ix0) * (unsigned)sizeof(RT) : BOOB;
-                o1[k] = vy && v1 ? (rb + (unsigned)ix1) * (unsigned)sizeof(RT) : BOOB;
+            for (int k = 1; k <= R; k++) {
+                o0[k] = c0 + r1 + (unsigned)(k - 1) * rstep;
+                o1[k] = c1 + r1 + (unsigned)(k - 1) * rstep;
             }
+            const unsigned nrec = (unsigned)max(m.ay1 - m.ay0, 0) * rstep;
             // systems past ns read system ns-1 (in range; never stored)
 #pragma unroll
             for (int si = 0; si < CS; si++) {
-                const auto rs = brsrc(q + ((long)(s0 + min(si, ns - 1)) * G.nwfs + w) * G.nploc, nrec);
+                const auto rs = brsrc(q + ((long)(s0 + min(si, ns - 1)) * G.nwfs + w) * G.nploc + (long)m.ay0 * pnx, nrec);
                 RT h[R + 1];
 #pragma unroll
                 for (int k = 0; k <= R; k++) h[k] = gx * bload<RT>(rs, o1[k]) + fx * bload<RT>(rs, o0[k]);
