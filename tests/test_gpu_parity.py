@@ -23,10 +23,17 @@ EXTRA = {
     "b8_w2_l4": lambda: configs.batch_member(8.0, 2, 4),    # FFT 28 (radix 7), bs 4
     "b10_w3_l2": lambda: configs.batch_member(10.0, 3, 2),  # bs 2
     "b8_w5_l5": lambda: configs.batch_member(8.0, 5, 5),    # bs 5 (runtime block size)
+    # register FFT shapes: 64 = 32 x 2 (rows and columns in registers, bs 3),
+    # 40 = 8 x 5 (columns on 8 lanes, bs 6), 96 with 3 layers (32 lanes,
+    # runtime block size)
+    "b20_w3_l3": lambda: configs.batch_member(20.0, 3, 3),
+    "b10_w4_l6": lambda: configs.batch_member(10.0, 4, 6),
+    "b30_w2_l3": lambda: configs.batch_member(30.0, 2, 3),
 }
 
 
-@pytest.fixture(scope="module", params=["small8", "large30", "lgs8_flat", "b8_w2_l4", "b10_w3_l2", "b8_w5_l5"])
+@pytest.fixture(scope="module", params=["small8", "large30", "lgs8_flat", "b8_w2_l4", "b10_w3_l2", "b8_w5_l5",
+                                        "b20_w3_l3", "b10_w4_l6", "b30_w2_l3"])
 def sys_(request):
     if request.param in EXTRA:
         cfg = EXTRA[request.param]()
