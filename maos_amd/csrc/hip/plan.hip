@@ -1697,12 +1697,17 @@ void graph_solve(gpu_tomo *g, int n, hipStream_t st) {
             f.graph = nullptr;
         }
         g->fl_nsys = g->fl_k = 0;
-        for (int i = 0, s0 = 0; i < k; i++) {
-            const int ni = n / k + (i < n % k);
+        // the flights' streams first, back to back (no other stream created
+        // or destroyed between them, as the captures below do), so that the
+        // runtime's least-used choice gives them different hardware queues
+        for (int i = 0; i < k; i++) {
             auto &f = g->fl[i];
             if (!f.st) HIPCHK(hipStreamCreateWithFlags(&f.st, hipStreamNonBlocking));
             if (!f.done) HIPCHK(hipEventCreateWithFlags(&f.done, hipEventDisableTiming));
-            capture_solve(own_batch(g, s0, ni), &f.graph, &f.exec, &g->evs);
+        }
+        for (int i = 0, s0 = 0; i < k; i++) {
+            const int ni = n / k + (i < n % k);
+            capture_solve(own_batch(g, s0, ni), &g->fl[i].graph, &g->fl[i].exec, &g->evs);
             s0 += ni;
         }
         g->fl_nsys = n;
