@@ -982,21 +982,24 @@ __global__ __launch_bounds__(256) void k_layer_m(KARGS, int psel_, int osel, int
     }
 }
 
-// Layer pass of the matched path on 16 x 64 tiles: each thread owns four
-// vertically adjacent nodes of one column, so the five sample rows of a WFS
-// field serve four nodes (the horizontal interpolation of a row is shared by
-// the two nodes it falls between).  Same arithmetic per node as k_layer_m.
+// Layer pass of the matched path on 16 x 64 tiles with 128 threads: each
+// thread owns two adjacent columns of four vertically adjacent nodes, so the
+// five sample rows of a WFS field, read at three ray columns (the middle one
+// serves both columns), feed eight nodes.  Same arithmetic per node as
+// k_layer_m (the reference's bilinear transpose):
+//   acc += fy (gx q[iy0, ix1] + fx q[iy0, ix0]) + gy (gx q[iy1, ix1] + fx q[iy1, ix0]),
+// gx = 1 - fx, gy = 1 - fy; the p.Ap partial of a thread sums its nodes.
 template <bool LHS, typename RT, int CS>
-__global__ __launch_bounds__(256) void k_layer_m4(KARGS, int psel_, int osel, int partsel) {
+__global__ __launch_bounds__(128) void k_layer_m8(KARGS, int psel_, int osel, int partsel) {
     KJOB;
     const RT *q = (const RT *)PD.q;
     const float *p = vsel(PD, psel_);
     RT *out = rsel<RT>(PD, osel);
     double *part = psel(PD, partsel);
     const int partstride = PD.SC.maxpart;
-    constexpr int R = 4;
+    constexpr int R = 4, NT = 128;
     constexpr int PW = TILE + 4, PHH = LTY + 4, P2 = PW * PHH;  // p window (L2'L2 reach 2)
-    constexpr int UP = (P2 + 255) / 256;
+    constexpr int UP = (P2 + NT - 1) / NT;
     __shared__ float ps[CS][LHS ? P2 : 1];
     __shared__ double sh[8];
     const int l = by, s0 = J.s0 + bz * CS, ns = min(CS, J.s0 + J.nsys - s0);
@@ -1004,15 +1007,16 @@ __global__ __launch_bounds__(256) void k_layer_m4(KARGS, int psel_, int osel, in
     const int ntx = (nxl + TILE - 1) / TILE, nty = (nyl + LTY - 1) / LTY;
     const bool active = bx < ntx * nty;
     const int jx0 = (bx % ntx) * TILE, jy0 = (bx / ntx) * LTY;
-    const int tx = threadIdx.x % TILE, ty = threadIdx.x / TILE;
-    const int jx = jx0 + tx, jyb = jy0 + R * ty;
-    const int nr = active && jx < nxl ? max(0, min(R, nyl - jyb)) : 0;  // rows of this thread
+    const int tx = threadIdx.x % (TILE / 2), ty = threadIdx.x / (TILE / 2);
+    const int jx = jx0 + 2 * tx, jyb = jy0 + R * ty;
+    const int nr = active ? max(0, min(R, nyl - jyb)) : 0;  // rows of this thread
+    const int nc = active ? max(0, min(2, nxl - jx)) : 0;   // columns of this thread
     if (LHS && active) {
         // p windows, periodic (layers are >= LTY + 2 rows and TILE + 2 columns)
         int off[UP];
 #pragma unroll
         for (int u = 0; u < UP; u++) {
-            const int i = min((int)threadIdx.x + 256 * u, P2 - 1);
+            const int i = min((int)threadIdx.x + NT * u, P2 - 1);
             int xx = jx0 - 2 + i % PW, yy = jy0 - 2 + i / PW;
             xx += xx < 0 ? nxl : 0;
             xx -= xx >= nxl ? nxl : 0;
@@ -1020,9 +1024,6 @@ __global__ __launch_bounds__(256) void k_layer_m4(KARGS, int psel_, int osel, in
             yy -= yy >= nyl ? nyl : 0;
             off[u] = yy * nxl + xx;
         }
-        // unconditional loads (systems past ns read system ns-1, entries
-        // past the window the last entry, stored again with its own value):
-        // loads under a branch are each waited for before the branch closes
         float pv[CS][UP];
 #pragma unroll
         for (int si = 0; si < CS; si++) {
@@ -1033,23 +1034,25 @@ __global__ __launch_bounds__(256) void k_layer_m4(KARGS, int psel_, int osel, in
 #pragma unroll
         for (int si = 0; si < CS; si++)
 #pragma unroll
-            for (int u = 0; u < UP; u++) ps[si][min((int)threadIdx.x + 256 * u, P2 - 1)] = pv[si][u];
+            for (int u = 0; u < UP; u++) ps[si][min((int)threadIdx.x + NT * u, P2 - 1)] = pv[si][u];
     }
-    RT acc[CS][R];
+    RT acc[CS][2][R];
 #pragma unroll
     for (int si = 0; si < CS; si++)
 #pragma unroll
-        for (int r = 0; r < R; r++) acc[si][r] = 0;
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int r = 0; r < R; r++) acc[si][c][r] = 0;
     const int tw = min(TILE, nxl - jx0), th = min(LTY, nyl - jy0);
-    // One body per WFS covers every case: each of the four rays of a node
-    // (two per axis) is loaded through a buffer offset, and a ray outside the
-    // interval gets the out-of-range offset, which reads 0; the wrapped edge
-    // ray (the last ray's second node is node 0) is a per-lane select of the
-    // first ray's index for node 0.  Per node the arithmetic is that of the
-    // reference's bilinear transpose:
-    //   acc += fy (gx q[iy0, ix1] + fx q[iy0, ix0]) + gy (gx q[iy1, ix1] + fx q[iy1, ix0]),
-    // gx = 1 - fx, gy = 1 - fy.  Five sample rows serve the four nodes.
-    if (nr > 0) {
+    // Node jx takes rays ix0 = jx - cx - 1 (the last ray for node 0 on a
+    // wrapped axis) and ix1 = jx - cx; node jx + 1 takes ix1 and ix1 + 1.
+    // Sample rows k = 0..R: iy = jyb - cy - 1 + k (row 0: the last ray when
+    // jyb = 0 wraps).  The buffer of this WFS starts at the first valid row
+    // ay0 and holds the valid rows [ay0, ay1) only, so a sample row outside
+    // them falls outside it (rows above wrap to offsets past its end) and
+    // reads 0 with no test per row; an invalid ray column starts from BOOB,
+    // past the end for every row.
+    if (nr > 0 && nc > 0) {
         const int pnx = G.pnx;
         const unsigned rstep = (unsigned)pnx * (unsigned)sizeof(RT);
         for (int w = 0; w < G.nwfs; w++) {
@@ -1058,40 +1061,36 @@ __global__ __launch_bounds__(256) void k_layer_m4(KARGS, int psel_, int osel, in
             if (!edgex && (jx0 + tw - 1 - m.cx < m.ax0 || jx0 - m.cx - 1 >= m.ax1)) continue;
             if (!edgey && (jy0 + th - 1 - m.cy < m.ay0 || jy0 - m.cy - 1 >= m.ay1)) continue;
             const RT fx = m.fx, fy = m.fy, gx = RT(1) - fx, gy = RT(1) - fy;
-            // columns: node jx takes ray ix0 = jx - cx - 1 (weight fx; the last
-            // ray for node 0 on a wrapped axis) and ix1 = jx - cx (weight gx)
-            const int ix0 = (m.wx && jx == 0) ? m.ax1 - 1 : jx - m.cx - 1, ix1 = jx - m.cx;
-            const bool v0 = ix0 >= m.ax0 && ix0 < m.ax1, v1 = ix1 >= m.ax0 && ix1 < m.ax1;
-            // sample rows k = 0..R: iy = jyb - cy - 1 + k; row 0 serves node
-            // jyb only (its first ray, the last ray when jyb = 0 wraps).  The
-            // buffer of this WFS starts at the first valid row ay0 and holds
-            // the valid rows [ay0, ay1) only, so a sample row outside them
-            // falls outside the buffer (rows above wrap to offsets past its
-            // end) and reads 0 with no test per row; a column outside its
-            // interval starts from BOOB, past the end for every row.
-            const unsigned c0 = v0 ? (unsigned)ix0 * (unsigned)sizeof(RT) : BOOB;
-            const unsigned c1 = v1 ? (unsigned)ix1 * (unsigned)sizeof(RT) : BOOB;
+            const int ixa = (m.wx && jx == 0) ? m.ax1 - 1 : jx - m.cx - 1, ixb = jx - m.cx, ixc = ixb + 1;
+            const bool va = ixa >= m.ax0 && ixa < m.ax1, vb = ixb >= m.ax0 && ixb < m.ax1,
+                       vc = ixc >= m.ax0 && ixc < m.ax1;
+            const unsigned ca = va ? (unsigned)ixa * (unsigned)sizeof(RT) : BOOB;
+            const unsigned cb = vb ? (unsigned)ixb * (unsigned)sizeof(RT) : BOOB;
+            const unsigned cc = vc ? (unsigned)ixc * (unsigned)sizeof(RT) : BOOB;
             const unsigned rw = (m.wy && jyb == 0) ? (unsigned)(m.ay1 - 1 - m.ay0) * rstep
                                                    : (unsigned)(jyb - m.cy - 1 - m.ay0) * rstep;
             const unsigned r1 = (unsigned)(jyb - m.cy - m.ay0) * rstep;
-            unsigned o0[R + 1], o1[R + 1];
-            o0[0] = c0 + rw;
-            o1[0] = c1 + rw;
+            unsigned ro[R + 1];
+            ro[0] = rw;
 #pragma unroll
-            for (int k = 1; k <= R; k++) {
-                o0[k] = c0 + r1 + (unsigned)(k - 1) * rstep;
-                o1[k] = c1 + r1 + (unsigned)(k - 1) * rstep;
-            }
+            for (int k = 1; k <= R; k++) ro[k] = r1 + (unsigned)(k - 1) * rstep;
             const unsigned nrec = (unsigned)max(m.ay1 - m.ay0, 0) * rstep;
-            // systems past ns read system ns-1 (in range; never stored)
 #pragma unroll
             for (int si = 0; si < CS; si++) {
                 const auto rs = brsrc(q + ((long)(s0 + min(si, ns - 1)) * G.nwfs + w) * G.nploc + (long)m.ay0 * pnx, nrec);
-                RT h[R + 1];
+                RT h0[R + 1], h1[R + 1];
 #pragma unroll
-                for (int k = 0; k <= R; k++) h[k] = gx * bload<RT>(rs, o1[k]) + fx * bload<RT>(rs, o0[k]);
+                for (int k = 0; k <= R; k++) {
+                    const RT qa = bload<RT>(rs, ca + ro[k]), qb = bload<RT>(rs, cb + ro[k]),
+                             qc = bload<RT>(rs, cc + ro[k]);
+                    h0[k] = gx * qb + fx * qa;
+                    h1[k] = gx * qc + fx * qb;
+                }
 #pragma unroll
-                for (int r = 0; r < R; r++) acc[si][r] += fy * h[r] + gy * h[r + 1];
+                for (int r = 0; r < R; r++) {
+                    acc[si][0][r] += fy * h0[r] + gy * h0[r + 1];
+                    acc[si][1][r] += fy * h1[r] + gy * h1[r + 1];
+                }
             }
         }
     }
@@ -1102,23 +1101,27 @@ __global__ __launch_bounds__(256) void k_layer_m4(KARGS, int psel_, int osel, in
         dot[si] = 0;
         if (si < ns) {
 #pragma unroll
-            for (int r = 0; r < R; r++) {
-                if (r >= nr) continue;
-                const int jy = jyb + r;
-                RT a = acc[si][r];
-                if (LHS) {
-                    const float *P0 = ps[si] + (R * ty + r + 2) * PW + tx + 2;
-                    auto P = [&](int dx, int dy) { return (RT)P0[dy * PW + dx]; };
-                    const RT c = P(0, 0);
-                    const RT s4 = P(-1, 0) + P(1, 0) + P(0, -1) + P(0, 1);
-                    const RT sd = P(-1, -1) + P(1, -1) + P(-1, 1) + P(1, 1);
-                    const RT s2 = P(-2, 0) + P(2, 0) + P(0, -2) + P(0, 2);
-                    a += (RT)G.l2c[l] * (RT(20) * c - RT(8) * s4 + RT(2) * sd + s2);
-                    dot[si] = dfma(c, a, dot[si], G.dot32);
-                } else {
-                    dot[si] = dfma(a, a, dot[si], G.dot32);
+            for (int c = 0; c < 2; c++) {
+                if (c >= nc) continue;
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    if (r >= nr) continue;
+                    const int jy = jyb + r;
+                    RT a = acc[si][c][r];
+                    if (LHS) {
+                        const float *P0 = ps[si] + (R * ty + r + 2) * PW + 2 * tx + c + 2;
+                        auto P = [&](int dx, int dy) { return (RT)P0[dy * PW + dx]; };
+                        const RT cv = P(0, 0);
+                        const RT s4 = P(-1, 0) + P(1, 0) + P(0, -1) + P(0, 1);
+                        const RT sd = P(-1, -1) + P(1, -1) + P(-1, 1) + P(1, 1);
+                        const RT s2 = P(-2, 0) + P(2, 0) + P(0, -2) + P(0, 2);
+                        a += (RT)G.l2c[l] * (RT(20) * cv - RT(8) * s4 + RT(2) * sd + s2);
+                        dot[si] = dfma(cv, a, dot[si], G.dot32);
+                    } else {
+                        dot[si] = dfma(a, a, dot[si], G.dot32);
+                    }
+                    out[(long)(s0 + si) * G.nxtot + G.xoff[l] + jy * nxl + jx + c] = a;
                 }
-                out[(long)(s0 + si) * G.nxtot + G.xoff[l] + jy * nxl + jx] = a;
             }
         }
     }
@@ -1912,9 +1915,10 @@ static void launch_layer_rt(const LaunchTab &t, int matched, int cs, int psel_, 
     const dim3 grid(t.nwg), blk(256);
     if (!matched) hipLaunchKernelGGL((k_layer<LHS, RT>), grid, blk, 0, st, KT, psel_, osel, partsel);
     else if (matched == 2) {  // 16 x 64 tiles
-        if (cs == 4) hipLaunchKernelGGL((k_layer_m4<LHS, RT, 4>), grid, blk, 0, st, KT, psel_, osel, partsel);
-        else if (cs == 2) hipLaunchKernelGGL((k_layer_m4<LHS, RT, 2>), grid, blk, 0, st, KT, psel_, osel, partsel);
-        else hipLaunchKernelGGL((k_layer_m4<LHS, RT, 1>), grid, blk, 0, st, KT, psel_, osel, partsel);
+        const dim3 b8(128);
+        if (cs == 4) hipLaunchKernelGGL((k_layer_m8<LHS, RT, 4>), grid, b8, 0, st, KT, psel_, osel, partsel);
+        else if (cs == 2) hipLaunchKernelGGL((k_layer_m8<LHS, RT, 2>), grid, b8, 0, st, KT, psel_, osel, partsel);
+        else hipLaunchKernelGGL((k_layer_m8<LHS, RT, 1>), grid, b8, 0, st, KT, psel_, osel, partsel);
     }
     else if (cs == 4) hipLaunchKernelGGL((k_layer_m<LHS, RT, 4>), grid, blk, 0, st, KT, psel_, osel, partsel);
     else if (cs == 2) hipLaunchKernelGGL((k_layer_m<LHS, RT, 2>), grid, blk, 0, st, KT, psel_, osel, partsel);
