@@ -370,6 +370,40 @@ int  gpu_tomo_profile(gpu_tomo_t *g, int enable);
 int  gpu_tomo_profile_read(gpu_tomo_t *g, const char **names, double *ms,
                            double *calls, int *n);
 
+/* ------------------------------------------------------------------ */
+/* MAOS .bin files                                                     */
+/* ------------------------------------------------------------------ */
+/* Double arrays and cells of them in MAOS's .bin layout (sys/bin.c:513-811,
+ * math/cell.c:227-270; .bin.gz through zlib), so that gradients saved by MAOS
+ * (gradlastcl, the input of gpu_tomo) and solutions (opdr) pass through
+ * unchanged.  A node is a double matrix (nx rows, ny columns, column-major)
+ * or a cell of nodes (an empty element is NULL); each may carry the header
+ * text MAOS stores before it (key=value lines).  Other data types are
+ * refused (MT_EINVAL).  No reference counterpart as a C API: MAOS does this
+ * I/O itself with readbin / writebin (math/cell.c). */
+typedef struct mt_bin mt_bin_t;
+
+int  mt_bin_read(const char *path, mt_bin_t **out);       /* .bin or .bin.gz */
+int  mt_bin_write(const char *path, const mt_bin_t *b);   /* gzip if the path ends in .gz */
+void mt_bin_free(mt_bin_t *b);
+int  mt_bin_info(const mt_bin_t *b, int *is_cell, long *nx, long *ny);
+const double   *mt_bin_data(const mt_bin_t *b);            /* NULL for a cell */
+const mt_bin_t *mt_bin_child(const mt_bin_t *b, long i);   /* NULL: empty or not a cell */
+const char     *mt_bin_header(const mt_bin_t *b);
+int  mt_bin_new_dbl(long nx, long ny, const double *data, const char *header, mt_bin_t **out);
+int  mt_bin_new_cell(long nx, long ny, const char *header, mt_bin_t **out);
+/* Put child at element i of cell (column-major); the cell takes ownership
+ * (and frees any previous element). */
+int  mt_bin_set(mt_bin_t *cell, long i, mt_bin_t *child);
+
+/* The solve's vectors in MAOS's layouts: gradients from an nwfs x 1 cell of
+ * 2*nsa x 1 arrays, x slopes then y slopes (gradlastcl,
+ * maos/recon_utils.c:699; the elements of skipped WFS are ignored) into the
+ * gradient vector of gpu_tomo_solve; and a solution x into an npsr x 1 cell
+ * of xnx x xny arrays, as the GPU path holds opdr (cuda/recon.cu:72). */
+int  mt_bin_grad(const mt_bin_t *b, const mt_recon_t *r, double *grad);
+int  mt_bin_opdr(const mt_recon_t *r, const double *x, mt_bin_t **out);
+
 #ifdef __cplusplus
 }
 #endif

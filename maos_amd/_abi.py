@@ -130,6 +130,22 @@ def lib():
     L.gpu_tomo_precision.argtypes = [vp, C.c_int]
     L.gpu_tomo_profile.argtypes = [vp, C.c_int]
     L.gpu_tomo_profile_read.argtypes = [vp, C.POINTER(C.c_char_p), dp, dp, ip]
+    L.mt_bin_read.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.mt_bin_write.argtypes = [C.c_char_p, vp]
+    L.mt_bin_free.argtypes = [vp]
+    L.mt_bin_free.restype = None
+    L.mt_bin_info.argtypes = [vp, ip, lp, lp]
+    L.mt_bin_data.argtypes = [vp]
+    L.mt_bin_data.restype = dp
+    L.mt_bin_child.argtypes = [vp, C.c_long]
+    L.mt_bin_child.restype = vp
+    L.mt_bin_header.argtypes = [vp]
+    L.mt_bin_header.restype = C.c_char_p
+    L.mt_bin_new_dbl.argtypes = [C.c_long, C.c_long, dp, C.c_char_p, C.POINTER(vp)]
+    L.mt_bin_new_cell.argtypes = [C.c_long, C.c_long, C.c_char_p, C.POINTER(vp)]
+    L.mt_bin_set.argtypes = [vp, C.c_long, vp]
+    L.mt_bin_grad.argtypes = [vp, C.POINTER(MtRecon), dp]
+    L.mt_bin_opdr.argtypes = [C.POINTER(MtRecon), dp, C.POINTER(vp)]
     _lib = L
     return L
 
@@ -146,6 +162,8 @@ EXPORTS = [
     "gpu_tomo_server_submit", "gpu_tomo_server_poll", "gpu_tomo_server_drain", "gpu_tomo_server_get_x",
     "gpu_tomo_server_usage", "gpu_tomo_server_free", "gpu_tomo_autotune", "gpu_tomo_tune_set", "gpu_tomo_info", "gpu_tomo_residuals", "gpu_tomo_L", "gpu_tomo_R", "gpu_tomo_P",
     "gpu_tomo_sizes", "gpu_tomo_profile", "gpu_tomo_profile_read", "gpu_tomo_precision",
+    "mt_bin_read", "mt_bin_write", "mt_bin_free", "mt_bin_info", "mt_bin_data", "mt_bin_child",
+    "mt_bin_header", "mt_bin_new_dbl", "mt_bin_new_cell", "mt_bin_set", "mt_bin_grad", "mt_bin_opdr",
 ]
 
 

@@ -17,7 +17,7 @@ PKG = os.path.join(ROOT, "maos_amd")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(BUILD, "libmaos_tomo.so")
 
-HOST_SRC = ["csrc/host/setup.cpp", "csrc/host/geom.cpp", "csrc/host/fdpcg_setup.cpp"]
+HOST_SRC = ["csrc/host/setup.cpp", "csrc/host/geom.cpp", "csrc/host/fdpcg_setup.cpp", "csrc/host/binio.cpp"]
 HIP_SRC = ["csrc/hip/kernels.hip", "csrc/hip/plan.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled for all device code (wrong results
@@ -52,7 +52,7 @@ def build_lib(verbose=False):
             if verbose and out.strip():
                 print(out)
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lz"])
     os.replace(tmp, LIB)
     return LIB
 

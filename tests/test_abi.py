@@ -10,7 +10,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def test_header_symbols_exported():
     hdr = open(os.path.join(ROOT, "include", "maos_tomo.h")).read()
-    decl = set(re.findall(r"^\s*(?:const\s+char\s*\*\s*|int\s+|void\s+)(\w+)\s*\(", hdr, re.M))
+    decl = set(re.findall(r"^\s*(?:const\s+\w+\s*\*\s*|int\s+|void\s+)(\w+)\s*\(", hdr, re.M))
     assert decl == set(_abi.EXPORTS)
     lib = ctypes.CDLL(_abi.LIB_PATH)
     for name in decl:

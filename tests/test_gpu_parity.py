@@ -198,3 +198,27 @@ def test_precision_modes(large30, mode, tol):
     e = rel(xg, xo)
     print("precision mode %d: rel L2 %.3e residual gpu %.5e oracle %.5e" % (mode, e, resg, reso))
     assert e <= tol
+
+
+def test_bin_files_end_to_end(tmp_path):
+    """MAOS's file flow at the boundary: gradients saved as MAOS saves
+    gradlastcl (a cell of one 2 nsa x 1 array per WFS, .bin), read into the
+    solve, the solution written as opdr (.bin.gz) and read back: the same
+    bits as the solve from memory."""
+    from maos_amd import binio
+    cfg = configs.small8()
+    r = tomo.Recon(cfg)
+    grad = workload.make_inputs(cfg, r, 4)
+    offs = r.grad_offsets()
+    binio.write(tmp_path / "gradlastcl_1.bin", [grad[offs[w]:offs[w + 1]] for w in range(r.nwfs)],
+                "Created by the test")
+    g = tomo.GpuTomo(r, maxit=cfg["maxit"])
+    x_mem = g.tomo(grad)[0]
+    x_file = g.tomo(binio.read_grad(tmp_path / "gradlastcl_1.bin", r))[0]
+    assert np.array_equal(x_file, x_mem)
+    binio.write_opdr(tmp_path / "opdr_1.bin.gz", r, x_file)
+    back = binio.read(tmp_path / "opdr_1.bin.gz")
+    off = r.layer_offsets()
+    for l in range(r.npsr):
+        assert np.array_equal(back[l, 0].reshape(-1, order="F"), x_mem[off[l]:off[l + 1]])
+    g.close()
