@@ -12,12 +12,47 @@
 // pupil frequencies only through the few separations kept in sel.  The sums,
 // scalings, drop tolerances, symmetrisation and thresholds are the reference's.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <exception>
+#include <mutex>
+#include <thread>
 
 #include "setup_internal.hpp"
 
 namespace mt {
+
+namespace {
+// f(i) for i in [0, n), indices independent, on up to 16 host threads (the
+// host share of one GPU); the first exception is rethrown after all join.
+// Each index computes what it computed serially, so results are identical.
+template <typename F>
+void parallel_for(long n, F f) {
+    const unsigned nt = std::max(1u, std::min(std::thread::hardware_concurrency(), 16u));
+    if (nt == 1 || n < 2) {
+        for (long i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::atomic<long> next(0);
+    std::exception_ptr err;
+    std::mutex mu;
+    auto work = [&] {
+        try {
+            for (long i; (i = next.fetch_add(1)) < n;) f(i);
+        } catch (...) {
+            std::lock_guard<std::mutex> lk(mu);
+            if (!err) err = std::current_exception();
+            next = n;
+        }
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < std::min<long>(nt, n); t++) th.emplace_back(work);
+    work();
+    for (auto &t : th) t.join();
+    if (err) std::rethrow_exception(err);
+}
+}  // namespace
 
 void fftshift2(std::vector<cplx> &a, long nx, long ny) {
     // math/mat.c:389-420 (even sizes): swap quadrants.
@@ -144,6 +179,12 @@ void herm_pinv(const cplx *A, cplx *out, int n, double thres) {
 }
 
 FdpcgOut fdpcg_prepare(const FdpcgIn &in) {
+    FdpcgOut out = fdpcg_geometry(in);
+    fdpcg_finish(out, in);
+    return out;
+}
+
+FdpcgOut fdpcg_geometry(const FdpcgIn &in) {
     const int nps = in.nps;
     const long *nx = in.nx.data(), *ny = in.ny.data(), *os = in.os.data();
     const long pos = in.pos;
@@ -248,53 +289,17 @@ FdpcgOut fdpcg_prepare(const FdpcgIn &in) {
             }
         }
     }
+    // S by separation: index into sdiff, or -1
+    std::vector<int> sat(nxp * nyp, -1);
+    for (size_t i = 0; i < sdiff.size(); i++) sat[sdiff[i].x + sdiff[i].y * nxp] = (int)i;
     auto mmid = [&](long p, long q) -> cplx {
         // Mmid(p,q) = sum_g conj(g_p) S(p-q) g_q, pupil indices in shifted order
         long ddx = ((p % nxp - q % nxp) % nxp + nxp) % nxp;
         long ddy = ((p / nxp - q / nxp) % nyp + nyp) % nyp;
-        for (const SS &s : sdiff) {
-            if (s.x == ddx && s.y == ddy)
-                return s.v * (std::conj(gx[p]) * gx[q] + std::conj(gy[p]) * gy[q]);
-        }
-        return 0.;
+        const int i = sat[ddx + ddy * nxp];
+        if (i < 0) return 0.;
+        return sdiff[i].v * (std::conj(gx[p]) * gx[q] + std::conj(gy[p]) * gy[q]);
     };
-
-    // --- invpsd (maos/fdpcg.c:373-407) ---------------------------------------
-    std::vector<double> invpsd(nxtot);
-    for (int ips = 0; ips < nps; ips++) {
-        const long n1 = nx[ips], n2 = ny[ips];
-        // first column of L2'L2 for the periodic 5-point L2 of
-        // lib/laplacian.c:70-97, scaled by sqrt(cxxscale*TOMOSCALE)
-        const double c = in.l2cf[ips];
-        std::vector<double> l2e(n1 * n2, 0.);  // L2 * e0
-        auto idx = [&](long ix, long iy) { return ((ix % n1 + n1) % n1) + ((iy % n2 + n2) % n2) * n1; };
-        // row r of L2 has -c at r and 0.25c at its 4 periodic neighbours;
-        // L2 is symmetric so column 0 equals row 0.
-        l2e[idx(0, 0)] += -c;
-        l2e[idx(-1, 0)] += 0.25 * c;
-        l2e[idx(1, 0)] += 0.25 * c;
-        l2e[idx(0, -1)] += 0.25 * c;
-        l2e[idx(0, 1)] += 0.25 * c;
-        std::vector<cplx> psd(n1 * n2, cplx(0, 0));
-        for (long iy = 0; iy < n2; iy++)
-            for (long ix = 0; ix < n1; ix++) {
-                const double v = l2e[idx(ix, iy)];
-                if (v == 0) continue;
-                // L2' applied: column of L2 at (ix,iy) spreads to its stencil
-                psd[idx(ix, iy)] += -c * v;
-                psd[idx(ix - 1, iy)] += 0.25 * c * v;
-                psd[idx(ix + 1, iy)] += 0.25 * c * v;
-                psd[idx(ix, iy - 1)] += 0.25 * c * v;
-                psd[idx(ix, iy + 1)] += 0.25 * c * v;
-            }
-        dft2(psd, n1, n2, -1);
-        fftshift2(psd, n1, n2);
-        const double eps = 2.220446049250313e-16;
-        double mx = 0;
-        for (const cplx &v : psd) mx = std::max(mx, std::abs(v));
-        mx *= std::sqrt(eps);
-        for (long i = 0; i < n1 * n2; i++) invpsd[noff[ips] + i] = psd[i].real() + mx;
-    }
 
     // --- block membership (fdpcg_perm, maos/fdpcg.c:118-193) ----------------
     int bs = 0;
@@ -308,6 +313,8 @@ FdpcgOut fdpcg_prepare(const FdpcgIn &in) {
     out.nb = nb;
     out.nbhalf = nbhalf;
     out.scale = needscale;
+    out.diag_at.resize(nb * bs);
+    out.lsize.resize(nb * bs);
     out.perm.resize(nb * bs);
     out.permhf.resize(nbhalf * bs);
     struct Member {
@@ -360,16 +367,16 @@ FdpcgOut fdpcg_prepare(const FdpcgIn &in) {
     const double dkx = 1. / (nxp * dxp), dky = 1. / (nyp * dxp);
     const long nxp2 = nxp / 2, nyp2 = nyp / 2;
     const int nwfs = (int)in.thetax.size();
-    out.Mhat.assign(nb * bs * bs * 2, 0.);
-    std::vector<cplx> blk(bs * bs);
+    out.MhatW.assign(nb * bs * bs * 2, 0.);
     // pupil frequencies mapped onto a member (propx rows), maos/fdpcg.c:252-326
     struct PF {
         long p;
         long px, py;
     };
-    std::vector<std::vector<PF>> pre(bs);
-    std::vector<std::vector<cplx>> fac(bs);
-    for (long ib = 0; ib < nb; ib++) {
+    parallel_for(nb, [&](long ib) {
+        std::vector<cplx> blk(bs * bs);
+        std::vector<std::vector<PF>> pre(bs);
+        std::vector<std::vector<cplx>> fac(bs);
         for (int m = 0; m < bs; m++) {
             const Member &mb = members[ib * bs + m];
             const long n1 = nx[mb.l], n2 = ny[mb.l];
@@ -422,11 +429,68 @@ FdpcgOut fdpcg_prepare(const FdpcgIn &in) {
         }
         for (int m = 0; m < bs; m++) {
             const Member &mb = members[ib * bs + m];
-            blk[m + m * bs] += invpsd[noff[mb.l] + mb.jxs + mb.jys * nx[mb.l]];
+            out.diag_at[ib * bs + m] = noff[mb.l] + mb.jxs + mb.jys * nx[mb.l];
+            out.lsize[ib * bs + m] = (double)(nx[mb.l] * ny[mb.l]);
         }
-        std::memcpy(&out.Mhat[ib * bs * bs * 2], blk.data(), sizeof(cplx) * bs * bs);
+        std::memcpy(&out.MhatW[ib * bs * bs * 2], blk.data(), sizeof(cplx) * bs * bs);
+    });
+    return out;
+}
+
+// The regulariser diagonal (invpsd) added to the WFS sums, then the drop
+// tolerances, symmetrisation, scaling and block pseudo inverses.
+void fdpcg_finish(FdpcgOut &out, const FdpcgIn &in) {
+    const int nps = in.nps;
+    const long *nx = in.nx.data(), *ny = in.ny.data();
+    const int bs = out.bs;
+    const long nb = out.nb;
+    const int needscale = out.scale;
+    long nxtot = 0;
+    std::vector<long> noff(nps);
+    for (int ips = 0; ips < nps; ips++) {
+        noff[ips] = nxtot;
+        nxtot += nx[ips] * ny[ips];
     }
+    // --- invpsd (maos/fdpcg.c:373-407) ---------------------------------------
+    std::vector<double> invpsd(nxtot);
+    parallel_for(nps, [&](long ips) {
+        const long n1 = nx[ips], n2 = ny[ips];
+        // first column of L2'L2 for the periodic 5-point L2 of
+        // lib/laplacian.c:70-97, scaled by sqrt(cxxscale*TOMOSCALE)
+        const double c = in.l2cf[ips];
+        std::vector<double> l2e(n1 * n2, 0.);  // L2 * e0
+        auto idx = [&](long ix, long iy) { return ((ix % n1 + n1) % n1) + ((iy % n2 + n2) % n2) * n1; };
+        // row r of L2 has -c at r and 0.25c at its 4 periodic neighbours;
+        // L2 is symmetric so column 0 equals row 0.
+        l2e[idx(0, 0)] += -c;
+        l2e[idx(-1, 0)] += 0.25 * c;
+        l2e[idx(1, 0)] += 0.25 * c;
+        l2e[idx(0, -1)] += 0.25 * c;
+        l2e[idx(0, 1)] += 0.25 * c;
+        std::vector<cplx> psd(n1 * n2, cplx(0, 0));
+        for (long iy = 0; iy < n2; iy++)
+            for (long ix = 0; ix < n1; ix++) {
+                const double v = l2e[idx(ix, iy)];
+                if (v == 0) continue;
+                // L2' applied: column of L2 at (ix,iy) spreads to its stencil
+                psd[idx(ix, iy)] += -c * v;
+                psd[idx(ix - 1, iy)] += 0.25 * c * v;
+                psd[idx(ix + 1, iy)] += 0.25 * c * v;
+                psd[idx(ix, iy - 1)] += 0.25 * c * v;
+                psd[idx(ix, iy + 1)] += 0.25 * c * v;
+            }
+        dft2(psd, n1, n2, -1);
+        fftshift2(psd, n1, n2);
+        const double eps = 2.220446049250313e-16;
+        double mx = 0;
+        for (const cplx &v : psd) mx = std::max(mx, std::abs(v));
+        mx *= std::sqrt(eps);
+        for (long i = 0; i < n1 * n2; i++) invpsd[noff[ips] + i] = psd[i].real() + mx;
+    });
+
+    out.Mhat = out.MhatW;
     cplx *M = reinterpret_cast<cplx *>(out.Mhat.data());
+    for (long i = 0; i < nb * bs; i++) M[(i / bs) * bs * bs + (i % bs) * (bs + 1)] += invpsd[out.diag_at[i]];
     const long ntot = nb * bs * bs;
     auto droptol = [&]() {  // math/sp.c:836-843 with thres = EPS
         double mx = 0;
@@ -451,16 +515,14 @@ FdpcgOut fdpcg_prepare(const FdpcgIn &in) {
         // scale by N per column so the unnormalized FFT pair needs no scaling
         for (long ib = 0; ib < nb; ib++)
             for (int b = 0; b < bs; b++) {
-                const Member &mb = members[ib * bs + b];
-                const double s = (double)(nx[mb.l] * ny[mb.l]);
+                const double s = out.lsize[ib * bs + b];
                 for (int a = 0; a < bs; a++) M[ib * bs * bs + a + b * bs] *= s;
             }
     }
     // --- block pseudo inverse (maos/fdpcg.c:555-569) ------------------------
     out.Minv.assign(nb * bs * bs * 2, 0.);
     cplx *Mi = reinterpret_cast<cplx *>(out.Minv.data());
-    for (long ib = 0; ib < nb; ib++) herm_pinv(M + ib * bs * bs, Mi + ib * bs * bs, bs, 1e-7);
-    return out;
+    parallel_for(nb, [&](long ib) { herm_pinv(M + ib * bs * bs, Mi + ib * bs * bs, bs, 1e-7); });
 }
 
 }  // namespace mt

@@ -92,11 +92,7 @@ void build_regularisation(mt_setup *s) {
     }
 }
 
-void build_fdpcg(mt_setup *s) {
-    if (s->precond != 1) {
-        s->fd = FdpcgOut();
-        return;
-    }
+FdpcgIn fdpcg_input(const mt_setup *s) {
     // the single high-order powfs (maos/fdpcg.c:332-342)
     int hipowfs = -1;
     for (size_t ip = 0; ip < s->powfs.size(); ip++) {
@@ -124,7 +120,27 @@ void build_fdpcg(mt_setup *s) {
         in.thetay.push_back(s->wfs[iw].thetay);
         in.neam.push_back(s->neam[iw]);
     }
-    s->fd = fdpcg_prepare(in);
+    return in;
+}
+
+void build_fdpcg(mt_setup *s) {
+    if (s->precond != 1) {
+        s->fd = FdpcgOut();
+        return;
+    }
+    s->fd = fdpcg_prepare(fdpcg_input(s));
+}
+
+// A new turbulence profile changes only the regulariser (layer weights and
+// r0 enter through l2cf): the blocks' WFS sums are kept from the setup and
+// only the diagonal, tolerances and block inverses are redone.
+void update_fdpcg(mt_setup *s) {
+    if (s->precond != 1) return;
+    if (s->fd.MhatW.empty()) {
+        build_fdpcg(s);
+        return;
+    }
+    fdpcg_finish(s->fd, fdpcg_input(s));
 }
 
 void build(mt_setup *s, const mt_parms_t *p) {
@@ -386,7 +402,7 @@ int mt_setup_update_cn2(mt_setup_t *s, const double *wt, double r0z) {
         s->wt0.assign(wt, wt + s->npsr);
         s->r0 = r0z * std::pow(std::cos(s->za), 3. / 5.);
         build_regularisation(s);
-        build_fdpcg(s);
+        update_fdpcg(s);
     });
 }
 

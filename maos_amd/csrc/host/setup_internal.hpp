@@ -91,8 +91,19 @@ struct FdpcgOut {
     std::vector<double> Minv;   // same layout, after the SVD pseudo inverse
     std::vector<long> perm;     // nb*bs
     std::vector<long> permhf;   // nbhalf*bs
+    // The part that does not depend on the turbulence profile, kept for
+    // profile updates: the WFS sum of every block (before the regulariser
+    // diagonal), and per block member its index in the stacked fftshifted
+    // layer spectra and the size of its layer.
+    std::vector<double> MhatW;  // nb*bs*bs complex
+    std::vector<long> diag_at;  // nb*bs
+    std::vector<double> lsize;  // nb*bs
 };
+// fdpcg_prepare = fdpcg_geometry, then fdpcg_finish.  A profile update
+// (layer weights, r0: only in.l2cf changes) reruns fdpcg_finish alone.
 FdpcgOut fdpcg_prepare(const FdpcgIn &in);
+FdpcgOut fdpcg_geometry(const FdpcgIn &in);
+void fdpcg_finish(FdpcgOut &out, const FdpcgIn &in);
 
 // Hermitian pseudo inverse of a bs x bs block with relative singular value
 // threshold (math/blas.c:456-500 csvd_pow(A,-1,thres)).
