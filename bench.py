@@ -280,9 +280,6 @@ def bench_uneven(steps, check=3):
         cfg = cfgs[idx[0]]
         r = tomo.Recon(cfg)
         g = tomo.GpuTomo(r, nbatch=len(idx), maxit=cfg["maxit"])
-        # FFTs in LDS stages: one launch per FFT stage for all 47 plans (the
-        # register kernels would add a launch per point count; launch-bound here)
-        g.tune_set(0, 0, 0, 0, 2)
         plans[name] = (cfg, r, g)
         members.append((g, 0, len(idx)))
         for s, i in enumerate(idx):
@@ -294,6 +291,9 @@ def bench_uneven(steps, check=3):
         grads[i] = workload.make_inputs(cfg, r, 1000 + i)
         g.set_grad(s, grads[i])
     grp = tomo.GpuGroup(members)
+    # one launch shape for the whole group, set on every plan (the plans'
+    # solves alone below use it too)
+    tuned = grp.autotune()
     for _ in range(2):
         grp.solve(mode=1)
     lat = []
@@ -336,6 +336,8 @@ def bench_uneven(steps, check=3):
                 plans=len(plans), solves_per_s=n * len(lat) / lat.sum(),
                 p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
                 sequential_plain_solves_per_s=n / seq, speedup=float((n * len(lat) / lat.sum()) / (n / seq)),
+                autotuned=dict(wfs=tuned[0], layer=tuned[1], layer_tile_rows=tuned[3],
+                               column_fft={0: "registers where available", 2: "lds"}.get(tuned[4], tuned[4])),
                 vs_oracle_first=errs)
 
 

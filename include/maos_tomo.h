@@ -286,6 +286,15 @@ typedef struct gpu_tomo_group gpu_tomo_group_t;
 int  gpu_tomo_group_create(int n, gpu_tomo_t *const *plans, const int *s0,
                            const int *nsys, gpu_tomo_group_t **out);
 int  gpu_tomo_group_solve(gpu_tomo_group_t *grp, int mode, void *stream);
+/* Autotuner for a group: times whole graph solves of the group under the
+ * column kernel (registers where a plan allows / LDS stages), the layer
+ * tile rows (64 / 16) and the systems per workgroup of the WFS and layer
+ * passes (1, 2, 4), by coordinate descent from the built-in heuristic, and
+ * sets the fastest on every member plan (as gpu_tomo_tune_set would, so a
+ * plan's results alone and in the group stay the same bits).  choice[6]
+ * (may be NULL) receives it in gpu_tomo_tune_set's order.  Overwrites the
+ * solutions of the group's slots.  No reference counterpart. */
+int  gpu_tomo_group_autotune(gpu_tomo_group_t *grp, int *choice);
 void gpu_tomo_group_free(gpu_tomo_group_t *grp);
 
 /* Request loop with continuous batching.  Each AO loop (add_loop) owns a

@@ -106,6 +106,7 @@ def lib():
     L.gpu_tomo_get_x.argtypes = [vp, C.c_int, dp, dp]
     L.gpu_tomo_group_create.argtypes = [C.c_int, C.POINTER(vp), ip, ip, C.POINTER(vp)]
     L.gpu_tomo_group_solve.argtypes = [vp, C.c_int, vp]
+    L.gpu_tomo_group_autotune.argtypes = [vp, ip]
     L.gpu_tomo_group_free.argtypes = [vp]
     L.gpu_tomo_group_free.restype = None
     L.gpu_tomo_autotune.argtypes = [vp, C.c_int, ip]
@@ -157,7 +158,7 @@ EXPORTS = [
     "mt_fdpcg_free", "gpu_tomo_setup",
     "gpu_tomo_update_cn2", "gpu_tomo_reset", "gpu_tomo_free", "gpu_tomo_solve",
     "gpu_tomo_solve_batch", "gpu_tomo_solve_dev", "gpu_tomo_set_grad", "gpu_tomo_get_x",
-    "gpu_tomo_group_create", "gpu_tomo_group_solve", "gpu_tomo_group_free",
+    "gpu_tomo_group_create", "gpu_tomo_group_solve", "gpu_tomo_group_autotune", "gpu_tomo_group_free",
     "gpu_tomo_server_create", "gpu_tomo_server_add_loop", "gpu_tomo_server_remove_loop",
     "gpu_tomo_server_submit", "gpu_tomo_server_poll", "gpu_tomo_server_drain", "gpu_tomo_server_get_x",
     "gpu_tomo_server_usage", "gpu_tomo_server_free", "gpu_tomo_autotune", "gpu_tomo_tune_set", "gpu_tomo_info", "gpu_tomo_residuals", "gpu_tomo_L", "gpu_tomo_R", "gpu_tomo_P",

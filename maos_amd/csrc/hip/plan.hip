@@ -1817,6 +1817,8 @@ int gpu_tomo_get_x(gpu_tomo_t *g, int isys, double *x, double *cgres) {
 }  // extern "C"
 
 struct gpu_tomo_group {
+    std::vector<gpu_tomo *> plans;
+    std::vector<int> s0, nsys;
     std::unique_ptr<Batch> B;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -1839,7 +1841,10 @@ int gpu_tomo_group_create(int n, gpu_tomo_t *const *plans, const int *s0, const 
             if (!g) fail(MT_EINVAL, "null plan");
         grp->device = pl[0]->device;
         HIPCHK(hipSetDevice(grp->device));
-        grp->B = make_batch(pl, std::vector<int>(s0, s0 + n), std::vector<int>(nsys, nsys + n));
+        grp->plans = pl;
+        grp->s0.assign(s0, s0 + n);
+        grp->nsys.assign(nsys, nsys + n);
+        grp->B = make_batch(pl, grp->s0, grp->nsys);
         HIPCHK(hipStreamCreateWithFlags(&grp->stream, hipStreamNonBlocking));
     });
     if (rc == MT_OK) *out = grp.release();
@@ -1859,6 +1864,81 @@ int gpu_tomo_group_solve(gpu_tomo_group_t *grp, int mode, void *stream) {
             enqueue_solve(*grp->B, rec);
         }
         if (!stream) HIPCHK(hipStreamSynchronize(st));
+    });
+}
+
+// Launch shape of a group: the same choice set on every member plan (a
+// plan property, so each system's bits stay those of the plan alone), the
+// tables rebuilt and the graph captured again on next use.
+namespace {
+void group_apply(gpu_tomo_group *grp, const int *c) {
+    for (auto *g : grp->plans) {
+        g->tune_wfs = c[0];
+        g->tune_layer = c[1];
+        g->tune_col = c[2];
+        g->tune_lty = c[3];
+        g->tune_colfft = c[4];
+        drop_batches(g);
+        apply_lty(g);
+    }
+    if (grp->gexec) (void)hipGraphExecDestroy(grp->gexec);
+    if (grp->graph) (void)hipGraphDestroy(grp->graph);
+    grp->gexec = nullptr;
+    grp->graph = nullptr;
+    grp->B = make_batch(grp->plans, grp->s0, grp->nsys);
+}
+}  // namespace
+
+int gpu_tomo_group_autotune(gpu_tomo_group_t *grp, int *choice) {
+    if (!grp) return MT_EINVAL;
+    return guarded([&] {
+        HIPCHK(hipSetDevice(grp->device));
+        HIPCHK(hipDeviceSynchronize());
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        auto timed = [&](const int *c) {
+            group_apply(grp, c);
+            capture_solve(*grp->B, &grp->graph, &grp->gexec, &grp->evs);
+            std::vector<float> ts;
+            for (int rep = 0; rep < 4; rep++) {
+                HIPCHK(hipEventRecord(e0, grp->stream));
+                HIPCHK(hipGraphLaunch(grp->gexec, grp->stream));
+                HIPCHK(hipEventRecord(e1, grp->stream));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (rep > 0) ts.push_back(ms);
+            }
+            std::sort(ts.begin(), ts.end());
+            return (double)ts[ts.size() / 2];
+        };
+        // coordinate descent from the heuristic: column kernel, layer tile
+        // rows, then systems per workgroup of the WFS and layer passes; a
+        // change is kept when it saves more than 2 %
+        int best[6] = {0, 0, 0, 0, 0, 1};
+        double tbest = timed(best);
+        auto trial = [&](int k, std::initializer_list<int> vals) {
+            for (int v : vals) {
+                if (v == best[k]) continue;
+                int c[6];
+                std::copy(best, best + 6, c);
+                c[k] = v;
+                const double t = timed(c);
+                if (t < tbest * 0.98) {
+                    tbest = t;
+                    best[k] = v;
+                }
+            }
+        };
+        trial(4, {2});
+        trial(3, {TILE});
+        trial(0, {1, 2, 4});
+        trial(1, {1, 2, 4});
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        group_apply(grp, best);
+        if (choice) std::copy(best, best + 6, choice);
     });
 }
 

@@ -303,6 +303,13 @@ class GpuGroup:
         _abi.check(_abi.lib().gpu_tomo_group_solve(self._h, int(mode), C.c_void_p(stream or 0)),
                    "gpu_tomo_group_solve")
 
+    def autotune(self):
+        """gpu_tomo_group_autotune: the fastest launch shape for the whole
+        group, set on every member plan; returns it in tune_set's order."""
+        ch = (C.c_int * 6)()
+        _abi.check(_abi.lib().gpu_tomo_group_autotune(self._h, ch), "gpu_tomo_group_autotune")
+        return tuple(ch)
+
     def close(self):
         if getattr(self, "_h", None):
             _abi.lib().gpu_tomo_group_free(self._h)

@@ -101,3 +101,24 @@ def test_group_register_kernels_equal_alone():
     grp.close()
     for cfg, g, grad, x1 in plans:
         g.close()
+
+
+def test_group_autotune_keeps_alone_bits(plans):
+    """The group autotuner sets one launch shape on every member plan; each
+    system of the tuned group still equals its plan solved alone."""
+    grp = tomo.GpuGroup([(g, 0, 2) for cfg, r, g, grads in plans])
+    choice = grp.autotune()
+    print("group autotune", choice, flush=True)
+    assert choice[4] in (0, 2) and choice[3] in (0, 16) and choice[0] in (0, 1, 2, 4) and choice[1] in (0, 1, 2, 4)
+    alone = []
+    for cfg, r, g, grads in plans:
+        alone.append([g.tomo(grads[s], isys=s)[0] for s in range(2)])
+    for cfg, r, g, grads in plans:
+        for s in range(2):
+            g.set_grad(s, grads[s])
+    for mode in (0, 1):
+        grp.solve(mode=mode)
+        for k, (cfg, r, g, grads) in enumerate(plans):
+            for s in range(2):
+                assert np.array_equal(g.get_x(s)[0], alone[k][s]), "member %d slot %d (mode %d)" % (k, s, mode)
+    grp.close()
