@@ -328,12 +328,21 @@ int  gpu_tomo_server_get_x(gpu_tomo_server_t *s, int loop, double *x,
                            double *cgres);
 int  gpu_tomo_server_usage(gpu_tomo_server_t *s, double *used_gb,
                            double *budget_gb);
+/* Autotuner for a server: gpu_tomo_group_autotune over the admitted loops
+ * (one system each, as a batch carries them), the result set on every
+ * loop's plan and the cached batch graphs captured again before their next
+ * launch.  Each loop's warm-start solution is kept.  Call it while the
+ * server is idle (nothing submitted and not yet completed: after drain());
+ * MT_ESTATE otherwise, MT_EINVAL with no loops.  A loop added later keeps
+ * the default launch shape.  choice[6] (may be NULL) receives the choice in
+ * gpu_tomo_tune_set's order.  No reference counterpart. */
+int  gpu_tomo_server_autotune(gpu_tomo_server_t *s, int *choice);
 void gpu_tomo_server_free(gpu_tomo_server_t *s);
 
 /* Autotuner: times the solve of nsys systems under each layer tile shape
  * (16 x 64 or 16 x 16 nodes, where both fit), each column kernel (in
  * registers, where every FDPCG block couples one frequency across the
- * layers, at most 6, and the column length is 2^a x P with P in 2, 3, 4, 5,
+ * layers, at most 8, and the column length is 2^a x P with P in 2, 3, 4, 5,
  * 7; or in LDS stages, always available) and each choice of systems per
  * workgroup (1, 2, 4) of the WFS, layer and LDS column kernels, and keeps
  * the fastest for this plan; then times whole graph solves of the nsys

@@ -2512,6 +2512,48 @@ int gpu_tomo_server_remove_loop(gpu_tomo_server_t *S, int loop) {
     return MT_OK;
 }
 
+int gpu_tomo_server_autotune(gpu_tomo_server_t *S, int *choice) {
+    if (!S) return MT_EINVAL;
+    // the lock is held throughout: the worker admits nothing meanwhile, and
+    // with no request pending or in flight no batch is on the GPU
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (!S->pending.empty() || S->inflight_reqs) {
+        mt::set_error("server autotune needs an idle server (drain first)");
+        return MT_ESTATE;
+    }
+    std::vector<gpu_tomo_t *> pl;
+    for (auto &L : S->loops)
+        if (L->plan) pl.push_back(L->plan);
+    if (pl.empty()) return MT_EINVAL;
+    const int n = (int)pl.size();
+    gpu_tomo_group_t *grp = nullptr;
+    std::vector<DevBuf<float>> keep(n);
+    int rc = guarded([&] {
+        HIPCHK(hipSetDevice(S->device));
+        // the timed solves overwrite each plan's warm-start solution
+        for (int j = 0; j < n; j++) {
+            keep[j].alloc(pl[j]->x.n);
+            HIPCHK(hipMemcpy(keep[j].p, pl[j]->x.p, sizeof(float) * pl[j]->x.n, hipMemcpyDeviceToDevice));
+        }
+    });
+    if (rc != MT_OK) return rc;
+    const std::vector<int> s0(n, 0), ns(n, 1);
+    rc = gpu_tomo_group_create(n, pl.data(), s0.data(), ns.data(), &grp);
+    if (rc == MT_OK) rc = gpu_tomo_group_autotune(grp, choice);
+    gpu_tomo_group_free(grp);
+    // every flight's batch tables and graph predate the new shapes: rebuild
+    // them now rather than at their first launch, inside the request stream
+    S->gen++;
+    const int rc2 = guarded([&] {
+        for (int j = 0; j < n; j++)
+            HIPCHK(hipMemcpy(pl[j]->x.p, keep[j].p, sizeof(float) * pl[j]->x.n, hipMemcpyDeviceToDevice));
+        HIPCHK(hipDeviceSynchronize());
+        if (rc == MT_OK)
+            for (auto &f : S->flights) flight_build(S, f);
+    });
+    return rc != MT_OK ? rc : rc2;
+}
+
 int gpu_tomo_server_submit(gpu_tomo_server_t *S, int loop, const double *grad, long tag) {
     if (!S || !grad) return MT_EINVAL;
     SrvReq r;

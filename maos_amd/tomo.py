@@ -374,6 +374,14 @@ class Server:
                    "gpu_tomo_server_get_x")
         return x, res.value
 
+    def autotune(self):
+        """gpu_tomo_server_autotune: the fastest launch shape for the
+        admitted loops as one batch (call after drain()); returns it in
+        GpuTomo.tune_set's order."""
+        ch = (C.c_int * 6)()
+        _abi.check(_abi.lib().gpu_tomo_server_autotune(self._h, ch), "gpu_tomo_server_autotune")
+        return tuple(ch)
+
     def usage(self):
         u, b = C.c_double(), C.c_double()
         _abi.lib().gpu_tomo_server_usage(self._h, C.byref(u), C.byref(b))

@@ -88,3 +88,35 @@ def test_admission_by_memory(loops):
     srv.remove_loop(a)
     srv.add_loop(r)  # admitted again once memory is free
     srv.close()
+
+
+def test_autotune_between_steps(loops):
+    """A server autotuned between steps keeps each loop's warm-start
+    solution, and each loop's steps are those of its plan alone (warm start)
+    switched to the same launch shape at the same step, bit for bit."""
+    which = [0, 1, 2]
+    srv = tomo.Server(mem_budget_gb=8.0, max_inflight=2)
+    with pytest.raises(_abi.MtError):
+        srv.autotune()  # no loops yet
+    ids = [srv.add_loop(loops[i][1], maxit=10) for i in which]
+    xs = {i: [] for i in which}
+    choice = None
+    for k in range(NSTEP):
+        if k == 2:
+            choice = srv.autotune()
+        for i, lid in zip(which, ids):
+            srv.submit(lid, loops[i][2][k], tag=k)
+        srv.drain()
+        for i, lid in zip(which, ids):
+            xs[i].append(srv.get_x(lid)[0])
+    srv.close()
+    print("server autotune choice", choice)
+    for i in which:
+        g = tomo.GpuTomo(loops[i][1], maxit=10, warm=1)
+        g.tune_set(0, 0, 0, 0, 2, 0)  # a server loop's shape before tuning
+        for k in range(NSTEP):
+            if k == 2:
+                g.tune_set(*choice[:5], 0)
+            x = g.tomo(loops[i][2][k])[0]
+            assert np.array_equal(x, xs[i][k]), "loop %d step %d" % (i, k)
+        g.close()
