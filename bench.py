@@ -385,6 +385,28 @@ def served_loops():
     return cf
 
 
+def bench_mvm(r, maxit, ncol=2048, batch=256):
+    """Tomography half of MVM assembly on large30: the first ncol columns of
+    T (x = T g), each the cold maxit-iteration solve of a unit gradient
+    vector, batch columns per graph solve, written to host memory as
+    doubles (gpu_tomo_mvm_tomo).  The launch shape is autotuned at that
+    batch first."""
+    from maos_amd import tomo
+    g = tomo.GpuTomo(r, nbatch=batch, maxit=maxit)
+    tuned = g.autotune(nsys=batch)
+    g.mvm_tomo(0, batch)  # warm-up: graph capture at this batch
+    t0 = time.perf_counter()
+    T = g.mvm_tomo(0, ncol)
+    dt = time.perf_counter() - t0
+    g.close()
+    return dict(workload="large30 T columns 0..%d (ng = %d, nx = %d), %d iterations, %d per graph solve"
+                         % (ncol - 1, r.ng, r.nx, maxit, batch),
+                columns_per_s=ncol / dt, seconds=dt, full_T_seconds_projected=r.ng / (ncol / dt),
+                output_GB=T.nbytes / 1e9, nonzero_columns=int(np.count_nonzero(np.abs(T).sum(0))),
+                autotuned=dict(wfs=tuned[0], layer=tuned[1], layer_tile_rows=tuned[3],
+                               column_fft="registers" if tuned[4] != 2 else "lds", flights=tuned[5]))
+
+
 def bench_served(nsteps=40, cycles=3, rate_hz=800.0, seed=11, low_rate_hz=400.0):
     """Served stream: 16 loops at rate_hz with Poisson-jittered arrivals,
     time-correlated (frozen-flow, 10 m/s) gradients, warm start and 10
@@ -494,7 +516,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", type=int, default=None,
                     help="solve precision mode (0, 1, 2; default: the library default)")
-    ap.add_argument("--modes", default="plain,graph,batched,uneven,served,precision,sweep",
+    ap.add_argument("--modes", default="plain,graph,batched,uneven,served,mvm,precision,sweep",
                     help="comma list of modes to time (profiling runs use one)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of 1 GiB (known HBM traffic) for counter scaling")
@@ -614,6 +636,13 @@ def main():
         except Exception as e:  # reported, not fatal for the headline line
             precision = {"error": repr(e)}
 
+    mvm = None
+    if "mvm" in modes:
+        try:
+            mvm = bench_mvm(r, maxit)
+        except Exception as e:
+            mvm = {"error": repr(e)}
+
     served = None
     if "served" in modes:
         try:
@@ -657,6 +686,7 @@ def main():
         "cpu_baseline": cpu,
         "uneven64": uneven,
         "served16": served,
+        "mvm_tomography": mvm,
         "precision_study": precision,
         "batch_sweep": sweep,
     }

@@ -241,6 +241,16 @@ int  gpu_tomo_solve_batch(gpu_tomo_t *g, int nsys, const double *grad,
  * results do not depend on k. */
 int  gpu_tomo_solve_dev(gpu_tomo_t *g, const float *d_grad, float *d_x,
                         int nbatch, int mode, void *stream);
+/* Tomography half of MVM assembly: columns [col0, col0 + ncol) of the
+ * matrix T with x = T g, each the solve of a unit gradient vector (cold
+ * start, maxit iterations, nbatch columns per graph solve), into out
+ * (host, nx x ncol doubles, column-major: column k = solution of e_{col0+k}
+ * in gpu_tomo_solve's layer layout).  Columns of WFS the solve does not use
+ * are zero.  Overwrites the gradients and solutions of the plan's slots.
+ * MAOS assembles its MVM by solving tomography and fitting for every
+ * gradient column, in batches (mvm_direct, cuda/mvm_direct.cu:132); this is
+ * the tomography part. */
+int  gpu_tomo_mvm_tomo(gpu_tomo_t *g, long col0, long ncol, double *out);
 /* Residual sqrt(r'r/b'b) of each system after the last solve (host copy). */
 int  gpu_tomo_residuals(gpu_tomo_t *g, double *cgres, int nbatch);
 

@@ -1873,6 +1873,15 @@ __global__ __launch_bounds__(256) void k_stage_out(const Job *__restrict__ jobs,
 __global__ void k_copy(const float *a, float *b, long n) {
     for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
 }
+// Unit vectors: slot k (blockIdx.y) of n slots of len values each is zero
+// but for 1 at idx[k] (all zero when idx is null).
+__global__ void k_unit(float *v, long len, const int *idx) {
+    const int k = blockIdx.y;
+    const long one = idx ? idx[k] : -1;
+    float *vk = v + (long)k * len;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < len; i += (long)gridDim.x * blockDim.x)
+        vk[i] = i == one ? 1.f : 0.f;
+}
 __global__ void k_zero_stop(KARGS) {
     KJOB;
     for (int i = threadIdx.x; i < J.nsys; i += blockDim.x) PD.stop[J.s0 + i] = 0;
@@ -2088,6 +2097,10 @@ void launch_stage_out(const Job *jobs, int njobs, const StageOff *off, float *dx
 void launch_copy(const float *a, float *b, long n, hipStream_t st) {
     const long nb = (n + 255) / 256 < 1024 ? (n + 255) / 256 : 1024;
     hipLaunchKernelGGL(k_copy, dim3((unsigned)nb), dim3(256), 0, st, a, b, n);
+}
+void launch_unit(float *v, long len, const int *idx, int n, hipStream_t st) {
+    const long nb = (len + 255) / 256 < 256 ? (len + 255) / 256 : 256;
+    hipLaunchKernelGGL(k_unit, dim3((unsigned)nb, (unsigned)n), dim3(256), 0, st, v, len, idx);
 }
 
 }  // namespace mtk

@@ -46,6 +46,8 @@ void launch_irow(const LaunchTab &t, int mode, int rm, int rp, int slot, int ose
 void launch_final(const LaunchTab &t, hipStream_t st);
 void launch_zero_stop(const LaunchTab &t, hipStream_t st);
 void launch_copy(const float *a, float *b, long n, hipStream_t st);
+// n unit vectors of len values at v (stride len), 1 at idx[k]; zeros if idx is null
+void launch_unit(float *v, long len, const int *idx, int n, hipStream_t st);
 // Packed staging of the jobs that are on: gradients from src into each
 // plan (in), solutions and residuals from each plan into dx, dres (out).
 void launch_stage_in(const Job *jobs, int njobs, const StageOff *off, const float *src, hipStream_t st);

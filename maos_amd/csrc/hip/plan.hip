@@ -1745,6 +1745,89 @@ int gpu_tomo_solve_dev(gpu_tomo_t *g, const float *d_grad, float *d_x, int nbatc
     });
 }
 
+int gpu_tomo_mvm_tomo(gpu_tomo_t *g, long col0, long ncol, double *out) {
+    if (!g || !out || col0 < 0 || ncol <= 0 || col0 + ncol > g->ngtot_all) return MT_EINVAL;
+    return guarded([&] {
+        HIPCHK(hipSetDevice(g->device));
+        const Geom &G = g->G;
+        const long nx = g->nxtot;
+        hipStream_t st = g->stream;
+        // device gradient index of each requested column; the columns of a
+        // WFS the solve does not use are zero
+        std::vector<int> kof(g->goff_all.size() - 1, -1);
+        for (int k = 0; k < G.nwfs; k++) kof[g->wfs_active[k]] = k;
+        std::vector<int> idx;
+        std::vector<long> col;
+        for (long j = col0; j < col0 + ncol; j++) {
+            const int w = (int)(std::upper_bound(g->goff_all.begin(), g->goff_all.end(), j) - g->goff_all.begin()) - 1;
+            const int k = kof[w];
+            if (k < 0) {
+                std::fill(out + (j - col0) * nx, out + (j - col0 + 1) * nx, 0.0);
+            } else {
+                idx.push_back((int)(G.goff[k] + (j - g->goff_all[w])));
+                col.push_back(j - col0);
+            }
+        }
+        if (idx.empty()) return;
+        DevBuf<int> didx;
+        didx.upload(idx);
+        // chunks of nbatch columns; two staging buffers, so that the host
+        // widens chunk t-1 to double while the GPU solves chunk t
+        const int B = g->nbatch;
+        const long nch = ((long)idx.size() + B - 1) / B;
+        float *h[2] = {nullptr, nullptr};
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        auto widen = [&](long t) {
+            HIPCHK(hipEventSynchronize(ev[t % 2]));
+            const long c = t * B;
+            const int n = (int)std::min<long>(B, (long)idx.size() - c);
+            // columns over up to 8 host threads (the output's first touch
+            // and the widening are host-bound otherwise)
+            auto cols = [&](int k0, int k1) {
+                for (int k = k0; k < k1; k++) {
+                    double *o = out + col[c + k] * nx;
+                    const float *hk = h[t % 2] + (long)k * nx;
+                    for (long i = 0; i < nx; i++) o[i] = hk[i];
+                }
+            };
+            const int nt = std::min(n, 8);
+            std::vector<std::thread> th;
+            for (int q = 1; q < nt; q++) th.emplace_back(cols, q * n / nt, (q + 1) * n / nt);
+            cols(0, n / nt);
+            for (auto &x : th) x.join();
+        };
+        try {
+            for (int b = 0; b < 2; b++) {
+                HIPCHK(hipHostMalloc((void **)&h[b], sizeof(float) * nx * B));
+                HIPCHK(hipEventCreateWithFlags(&ev[b], hipEventDisableTiming));
+            }
+            for (long t = 0; t < nch; t++) {
+                const long c = t * B;
+                const int n = (int)std::min<long>(B, (long)idx.size() - c);
+                launch_unit(g->g.p, G.ngtot, didx.p + c, n, st);
+                if (g->warm) launch_unit(g->x.p, G.nxtot, nullptr, n, st);  // cold start
+                graph_solve(g, n, st);
+                HIPCHK(hipMemcpy2DAsync(h[t % 2], sizeof(float) * nx, g->x.p, sizeof(float) * G.nxtot,
+                                        sizeof(float) * nx, n, hipMemcpyDeviceToHost, st));
+                HIPCHK(hipEventRecord(ev[t % 2], st));
+                if (t > 0) widen(t - 1);
+            }
+            widen(nch - 1);
+        } catch (...) {
+            (void)hipStreamSynchronize(st);
+            for (int b = 0; b < 2; b++) {
+                if (h[b]) (void)hipHostFree(h[b]);
+                if (ev[b]) (void)hipEventDestroy(ev[b]);
+            }
+            throw;
+        }
+        for (int b = 0; b < 2; b++) {
+            (void)hipHostFree(h[b]);
+            (void)hipEventDestroy(ev[b]);
+        }
+    });
+}
+
 int gpu_tomo_residuals(gpu_tomo_t *g, double *cgres, int nbatch) {
     if (!g || !cgres || nbatch <= 0 || nbatch > g->nbatch) return MT_EINVAL;
     return guarded([&] {

@@ -257,6 +257,15 @@ class GpuTomo:
         _abi.check(_abi.lib().gpu_tomo_P(self._h, _dp(xi), _dp(xo)), "gpu_tomo_P")
         return xo
 
+    def mvm_tomo(self, col0=0, ncol=None):
+        """gpu_tomo_mvm_tomo: columns [col0, col0 + ncol) of T (x = T g),
+        each the solve of a unit gradient vector; (nx, ncol) array."""
+        ng = self.recon.ng
+        ncol = ng - col0 if ncol is None else ncol
+        out = np.zeros((ncol, self.recon.nx))
+        _abi.check(_abi.lib().gpu_tomo_mvm_tomo(self._h, int(col0), int(ncol), _dp(out)), "gpu_tomo_mvm_tomo")
+        return out.T
+
     def solve_dev(self, d_grad, d_x, nbatch=None, mode=0, stream=None):
         """Batched solve on device buffers (int device pointers, float32)."""
         nb = self.nbatch if nbatch is None else nbatch
