@@ -407,6 +407,30 @@ def bench_mvm(r, maxit, ncol=2048, batch=256):
                                column_fft="registers" if tuned[4] != 2 else "lds", flights=tuned[5]))
 
 
+def bench_update(cfg, r, maxit, reps=5):
+    """A Cn2 profile change on large30 (new layer weights and r0): the host
+    update (mt_setup_update_cn2: regulariser, FDPCG block pseudo inverses)
+    and the GPU plan's re-upload (gpu_tomo_update_cn2), median of reps."""
+    from maos_amd import tomo
+    g = tomo.GpuTomo(r, nbatch=1, maxit=maxit)
+    wt0, r00 = np.array(cfg["wt"], float), cfg["r0z"]
+    th, tg = [], []
+    for k in range(reps):
+        w = wt0 * (1 + 0.1 * (k + 1) * np.arange(len(wt0)) / len(wt0))
+        t0 = time.perf_counter()
+        r.update_cn2(w / w.sum(), r00 * (1 + 0.02 * (k + 1)))
+        t1 = time.perf_counter()
+        g.update_cn2(r)
+        t2 = time.perf_counter()
+        th.append(t1 - t0)
+        tg.append(t2 - t1)
+    r.update_cn2(wt0 / wt0.sum(), r00)
+    g.close()
+    return dict(workload="large30 Cn2 update (6 layers, 9216 FDPCG blocks of 6 x 6)",
+                host_update_ms=1e3 * float(np.median(th)), gpu_upload_ms=1e3 * float(np.median(tg)),
+                host_threads=min(16, os.cpu_count() or 1))
+
+
 def bench_served(nsteps=40, cycles=3, rate_hz=800.0, seed=11, low_rate_hz=400.0):
     """Served stream: 16 loops at rate_hz with Poisson-jittered arrivals,
     time-correlated (frozen-flow, 10 m/s) gradients, warm start and 10
@@ -516,7 +540,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", type=int, default=None,
                     help="solve precision mode (0, 1, 2; default: the library default)")
-    ap.add_argument("--modes", default="plain,graph,batched,uneven,served,mvm,precision,sweep",
+    ap.add_argument("--modes", default="plain,graph,batched,uneven,served,mvm,update,precision,sweep",
                     help="comma list of modes to time (profiling runs use one)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of 1 GiB (known HBM traffic) for counter scaling")
@@ -636,6 +660,13 @@ def main():
         except Exception as e:  # reported, not fatal for the headline line
             precision = {"error": repr(e)}
 
+    update = None
+    if "update" in modes:
+        try:
+            update = bench_update(cfg, r, maxit)
+        except Exception as e:
+            update = {"error": repr(e)}
+
     mvm = None
     if "mvm" in modes:
         try:
@@ -687,6 +718,7 @@ def main():
         "uneven64": uneven,
         "served16": served,
         "mvm_tomography": mvm,
+        "cn2_update": update,
         "precision_study": precision,
         "batch_sweep": sweep,
     }

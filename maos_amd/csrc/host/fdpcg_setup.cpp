@@ -95,87 +95,77 @@ void dft2(std::vector<cplx> &a, long nx, long ny, int sign) {
     pass(ny, nx, nx, 1);   // along y for every column
 }
 
-namespace {
 
-// Real symmetric Jacobi eigen decomposition (cyclic), n <= 128.
-void jacobi_eig(std::vector<double> &A, int n, std::vector<double> &w, std::vector<double> &V) {
-    V.assign(n * n, 0.);
-    for (int i = 0; i < n; i++) V[i + i * n] = 1.;
+void herm_pinv(const cplx *A0, cplx *out, int n, double thres) {
+    // math/blas.c:456-500 csvd_pow(A, -1, thres) for a Hermitian block: the
+    // singular values are the |eigenvalues|, and the SVD pseudo inverse is
+    // sum_k v_k v_k^H / lambda_k over the eigenpairs with |lambda_k| >
+    // thres * max |lambda|.  Eigenpairs by cyclic complex Jacobi: on the
+    // (p, q) plane, J = diag(1, e^{-i phi}) R zeroes a_pq = r e^{i phi},
+    // where R = [[c, s], [-s, c]] is the real Jacobi rotation of the real
+    // symmetric [[a_pp, r], [r, a_qq]].
+    constexpr int NMAX = 16;
+    if (n < 1 || n > NMAX) throw Error(MT_EINVAL, "fdpcg: block size out of range");
+    cplx A[NMAX * NMAX], V[NMAX * NMAX];
+    for (int i = 0; i < n * n; i++) {
+        A[i] = A0[i];
+        V[i] = 0;
+    }
+    for (int i = 0; i < n; i++) V[i + i * n] = 1;
     for (int sweep = 0; sweep < 100; sweep++) {
         double off = 0, tot = 0;
         for (int j = 0; j < n; j++)
             for (int i = 0; i < n; i++) {
-                double v = A[i + j * n] * A[i + j * n];
+                const double v = std::norm(A[i + j * n]);
                 tot += v;
                 if (i != j) off += v;
             }
         if (off <= 1e-30 * tot || off == 0) break;
         for (int p = 0; p < n - 1; p++) {
             for (int q = p + 1; q < n; q++) {
-                double apq = A[p + q * n];
-                if (apq == 0) continue;
-                double app = A[p + p * n], aqq = A[q + q * n];
-                double theta = (aqq - app) / (2 * apq);
-                double t = (theta >= 0 ? 1. : -1.) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
-                double c = 1 / std::sqrt(t * t + 1), s = t * c;
-                for (int k = 0; k < n; k++) {
-                    double akp = A[k + p * n], akq = A[k + q * n];
-                    A[k + p * n] = c * akp - s * akq;
-                    A[k + q * n] = s * akp + c * akq;
+                const cplx apq = A[p + q * n];
+                const double r = std::abs(apq);
+                if (r == 0) continue;
+                const cplx eph = std::conj(apq) / r;  // e^{-i phi}
+                const double app = A[p + p * n].real(), aqq = A[q + q * n].real();
+                const double theta = (aqq - app) / (2 * r);
+                const double t = (theta >= 0 ? 1. : -1.) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+                const double c = 1 / std::sqrt(t * t + 1), s = t * c;
+                const cplx se = s * eph, ce = c * eph;  // J = [[c, s], [-se, ce]]
+                for (int k = 0; k < n; k++) {           // A J
+                    const cplx akp = A[k + p * n], akq = A[k + q * n];
+                    A[k + p * n] = c * akp - se * akq;
+                    A[k + q * n] = s * akp + ce * akq;
                 }
-                for (int k = 0; k < n; k++) {
-                    double apk = A[p + k * n], aqk = A[q + k * n];
-                    A[p + k * n] = c * apk - s * aqk;
-                    A[q + k * n] = s * apk + c * aqk;
+                for (int k = 0; k < n; k++) {  // J^H A
+                    const cplx apk = A[p + k * n], aqk = A[q + k * n];
+                    A[p + k * n] = c * apk - std::conj(se) * aqk;
+                    A[q + k * n] = s * apk + std::conj(ce) * aqk;
                 }
-                for (int k = 0; k < n; k++) {
-                    double vkp = V[k + p * n], vkq = V[k + q * n];
-                    V[k + p * n] = c * vkp - s * vkq;
-                    V[k + q * n] = s * vkp + c * vkq;
+                A[p + q * n] = A[q + p * n] = 0;
+                A[p + p * n] = app - t * r;
+                A[q + q * n] = aqq + t * r;
+                for (int k = 0; k < n; k++) {  // V J
+                    const cplx vkp = V[k + p * n], vkq = V[k + q * n];
+                    V[k + p * n] = c * vkp - se * vkq;
+                    V[k + q * n] = s * vkp + ce * vkq;
                 }
             }
         }
     }
-    w.resize(n);
-    for (int i = 0; i < n; i++) w[i] = A[i + i * n];
-}
-
-}  // namespace
-
-void herm_pinv(const cplx *A, cplx *out, int n, double thres) {
-    // math/blas.c:456-500 csvd_pow(A, -1, thres) for a Hermitian block:
-    // keep singular values > thres*max, invert them.  For a Hermitian matrix
-    // the singular values are |eigenvalues| and the SVD pseudo inverse equals
-    // sum_k v_k v_k^H / lambda_k over the kept eigenpairs.  The eigenpairs
-    // come from the real symmetric embedding [[Re,-Im],[Im,Re]] (each
-    // eigenvalue of A appears twice there; its pseudo inverse embeds
-    // pinv(A)).
-    const int m = 2 * n;
-    std::vector<double> E(m * m);
-    for (int j = 0; j < n; j++)
-        for (int i = 0; i < n; i++) {
-            const cplx a = A[i + j * n];
-            E[i + j * m] = a.real();
-            E[(i + n) + (j + n) * m] = a.real();
-            E[(i + n) + j * m] = a.imag();
-            E[i + (j + n) * m] = -a.imag();
-        }
-    std::vector<double> w, V;
-    jacobi_eig(E, m, w, V);
     double maxeig = 0;
-    for (int k = 0; k < m; k++) maxeig = std::max(maxeig, std::fabs(w[k]));
+    for (int k = 0; k < n; k++) maxeig = std::max(maxeig, std::fabs(A[k + k * n].real()));
     const double thres0 = std::fabs(thres) * maxeig;
-    std::vector<double> P(m * m, 0.);
-    for (int k = 0; k < m; k++) {
-        if (!(std::fabs(w[k]) > thres0)) continue;
-        const double inv = 1. / w[k];
-        for (int j = 0; j < m; j++) {
-            const double vj = V[j + k * m] * inv;
-            for (int i = 0; i < m; i++) P[i + j * m] += V[i + k * m] * vj;
+    for (int i = 0; i < n * n; i++) out[i] = 0;
+    for (int k = 0; k < n; k++) {
+        const double w = A[k + k * n].real();
+        if (!(std::fabs(w) > thres0)) continue;
+        const double inv = 1. / w;
+        for (int j = 0; j < n; j++) {
+            const cplx vj = std::conj(V[j + k * n]) * inv;
+            for (int i = 0; i < n; i++) out[i + j * n] += V[i + k * n] * vj;
         }
     }
-    for (int j = 0; j < n; j++)
-        for (int i = 0; i < n; i++) out[i + j * n] = cplx(P[i + j * m], P[(i + n) + j * m]);
 }
 
 FdpcgOut fdpcg_prepare(const FdpcgIn &in) {
@@ -455,31 +445,23 @@ void fdpcg_finish(FdpcgOut &out, const FdpcgIn &in) {
     std::vector<double> invpsd(nxtot);
     parallel_for(nps, [&](long ips) {
         const long n1 = nx[ips], n2 = ny[ips];
-        // first column of L2'L2 for the periodic 5-point L2 of
-        // lib/laplacian.c:70-97, scaled by sqrt(cxxscale*TOMOSCALE)
-        const double c = in.l2cf[ips];
-        std::vector<double> l2e(n1 * n2, 0.);  // L2 * e0
-        auto idx = [&](long ix, long iy) { return ((ix % n1 + n1) % n1) + ((iy % n2 + n2) % n2) * n1; };
-        // row r of L2 has -c at r and 0.25c at its 4 periodic neighbours;
-        // L2 is symmetric so column 0 equals row 0.
-        l2e[idx(0, 0)] += -c;
-        l2e[idx(-1, 0)] += 0.25 * c;
-        l2e[idx(1, 0)] += 0.25 * c;
-        l2e[idx(0, -1)] += 0.25 * c;
-        l2e[idx(0, 1)] += 0.25 * c;
-        std::vector<cplx> psd(n1 * n2, cplx(0, 0));
-        for (long iy = 0; iy < n2; iy++)
-            for (long ix = 0; ix < n1; ix++) {
-                const double v = l2e[idx(ix, iy)];
-                if (v == 0) continue;
-                // L2' applied: column of L2 at (ix,iy) spreads to its stencil
-                psd[idx(ix, iy)] += -c * v;
-                psd[idx(ix - 1, iy)] += 0.25 * c * v;
-                psd[idx(ix + 1, iy)] += 0.25 * c * v;
-                psd[idx(ix, iy - 1)] += 0.25 * c * v;
-                psd[idx(ix, iy + 1)] += 0.25 * c * v;
+        // FFT of the first column of L2'L2, fftshifted.  L2 (lib/laplacian.c:
+        // 70-97, scaled by sqrt(cxxscale*TOMOSCALE) into c) is the periodic
+        // stencil -c at a node and c/4 at its four neighbours, so it is
+        // diagonal in the DFT basis with eigenvalue c (-1 + (cos ax + cos ay)/2),
+        // ax = 2 pi kx / n1, ay = 2 pi ky / n2; L2'L2 = L2^2 has its square,
+        // real and even, so the FFT of that column is exactly this (computed
+        // in long double, rounded once).
+        const long double c = in.l2cf[ips], pi2 = 6.283185307179586476925286766559L;
+        std::vector<long double> cx(n1), cy(n2);
+        for (long k = 0; k < n1; k++) cx[k] = std::cos(pi2 * k / n1);
+        for (long k = 0; k < n2; k++) cy[k] = std::cos(pi2 * k / n2);
+        std::vector<cplx> psd(n1 * n2);
+        for (long ky = 0; ky < n2; ky++)
+            for (long kx = 0; kx < n1; kx++) {
+                const long double e = c * (-1 + 0.5L * (cx[kx] + cy[ky]));
+                psd[kx + ky * n1] = cplx((double)(e * e), 0.);
             }
-        dft2(psd, n1, n2, -1);
         fftshift2(psd, n1, n2);
         const double eps = 2.220446049250313e-16;
         double mx = 0;
