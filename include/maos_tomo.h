@@ -298,7 +298,7 @@ int  gpu_tomo_group_create(int n, gpu_tomo_t *const *plans, const int *s0,
 int  gpu_tomo_group_solve(gpu_tomo_group_t *grp, int mode, void *stream);
 /* Autotuner for a group: times whole graph solves of the group under the
  * column kernel (registers where a plan allows / LDS stages), the layer
- * tile rows (64 / 16) and the systems per workgroup of the WFS and layer
+ * tile rows (64 / 32 / 16) and the systems per workgroup of the WFS and layer
  * passes (1, 2, 4), by coordinate descent from the built-in heuristic, and
  * sets the fastest on every member plan (as gpu_tomo_tune_set would, so a
  * plan's results alone and in the group stay the same bits).  choice[6]
@@ -350,7 +350,7 @@ int  gpu_tomo_server_autotune(gpu_tomo_server_t *s, int *choice);
 void gpu_tomo_server_free(gpu_tomo_server_t *s);
 
 /* Autotuner: times the solve of nsys systems under each layer tile shape
- * (16 x 64 or 16 x 16 nodes, where both fit), each column kernel (in
+ * (16 x 64, 16 x 32 or 16 x 16 nodes, where they fit), each column kernel (in
  * registers, where every FDPCG block couples one frequency across the
  * layers, at most 8, and the column length is 2^a x P with P in 2, 3, 4, 5,
  * 7; or in LDS stages, always available) and each choice of systems per
@@ -358,7 +358,7 @@ void gpu_tomo_server_free(gpu_tomo_server_t *s);
  * the fastest for this plan; then times whole graph solves of the nsys
  * systems as 1, 2 and 4 flights (gpu_tomo_solve_dev mode 1).  choice[6]
  * receives wfs, layer, col (0 with the register column kernel), the layer
- * tile rows (16 or 64), the column kernel (1 registers, 2 LDS stages) and
+ * tile rows (16, 32 or 64), the column kernel (1 registers, 2 LDS stages) and
  * the flights.  Overwrites the gradients and solutions of slots [0, nsys).  Call it before the plan joins a group.
  * The tile shape and the column kernel set the order of sums, so they can
  * change results in the last bits; systems per workgroup never do.  No
@@ -368,7 +368,7 @@ int  gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice);
 /* Set the launch shape directly, e.g. to replay an autotuned choice without
  * timing: choice[6] = systems per workgroup of the WFS, layer and LDS
  * column kernels (each 1, 2 or 4, or 0 for the built-in heuristic), the
- * layer tile rows (16, 64, or 0 for 64 where it fits), the column kernel
+ * layer tile rows (16, 32, 64, or 0 for 64 where it fits), the column kernel
  * (1 registers where it applies, 2 LDS stages, 0 as 1; 2 also keeps the row
  * FFTs in LDS stages, which otherwise run in registers for single-plan
  * batches with layers of 64 or 96 points) and the flights of a

@@ -982,14 +982,14 @@ __global__ __launch_bounds__(256) void k_layer_m(KARGS, int psel_, int osel, int
     }
 }
 
-// Layer pass of the matched path on 16 x 64 tiles with 128 threads: each
-// thread owns two adjacent columns of four vertically adjacent nodes, so the
-// five sample rows of a WFS field, read at three ray columns (the middle one
-// serves both columns), feed eight nodes.  Same arithmetic per node as
+// Layer pass of the matched path on 16 x LT tiles (LT = 64 or 32) with 128
+// threads: each thread owns two adjacent columns of R = LT / 16 vertically
+// adjacent nodes, so the R + 1 sample rows of a WFS field, read at three ray
+// columns (the middle one serves both columns), feed 2 R nodes.  Same arithmetic per node as
 // k_layer_m (the reference's bilinear transpose):
 //   acc += fy (gx q[iy0, ix1] + fx q[iy0, ix0]) + gy (gx q[iy1, ix1] + fx q[iy1, ix0]),
 // gx = 1 - fx, gy = 1 - fy; the p.Ap partial of a thread sums its nodes.
-template <bool LHS, typename RT, int CS>
+template <bool LHS, typename RT, int CS, int LT>
 __global__ __launch_bounds__(128) void k_layer_m8(KARGS, int psel_, int osel, int partsel) {
     KJOB;
     const RT *q = (const RT *)PD.q;
@@ -997,22 +997,22 @@ __global__ __launch_bounds__(128) void k_layer_m8(KARGS, int psel_, int osel, in
     RT *out = rsel<RT>(PD, osel);
     double *part = psel(PD, partsel);
     const int partstride = PD.SC.maxpart;
-    constexpr int R = 4, NT = 128;
-    constexpr int PW = TILE + 4, PHH = LTY + 4, P2 = PW * PHH;  // p window (L2'L2 reach 2)
+    constexpr int R = LT / 16, NT = 128;
+    constexpr int PW = TILE + 4, PHH = LT + 4, P2 = PW * PHH;  // p window (L2'L2 reach 2)
     constexpr int UP = (P2 + NT - 1) / NT;
     __shared__ float ps[CS][LHS ? P2 : 1];
     __shared__ double sh[8];
     const int l = by, s0 = J.s0 + bz * CS, ns = min(CS, J.s0 + J.nsys - s0);
     const int nxl = G.xnx[l], nyl = G.xny[l];
-    const int ntx = (nxl + TILE - 1) / TILE, nty = (nyl + LTY - 1) / LTY;
+    const int ntx = (nxl + TILE - 1) / TILE, nty = (nyl + LT - 1) / LT;
     const bool active = bx < ntx * nty;
-    const int jx0 = (bx % ntx) * TILE, jy0 = (bx / ntx) * LTY;
+    const int jx0 = (bx % ntx) * TILE, jy0 = (bx / ntx) * LT;
     const int tx = threadIdx.x % (TILE / 2), ty = threadIdx.x / (TILE / 2);
     const int jx = jx0 + 2 * tx, jyb = jy0 + R * ty;
     const int nr = active ? max(0, min(R, nyl - jyb)) : 0;  // rows of this thread
     const int nc = active ? max(0, min(2, nxl - jx)) : 0;   // columns of this thread
     if (LHS && active) {
-        // p windows, periodic (layers are >= LTY + 2 rows and TILE + 2 columns)
+        // p windows, periodic (layers are >= LT + 2 rows and TILE + 2 columns)
         int off[UP];
 #pragma unroll
         for (int u = 0; u < UP; u++) {
@@ -1043,7 +1043,7 @@ __global__ __launch_bounds__(128) void k_layer_m8(KARGS, int psel_, int osel, in
         for (int c = 0; c < 2; c++)
 #pragma unroll
             for (int r = 0; r < R; r++) acc[si][c][r] = 0;
-    const int tw = min(TILE, nxl - jx0), th = min(LTY, nyl - jy0);
+    const int tw = min(TILE, nxl - jx0), th = min(LT, nyl - jy0);
     // Node jx takes rays ix0 = jx - cx - 1 (the last ray for node 0 on a
     // wrapped axis) and ix1 = jx - cx; node jx + 1 takes ix1 and ix1 + 1.
     // Sample rows k = 0..R: iy = jyb - cy - 1 + k (row 0: the last ray when
@@ -1925,9 +1925,14 @@ static void launch_layer_rt(const LaunchTab &t, int matched, int cs, int psel_, 
     if (!matched) hipLaunchKernelGGL((k_layer<LHS, RT>), grid, blk, 0, st, KT, psel_, osel, partsel);
     else if (matched == 2) {  // 16 x 64 tiles
         const dim3 b8(128);
-        if (cs == 4) hipLaunchKernelGGL((k_layer_m8<LHS, RT, 4>), grid, b8, 0, st, KT, psel_, osel, partsel);
-        else if (cs == 2) hipLaunchKernelGGL((k_layer_m8<LHS, RT, 2>), grid, b8, 0, st, KT, psel_, osel, partsel);
-        else hipLaunchKernelGGL((k_layer_m8<LHS, RT, 1>), grid, b8, 0, st, KT, psel_, osel, partsel);
+        if (cs == 4) hipLaunchKernelGGL((k_layer_m8<LHS, RT, 4, LTY>), grid, b8, 0, st, KT, psel_, osel, partsel);
+        else if (cs == 2) hipLaunchKernelGGL((k_layer_m8<LHS, RT, 2, LTY>), grid, b8, 0, st, KT, psel_, osel, partsel);
+        else hipLaunchKernelGGL((k_layer_m8<LHS, RT, 1, LTY>), grid, b8, 0, st, KT, psel_, osel, partsel);
+    } else if (matched == 3) {  // 16 x 32 tiles
+        const dim3 b8(128);
+        if (cs == 4) hipLaunchKernelGGL((k_layer_m8<LHS, RT, 4, LTY2>), grid, b8, 0, st, KT, psel_, osel, partsel);
+        else if (cs == 2) hipLaunchKernelGGL((k_layer_m8<LHS, RT, 2, LTY2>), grid, b8, 0, st, KT, psel_, osel, partsel);
+        else hipLaunchKernelGGL((k_layer_m8<LHS, RT, 1, LTY2>), grid, b8, 0, st, KT, psel_, osel, partsel);
     }
     else if (cs == 4) hipLaunchKernelGGL((k_layer_m<LHS, RT, 4>), grid, blk, 0, st, KT, psel_, osel, partsel);
     else if (cs == 2) hipLaunchKernelGGL((k_layer_m<LHS, RT, 2>), grid, blk, 0, st, KT, psel_, osel, partsel);

@@ -27,6 +27,7 @@ constexpr int TILE = 16;      // output tile edge of the WFS / layer passes
 constexpr int TAPS = 4;       // max ploc samples landing on one layer node per axis
 constexpr int ROWS = 16;      // rows per row-FFT workgroup (8 complex FFTs)
 constexpr int LTY = 64;       // tile height of the 16 x 64 layer kernel (k_layer_m8)
+constexpr int LTY2 = 32;      // tile height of its 16 x 32 form
 constexpr int MAXN = 1024;    // largest FFT length (layer grid edge)
 
 // Matched-sampling ray of one (WFS, layer) pair: ploc index i maps to layer
@@ -68,7 +69,7 @@ struct Geom {
     const int4 *hxt_x, *hxt_y;    // per (w,l) per layer index: up to 4 ploc indices (-1 pad)
     long hxt_xoff[MAXL + 1];      // offset of layer l in a (w,*) transposed x table
     int matched;                  // every (w,l) ray is matched sampling (fast path)
-    int lty;                      // layer tile height: TILE, or LTY (k_layer_m8)
+    int lty;                      // layer tile height: TILE, or LTY / LTY2 (k_layer_m8)
     const MRay *mray;             // [w*MAXL + l]
     // subapertures
     const int *saidx;             // per WFS per ploc point: subaperture with that lower-left corner, -1

@@ -188,6 +188,7 @@ struct gpu_tomo {
     int tune_colfft = 0;  // column kernel: 0 = registers where available, 1 = registers, 2 = LDS stages
     bool colr_ok = false; // the register column kernel applies (per-frequency blocks, ny = shL x shP)
     bool lty_ok = false;  // LTY-row tiles fit every layer of this (matched) plan
+    bool lty2_ok = false; // LTY2-row tiles do
     // profiling
     int prof = 0;
     std::vector<ProfEv> evs;  // profiling events
@@ -611,7 +612,7 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
                 matched = 0;
         }
     // the matched kernels wrap the L2 window once: layers of at least TILE + 2
-    // (16 x 16 tiles) or LTY + 2 rows (16 x 64 tiles)
+    // (16 x 16 tiles), LTY2 + 2 (16 x 32) or LTY + 2 rows (16 x 64 tiles)
     G.lty = TILE;
     for (int l = 0; l < R->npsr; l++)
         if (R->xnx[l] < TILE + 2 || R->xny[l] < TILE + 2) matched = 0;
@@ -621,6 +622,9 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
             if (R->xny[l] < LTY + 2) G.lty = TILE;
     }
     g->lty_ok = G.lty == LTY;
+    g->lty2_ok = matched != 0;
+    for (int l = 0; l < R->npsr; l++)
+        if (R->xny[l] < LTY2 + 2) g->lty2_ok = false;
     G.matched = matched;
     g->mray.upload(hm);
     G.mray = g->mray.p;
@@ -910,7 +914,10 @@ void drop_batches(gpu_tomo *g);
 // partial counts follow; launch tables are rebuilt on next use).
 void apply_lty(gpu_tomo *g) {
     if (!g->G.matched) return;
-    const int lty = g->tune_lty == TILE ? TILE : g->lty_ok ? LTY : TILE;
+    const int lty = g->tune_lty == TILE   ? TILE
+                    : g->tune_lty == LTY2 ? (g->lty2_ok ? LTY2 : TILE)
+                    : g->lty_ok           ? LTY
+                                          : TILE;
     if (lty == g->G.lty) return;
     g->G.lty = lty;
     g->SC.npart_layer = layer_tiles(g->G) * g->G.npsr;
@@ -1200,7 +1207,7 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
                 });
     build_stage(b, b.layer,
                 [&](int j, std::vector<int> &k) {
-                    k = {G(j).matched ? (G(j).lty == LTY ? 2 : 1) : 0};
+                    k = {G(j).matched ? (G(j).lty == LTY ? 2 : G(j).lty == LTY2 ? 3 : 1) : 0};
                     return true;
                 },
                 [&](const Sub &sb, int j) {
@@ -2015,7 +2022,7 @@ int gpu_tomo_group_autotune(gpu_tomo_group_t *grp, int *choice) {
             }
         };
         trial(4, {2});
-        trial(3, {TILE});
+        trial(3, {LTY2, TILE});
         trial(0, {1, 2, 4});
         trial(1, {1, 2, 4});
         (void)hipEventDestroy(e0);
@@ -2039,7 +2046,7 @@ int gpu_tomo_tune_set(gpu_tomo_t *g, const int *choice) {
     if (!g || !choice) return MT_EINVAL;
     for (int i = 0; i < 3; i++)
         if (choice[i] != 0 && choice[i] != 1 && choice[i] != 2 && choice[i] != 4) return MT_EINVAL;
-    if (choice[3] != 0 && choice[3] != TILE && choice[3] != LTY) return MT_EINVAL;
+    if (choice[3] != 0 && choice[3] != TILE && choice[3] != LTY2 && choice[3] != LTY) return MT_EINVAL;
     if (choice[4] < 0 || choice[4] > 2) return MT_EINVAL;
     if (choice[5] != 0 && choice[5] != 1 && choice[5] != 2 && choice[5] != 4) return MT_EINVAL;
     return guarded([&] {
@@ -2089,13 +2096,14 @@ int gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice) {
             int *field, kc, cmax;
         };
         const int colmax = 4;  // clamped by the LDS per workgroup when the launch is built
-        // the layer stage: tile shape (16 x 64 or 16 x 16, where both fit)
-        // and systems per workgroup together
+        // the layer stage: tile shape (16 x 64, 16 x 32 or 16 x 16, where
+        // they fit) and systems per workgroup together
         {
             double best = 1e30;
             int bestl = g->G.lty, bestc = 1;
-            for (int lty : {LTY, TILE}) {
+            for (int lty : {LTY, LTY2, TILE}) {
                 if (lty == LTY && !(g->G.matched && g->lty_ok)) continue;
+                if (lty == LTY2 && !(g->G.matched && g->lty2_ok)) continue;
                 g->tune_lty = lty;
                 apply_lty(g);
                 for (int c = 1; c <= 4; c <<= 1) {

@@ -176,7 +176,7 @@ class GpuTomo:
 
     def tune_set(self, wfs, layer, col, layer_rows=0, col_fft=0, flights=0):
         """gpu_tomo_tune_set: fix the systems per workgroup (0: heuristic),
-        the layer tile rows (16, 64; 0: 64 where it fits), the FFT kernels
+        the layer tile rows (16, 32, 64; 0: 64 where it fits), the FFT kernels
         (1 registers where they apply, 2 LDS stages for columns and rows;
         0: as 1) and the
         flights of a graph solve (1, 2, 4; 0: 1)."""

@@ -16,24 +16,26 @@ def _batch(r, cfg, n):
     return np.stack([workload.make_inputs(cfg, r, 40 + s) for s in range(n)])
 
 
+@pytest.mark.parametrize("rows", [0, 32])
 @pytest.mark.parametrize("colk", [1, 2])
 @pytest.mark.parametrize("mode", [0, 1, 3])
-def test_systems_per_workgroup_bit_exact(large30, mode, colk):
+def test_systems_per_workgroup_bit_exact(large30, mode, colk, rows):
     """Five systems (one partial workgroup for every choice above 1) solved
     under each systems-per-workgroup choice and by the heuristic, with the
     register column kernel (1, which has no such choice) and the LDS column
-    kernel (2): identical solutions and residuals."""
+    kernel (2), on 16 x 64 (0: the default) and 16 x 32 layer tiles:
+    identical solutions and residuals."""
     cfg, r = large30
     grads = _batch(r, cfg, 5)
     g = tomo.GpuTomo(r, nbatch=5, maxit=cfg["maxit"])
     g.precision(mode)
-    g.tune_set(0, 0, 0, 0, colk)
+    g.tune_set(0, 0, 0, rows, colk)
     ref = g.tomo_batch(grads)
     ref_res = g.residuals(5)
     # (column 2 with an FP32 spectrum once launched the 4-system kernel on a
     # grid and LDS sized for 2)
     for choice in [(1, 1, 1), (2, 2, 2), (4, 4, 2), (2, 1, 2), (1, 1, 2), (1, 1, 4)]:
-        g.tune_set(*choice, 0, colk)
+        g.tune_set(*choice, rows, colk)
         x = g.tomo_batch(grads)
         print("choice", choice, "max |dx|", float(np.abs(x - ref).max()), flush=True)
         assert np.array_equal(x, ref), "systems per workgroup %s changed the solution" % (choice,)
@@ -47,7 +49,7 @@ def test_tune_set_rejects_bad_choice(large30):
     with pytest.raises(_abi.MtError):
         g.tune_set(3, 1, 1)  # 3 is not a choice
     with pytest.raises(_abi.MtError):
-        g.tune_set(1, 1, 1, 32)  # nor are 32-row tiles
+        g.tune_set(1, 1, 1, 48)  # nor are 48-row tiles
     with pytest.raises(_abi.MtError):
         g.tune_set(1, 1, 1, 0, 3)  # nor a third column kernel
     with pytest.raises(_abi.MtError):
@@ -66,7 +68,7 @@ def test_autotune_replay(large30):
     ref = g.tomo_batch(grads)
     choice = g.autotune(3)
     print("autotuned", choice, flush=True)
-    assert all(c in (1, 2, 4) for c in choice[:2]) and choice[3] in (16, 64) and choice[4] in (1, 2)
+    assert all(c in (1, 2, 4) for c in choice[:2]) and choice[3] in (16, 32, 64) and choice[4] in (1, 2)
     assert choice[2] in ((0,) if choice[4] == 1 else (1, 2, 4)) and choice[5] in (1, 2)
     x = g.tomo_batch(grads)
     assert np.linalg.norm(x - ref) <= 1e-5 * np.linalg.norm(ref)
@@ -77,9 +79,9 @@ def test_autotune_replay(large30):
     g2.close()
 
 
-@pytest.mark.parametrize("rows", [16, 64])
+@pytest.mark.parametrize("rows", [16, 32, 64])
 def test_layer_tile_shape_parity(large30, rows):
-    """Both layer tile shapes against the FP64 oracle."""
+    """Each layer tile shape against the FP64 oracle."""
     from oracle.oracle import Oracle
     cfg, r = large30
     grad = workload.make_inputs(cfg, r, 1)
