@@ -54,3 +54,15 @@ def test_warm_plan_gives_cold_columns():
     e = float(np.linalg.norm(a - b) / np.linalg.norm(a))
     print("warm plan vs cold plan: rel L2 %.3e" % e)
     assert e <= 1e-6
+
+
+def test_rejects_bad_ranges():
+    from maos_amd import _abi
+    cfg = configs.small8(maxit=5)
+    r = tomo.Recon(cfg)
+    g = tomo.GpuTomo(r, nbatch=4, maxit=5)
+    for col0, ncol in ((0, 0), (-1, 4), (r.ng - 2, 3), (r.ng, 1)):
+        with pytest.raises(_abi.MtError):
+            g.mvm_tomo(col0, ncol)
+    assert g.mvm_tomo(r.ng - 3, 3).shape == (r.nx, 3)  # the last columns
+    g.close()
