@@ -1813,7 +1813,13 @@ int gpu_tomo_mvm_tomo(gpu_tomo_t *g, long col0, long ncol, double *out) {
                 const int n = (int)std::min<long>(B, (long)idx.size() - c);
                 launch_unit(g->g.p, G.ngtot, didx.p + c, n, st);
                 if (g->warm) launch_unit(g->x.p, G.nxtot, nullptr, n, st);  // cold start
-                graph_solve(g, n, st);
+                if (!g->prof) {
+                    graph_solve(g, n, st);
+                } else {  // profiled plans launch kernel by kernel, as gpu_tomo_solve_dev does
+                    Rec rec{&g->evs, st, true, g->prof};
+                    enqueue_solve(own_batch(g, 0, n), rec);
+                    prof_collect(g);
+                }
                 HIPCHK(hipMemcpy2DAsync(h[t % 2], sizeof(float) * nx, g->x.p, sizeof(float) * G.nxtot,
                                         sizeof(float) * nx, n, hipMemcpyDeviceToHost, st));
                 HIPCHK(hipEventRecord(ev[t % 2], st));

@@ -66,3 +66,17 @@ def test_rejects_bad_ranges():
             g.mvm_tomo(col0, ncol)
     assert g.mvm_tomo(r.ng - 3, 3).shape == (r.nx, 3)  # the last columns
     g.close()
+
+
+def test_profiled_plan_gives_same_columns():
+    """With per-kernel profiling on, the columns come from kernel-by-kernel
+    launches instead of a graph replay: the same bits."""
+    cfg = configs.small8(maxit=10)
+    r = tomo.Recon(cfg)
+    g = tomo.GpuTomo(r, nbatch=8, maxit=10)
+    a = g.mvm_tomo(0, 12)
+    g.profile(True)
+    b = g.mvm_tomo(0, 12)
+    g.profile(False)
+    assert np.array_equal(a, b)
+    g.close()
