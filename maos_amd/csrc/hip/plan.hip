@@ -77,7 +77,6 @@ struct DevBuf {
     ~DevBuf() { free(); }
 };
 
-// Kernel classes timed by the profiler.
 // Kernel classes timed by the profiler: the iteration kernels (k_wfs ..
 // k_irow, the UPDATE-iteration forms) and the once-per-solve forms apart.
 enum KClass {
