@@ -207,8 +207,12 @@ typedef struct gpu_tomo gpu_tomo_t;
  * Replaces gpu_setup_recon (cuda/recon.cu:477). */
 int  gpu_tomo_setup(const mt_recon_t *recon, int device, int nbatch,
                     int maxit, int warm, gpu_tomo_t **out);
-/* Re-upload L2 coefficients and FDPCG blocks after a Cn2 update.
- * Replaces gpu_update_recon_cn2 (cuda/recon.cu:526). */
+/* Re-upload L2 coefficients and FDPCG blocks after a Cn2 update (both
+ * copies of the blocks: the LDS column kernel's and the register column
+ * kernel's), in place, so batches and captured graphs stay valid.  Waits for
+ * the plan's own streams only; no solve of this plan may be in flight on a
+ * caller's stream.  Replaces gpu_update_recon_cn2 (cuda/recon.cu:526,
+ * curecon_t::update_cn2 cuda/recon.cu:231-250). */
 int  gpu_tomo_update_cn2(gpu_tomo_t *g, const mt_recon_t *recon);
 /* Zero the warm-start state. Replaces gpu_recon_reset (cuda/recon.cu:538). */
 int  gpu_tomo_reset(gpu_tomo_t *g);
@@ -246,7 +250,8 @@ int  gpu_tomo_solve_dev(gpu_tomo_t *g, const float *d_grad, float *d_x,
  * start, maxit iterations, nbatch columns per graph solve), into out
  * (host, nx x ncol doubles, column-major: column k = solution of e_{col0+k}
  * in gpu_tomo_solve's layer layout).  Columns of WFS the solve does not use
- * are zero.  Overwrites the gradients and solutions of the plan's slots.
+ * are zero.  The gradients and solutions of the plan's slots (the warm-start
+ * state) are kept: the next gpu_tomo_solve starts where it would have.
  * MAOS assembles its MVM by solving tomography and fitting for every
  * gradient column, in batches (mvm_direct, cuda/mvm_direct.cu:132); this is
  * the tomography part. */
@@ -286,7 +291,10 @@ int  gpu_tomo_get_x(gpu_tomo_t *g, int isys, double *x, double *cgres);
  * geometry (aperture, WFS and layer counts), solved in lockstep with one
  * launch per kernel per iteration covering all of them; every workgroup
  * finds its plan and block in a table.  Plans must share the device,
- * precision mode, maxit and warm flag.  Inputs are the plans' gradient
+ * precision mode, maxit, warm flag and preconditioner.  A member whose
+ * launch shape or precision changes later (gpu_tomo_tune_set,
+ * gpu_tomo_precision, gpu_tomo_autotune) makes the group rebuild its tables
+ * and graph at its next solve.  Inputs are the plans' gradient
  * slots (gpu_tomo_set_grad or gpu_tomo_solve_dev's buffers); results stay in
  * the plans (gpu_tomo_get_x).  mode: 0 = launches, 1 = replay a captured
  * hipGraph.  stream 0: the group's own stream, synchronised on return.
@@ -338,6 +346,15 @@ int  gpu_tomo_server_get_x(gpu_tomo_server_t *s, int loop, double *x,
                            double *cgres);
 int  gpu_tomo_server_usage(gpu_tomo_server_t *s, double *used_gb,
                            double *budget_gb);
+/* New turbulence profile of one served loop (recon from
+ * mt_setup_update_cn2 of that loop's setup): the loop is held (its requests
+ * wait, every other loop keeps being served), the flight that carries it, if
+ * any, is waited for, and its plan's regulariser and FDPCG blocks are
+ * replaced as gpu_tomo_update_cn2 does; its warm-start solution is kept.
+ * The profile-update path of gpu_update_recon_cn2 (cuda/recon.cu:526) for a
+ * loop of the request loop, which has no counterpart in MAOS. */
+int  gpu_tomo_server_update_cn2(gpu_tomo_server_t *s, int loop,
+                                const mt_recon_t *recon);
 /* Autotuner for a server: gpu_tomo_group_autotune over the admitted loops
  * (one system each, as a batch carries them), the result set on every
  * loop's plan and the cached batch graphs captured again before their next

@@ -121,6 +121,7 @@ def lib():
     L.gpu_tomo_server_get_x.argtypes = [vp, C.c_int, dp, dp]
     L.gpu_tomo_server_usage.argtypes = [vp, dp, dp]
     L.gpu_tomo_server_autotune.argtypes = [vp, ip]
+    L.gpu_tomo_server_update_cn2.argtypes = [vp, C.c_int, C.POINTER(MtRecon)]
     L.gpu_tomo_server_free.argtypes = [vp]
     L.gpu_tomo_server_free.restype = None
     L.gpu_tomo_solve_dev.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp]
@@ -163,7 +164,7 @@ EXPORTS = [
     "gpu_tomo_group_create", "gpu_tomo_group_solve", "gpu_tomo_group_autotune", "gpu_tomo_group_free",
     "gpu_tomo_server_create", "gpu_tomo_server_add_loop", "gpu_tomo_server_remove_loop",
     "gpu_tomo_server_submit", "gpu_tomo_server_poll", "gpu_tomo_server_drain", "gpu_tomo_server_get_x",
-    "gpu_tomo_server_usage", "gpu_tomo_server_autotune", "gpu_tomo_server_free", "gpu_tomo_autotune", "gpu_tomo_tune_set", "gpu_tomo_info", "gpu_tomo_residuals", "gpu_tomo_L", "gpu_tomo_R", "gpu_tomo_P",
+    "gpu_tomo_server_usage", "gpu_tomo_server_autotune", "gpu_tomo_server_update_cn2", "gpu_tomo_server_free", "gpu_tomo_autotune", "gpu_tomo_tune_set", "gpu_tomo_info", "gpu_tomo_residuals", "gpu_tomo_L", "gpu_tomo_R", "gpu_tomo_P",
     "gpu_tomo_sizes", "gpu_tomo_profile", "gpu_tomo_profile_read", "gpu_tomo_precision",
     "mt_bin_read", "mt_bin_write", "mt_bin_free", "mt_bin_info", "mt_bin_data", "mt_bin_child",
     "mt_bin_header", "mt_bin_new_dbl", "mt_bin_new_cell", "mt_bin_set", "mt_bin_grad", "mt_bin_opdr",

@@ -479,16 +479,110 @@ __global__ __launch_bounds__(256) void k_ttr(KARGS) {
     }
 }
 
+// TT projection of the LHS gradients (TTFR inside TomoL when the LHS removes
+// TT, maos/recon_utils.c:461-465): ttl = PTT * (G HX p) per (system, WFS),
+// from the gradients k_wfsg wrote, kept in FP64.
+template <typename RT>
+__global__ __launch_bounds__(256) void k_ttl(KARGS) {
+    KJOB;
+    __shared__ double sh[8];
+    const int w = bx, s = J.s0 + bz;
+    double *out = PD.ttl + ((long)s * G.nwfs + w) * 2;
+    if (!G.ttr[w]) {
+        if (threadIdx.x == 0) out[0] = out[1] = 0.;
+        return;
+    }
+    const int nsa = G.nsa[w];
+    const RT *g = (const RT *)PD.gl + (long)s * G.ngtot + G.goff[w];
+    const float *P = G.ptt + 4 * G.saoff[w];
+    double a0 = 0, a1 = 0;
+    for (int i = threadIdx.x; i < 2 * nsa; i += blockDim.x) {
+        a0 += (double)P[2 * i] * (double)g[i];
+        a1 += (double)P[2 * i + 1] * (double)g[i];
+    }
+    a0 = block_sum(a0, sh);
+    a1 = block_sum(a1, sh);
+    if (threadIdx.x == 0) {
+        out[0] = a0;
+        out[1] = a1;
+    }
+}
+
+// Gradients of the LHS when it removes TT: gl = GP HX p for the subapertures
+// whose lower-left corner lies in this workgroup's TILE x TILE tile of one
+// WFS (each subaperture is written by exactly one workgroup).  The TT
+// projection (k_ttl) and W GP' (k_wfs<RHS, GL>) follow.
+template <int POS, typename RT>
+__global__ __launch_bounds__(256) void k_wfsg(KARGS, int xsel) {
+    KJOB;
+    const float *x = vsel(PD, xsel);
+    RT *gl = (RT *)PD.gl;
+    constexpr int H = TILE + 2 * POS;
+    constexpr int NP = POS + 1;
+    __shared__ RT opd[H * H];
+    const int w = by, s = J.s0 + bz;
+    const int pnx = G.pnx, pny = G.pny;
+    const int ntx = (pnx + TILE - 1) / TILE;
+    const int tx0 = (bx % ntx) * TILE, ty0 = (bx / ntx) * TILE;
+    const float *xs = x + (long)s * G.nxtot;
+    for (int i = threadIdx.x; i < H * H; i += blockDim.x) {
+        const int px = tx0 - POS + i % H, py = ty0 - POS + i / H;
+        RT v = 0;
+        if (px >= 0 && px < pnx && py >= 0 && py < pny) {
+            for (int l = 0; l < G.npsr; l++) {
+                const int wl = w * MAXL + l;
+                const int2 nx_ = G.hx_nx[(long)wl * pnx + px];
+                const int2 ny_ = G.hx_ny[(long)wl * pny + py];
+                if (nx_.x >= 0 && ny_.x >= 0) {
+                    const RT fx = G.hx_fx[(long)wl * pnx + px];
+                    const RT fy = G.hx_fy[(long)wl * pny + py];
+                    const float *xl = xs + G.xoff[l];
+                    const int nxl = G.xnx[l];
+                    const RT a = xl[ny_.x * nxl + nx_.x], b = xl[ny_.x * nxl + nx_.y];
+                    const RT c = xl[ny_.y * nxl + nx_.x], d = xl[ny_.y * nxl + nx_.y];
+                    v += (RT(1) - fy) * ((RT(1) - fx) * a + fx * b) + fy * ((RT(1) - fx) * c + fx * d);
+                }
+            }
+        }
+        opd[i] = v;
+    }
+    __syncthreads();
+    const int ox = threadIdx.x % TILE, oy = threadIdx.x / TILE;
+    const int cx = tx0 + ox, cy = ty0 + oy;
+    if (threadIdx.x < TILE * TILE && cx < pnx && cy < pny) {
+        const int sa = G.saidx[(long)w * G.nploc + cy * pnx + cx];
+        if (sa >= 0) {
+            const long so = G.saoff[w] + sa;
+            const float *cxp = G.gpx + so * NP * NP, *cyp = G.gpy + so * NP * NP;
+            RT gx = 0, gy = 0;
+#pragma unroll
+            for (int zy = 0; zy < NP; zy++)
+#pragma unroll
+                for (int zx = 0; zx < NP; zx++) {
+                    const RT o = opd[(oy + POS + zy) * H + ox + POS + zx];
+                    gx += (RT)cxp[zy * NP + zx] * o;
+                    gy += (RT)cyp[zy * NP + zx] * o;
+                }
+            RT *gw = gl + (long)s * G.ngtot + G.goff[w];
+            gw[sa] = gx;
+            gw[G.nsa[w] + sa] = gy;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // WFS-side pass.  One workgroup = one TILE x TILE tile of ploc points of one
 // WFS.  LHS: opd = HX p on the tile + halo (LDS), g = GP opd, W g, then
 // q = GP' (W g) gathered from the <= (pos+1)^2 subapertures touching each
-// point.  RHS: W (g - TT tt) then the same GP' gather.
+// point.  RHS: W (g - TT tt) then the same GP' gather; with GL the
+// gradients are the LHS's own (gl, ttl from k_wfsg and k_ttl) instead of the
+// measured ones (g, tt).
 // ---------------------------------------------------------------------------
-template <int POS, bool RHS, typename RT>
+template <int POS, bool RHS, typename RT, bool GL = false>
 __global__ __launch_bounds__(256) void k_wfs(KARGS, int xsel) {
     KJOB;
     const float *x = vsel(PD, xsel), *grad = PD.g, *tt = PD.tt;
+    const RT *gradl = (const RT *)PD.gl;
     RT *q = (RT *)PD.q;
     constexpr int H = TILE + 2 * POS;
     constexpr int S = TILE + POS;
@@ -527,10 +621,16 @@ __global__ __launch_bounds__(256) void k_wfs(KARGS, int xsel) {
     }
     const int nsa = G.nsa[w];
     const float *gw = grad + (long)s * G.ngtot + G.goff[w];
+    const RT *gwl = GL ? gradl + (long)s * G.ngtot + G.goff[w] : nullptr;
     RT ttx = 0, tty = 0;
     if (RHS && G.ttr[w]) {
-        ttx = tt[((long)s * G.nwfs + w) * 2];
-        tty = tt[((long)s * G.nwfs + w) * 2 + 1];
+        if (GL) {
+            ttx = (RT)PD.ttl[((long)s * G.nwfs + w) * 2];
+            tty = (RT)PD.ttl[((long)s * G.nwfs + w) * 2 + 1];
+        } else {
+            ttx = tt[((long)s * G.nwfs + w) * 2];
+            tty = tt[((long)s * G.nwfs + w) * 2 + 1];
+        }
     }
     for (int i = threadIdx.x; i < S * S; i += blockDim.x) {
         const int a = i % S, b = i / S;
@@ -542,7 +642,10 @@ __global__ __launch_bounds__(256) void k_wfs(KARGS, int xsel) {
         if (sa >= 0) {
             const long so = G.saoff[w] + sa;
             RT gx = 0, gy = 0;
-            if (RHS) {
+            if (RHS && GL) {
+                gx = gwl[sa] - ttx;
+                gy = gwl[nsa + sa] - tty;
+            } else if (RHS) {
                 gx = (RT)gw[sa] - ttx;
                 gy = (RT)gw[nsa + sa] - tty;
             } else {
@@ -1828,6 +1931,100 @@ __global__ __launch_bounds__(256) void k_irow(KARGS, int slot, int osel) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Plain CG (tomo.precond=0: lib/pcg.c:28-113 with Mmul = NULL, so z = r),
+// over the row blocks of the layers (rowblk: layer, first row; ROWS rows of
+// that layer's width), so the r.r partials have the row kernels' count:
+//   k_cgv  INIT_COLD  r = b, x = 0                      r.r partial
+//          INIT_WARM  r = b - A x (Ap holds A x)        r.r partial
+//          UPDATE     alpha = rz[iter] / p.Ap;
+//                     x += alpha p; r -= alpha Ap       r.r partial
+//          LAST       the same (r.r of the final residual)
+//   k_cgp  FIRST      rz[slot] = r.r; p = r             (start / restart)
+//          UPDATE     beta = r.r / rz[slot]; rz[slot+1] = r.r; p = r + beta p
+// alpha, beta and the sums are FP64 and reduced in fixed order (the same
+// value in every workgroup); a non-finite alpha stops the system as k_row
+// does (lib/pcg.c:74-77).
+// ---------------------------------------------------------------------------
+enum { CGV_INIT_COLD = 0, CGV_INIT_WARM = 1, CGV_UPDATE = 2, CGV_LAST = 3 };
+template <int MODE, typename RT>
+__global__ __launch_bounds__(256) void k_cgv(KARGS, int iter) {
+    KJOB;
+    const Scal SC = ldsc(PD.SC);
+    float *x = PD.x;
+    const float *p = PD.p;
+    RT *r = (RT *)PD.r;
+    const RT *b = (const RT *)PD.b, *Ap = (const RT *)PD.Ap;
+    int *stop = PD.stop;
+    __shared__ double sh[8];
+    const int s = J.s0 + bz;
+    const int2 rb = G.rowblk[bx];
+    const int l = rb.x, row0 = rb.y;
+    const int n = G.xnx[l];
+    const int tot = min(ROWS, G.xny[l] - row0) * n;
+    const long base = (long)s * G.nxtot + G.xoff[l] + (long)row0 * n;
+    double alpha = 0.;
+    if (MODE >= CGV_UPDATE) {
+        const double pap = reduce_partials(SC.part_pap + (long)s * SC.maxpart, SC.npart_layer, sh, G.dot32);
+        const double rz = SC.rz[(long)s * (SC.maxit + 2) + iter];
+        double a = ddiv(rz, pap, G.dot32);
+        const int stopped = stop[s];
+        if (!isfinite(a) || stopped) {
+            a = 0;
+            if (!stopped && threadIdx.x == 0 && bx == 0) stop[s] = 1;
+        }
+        alpha = a;
+    }
+    double dot = 0;
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+        const long idx = base + i;
+        RT rv;
+        if (MODE == CGV_INIT_COLD) {
+            rv = b[idx];
+            x[idx] = 0.f;
+        } else if (MODE == CGV_INIT_WARM) {
+            rv = b[idx] - Ap[idx];
+        } else {
+            const float pv = p[idx];
+            x[idx] = x[idx] + (float)alpha * pv;
+            rv = r[idx] - (RT)alpha * Ap[idx];
+        }
+        r[idx] = rv;
+        dot = dfma(rv, rv, dot, G.dot32);
+    }
+    dot = block_sum(dot, sh, G.dot32);
+    if (threadIdx.x == 0) SC.part_rr[(long)s * SC.maxpart + bx] = dot;
+}
+enum { CGP_FIRST = 0, CGP_UPDATE = 1 };
+template <int MODE, typename RT>
+__global__ __launch_bounds__(256) void k_cgp(KARGS, int slot) {
+    KJOB;
+    const Scal SC = ldsc(PD.SC);
+    float *p = PD.p;
+    const RT *r = (const RT *)PD.r;
+    __shared__ double sh[8];
+    const int s = J.s0 + bz;
+    const int2 rb = G.rowblk[bx];
+    const int l = rb.x, row0 = rb.y;
+    const int n = G.xnx[l];
+    const int tot = min(ROWS, G.xny[l] - row0) * n;
+    const long base = (long)s * G.nxtot + G.xoff[l] + (long)row0 * n;
+    const double rr = reduce_partials(SC.part_rr + (long)s * SC.maxpart, SC.npart_row, sh, G.dot32);
+    double *hist = SC.rz + (long)s * (SC.maxit + 2);
+    double beta = 0;
+    if (MODE == CGP_UPDATE) {
+        beta = ddiv(rr, hist[slot], G.dot32);
+        if (!isfinite(beta)) beta = 0;
+    }
+    if (bx == 0 && threadIdx.x == 0) hist[MODE == CGP_FIRST ? slot : slot + 1] = rr;
+    const float fb = (float)beta;
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+        const long idx = base + i;
+        const float rv = (float)r[idx];
+        p[idx] = MODE == CGP_FIRST ? rv : rv + fb * p[idx];
+    }
+}
+
 // Residual sqrt(r.r / b.b) per system (NaN when the solve stopped early).
 __global__ __launch_bounds__(256) void k_final(KARGS) {
     KJOB;
@@ -2085,6 +2282,43 @@ void launch_irow(const LaunchTab &t, int mode, int rm, int rp, int slot, int ose
     else if (rp == 3) launch_irow_r<3, 3>(t, mode, slot, osel, lds, st);
     else if (rp == 2) launch_irow_r<3, 2>(t, mode, slot, osel, lds, st);
     else launch_irow_r<3, 0>(t, mode, slot, osel, lds, st);
+}
+template <typename RT>
+static void launch_wfs_lhs_tt_rt(const LaunchTab &tg, const LaunchTab &tt, int pos, int xsel, hipStream_t st) {
+    const dim3 blk(256);
+    if (pos == 1) hipLaunchKernelGGL((k_wfsg<1, RT>), dim3(tg.nwg), blk, 0, st, tg.jobs, tg.wgi, xsel);
+    else hipLaunchKernelGGL((k_wfsg<2, RT>), dim3(tg.nwg), blk, 0, st, tg.jobs, tg.wgi, xsel);
+    hipLaunchKernelGGL((k_ttl<RT>), dim3(tt.nwg), blk, 0, st, tt.jobs, tt.wgi);
+    if (pos == 1) hipLaunchKernelGGL((k_wfs<1, true, RT, true>), dim3(tg.nwg), blk, 0, st, tg.jobs, tg.wgi, xsel);
+    else hipLaunchKernelGGL((k_wfs<2, true, RT, true>), dim3(tg.nwg), blk, 0, st, tg.jobs, tg.wgi, xsel);
+}
+void launch_wfs_lhs_tt(const LaunchTab &tg, const LaunchTab &tt, int pos, int rt64, int xsel, hipStream_t st) {
+    if (rt64) launch_wfs_lhs_tt_rt<double>(tg, tt, pos, xsel, st);
+    else launch_wfs_lhs_tt_rt<float>(tg, tt, pos, xsel, st);
+}
+template <typename RT>
+static void launch_cgv_rt(const LaunchTab &t, int mode, int iter, hipStream_t st) {
+    const dim3 grid(t.nwg), blk(256);
+    switch (mode) {
+        case CGV_INIT_COLD: hipLaunchKernelGGL((k_cgv<CGV_INIT_COLD, RT>), grid, blk, 0, st, KT, iter); break;
+        case CGV_INIT_WARM: hipLaunchKernelGGL((k_cgv<CGV_INIT_WARM, RT>), grid, blk, 0, st, KT, iter); break;
+        case CGV_UPDATE: hipLaunchKernelGGL((k_cgv<CGV_UPDATE, RT>), grid, blk, 0, st, KT, iter); break;
+        default: hipLaunchKernelGGL((k_cgv<CGV_LAST, RT>), grid, blk, 0, st, KT, iter); break;
+    }
+}
+void launch_cgv(const LaunchTab &t, int mode, int rt64, int iter, hipStream_t st) {
+    if (rt64) launch_cgv_rt<double>(t, mode, iter, st);
+    else launch_cgv_rt<float>(t, mode, iter, st);
+}
+void launch_cgp(const LaunchTab &t, int mode, int rt64, int slot, hipStream_t st) {
+    const dim3 grid(t.nwg), blk(256);
+    if (rt64) {
+        if (mode == CGP_FIRST) hipLaunchKernelGGL((k_cgp<CGP_FIRST, double>), grid, blk, 0, st, KT, slot);
+        else hipLaunchKernelGGL((k_cgp<CGP_UPDATE, double>), grid, blk, 0, st, KT, slot);
+    } else {
+        if (mode == CGP_FIRST) hipLaunchKernelGGL((k_cgp<CGP_FIRST, float>), grid, blk, 0, st, KT, slot);
+        else hipLaunchKernelGGL((k_cgp<CGP_UPDATE, float>), grid, blk, 0, st, KT, slot);
+    }
 }
 void launch_final(const LaunchTab &t, hipStream_t st) {
     hipLaunchKernelGGL(k_final, dim3(t.nwg), dim3(256), 0, st, KT);

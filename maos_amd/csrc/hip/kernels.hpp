@@ -84,6 +84,7 @@ struct Geom {
     const float2 *gpd;            // [class][nploc][np*np] (x coef, y coef)
     const float4 *nead;           // [class][nploc] (xx, yy, xy, 0)
     // FDPCG
+    int ptt_lhs;                  // the LHS removes TT from the ttr WFS' gradients (!split || splitlrt)
     int precond, bs, scale;
     int zero_dc;                  // zero the DC bin of the residual spectrum (see k_col)
     int dot32;                    // precision mode 3: dot products and alpha/beta in FP32
@@ -149,6 +150,11 @@ struct PlanDev {
     Scal SC;
     int *stop;
     int ltiles;         // layer tiles per layer (layer partial index stride)
+    // TT removal inside the LHS (ptt_lhs): the operator's own gradients
+    // G HX p (residual precision, ngtot per system) and their TT projection
+    // PTT G HX p (FP64, 2 per system and WFS); null otherwise
+    void *gl;
+    double *ttl;
 };
 
 // A job is a range of systems of one plan; a launch covers a list of jobs,

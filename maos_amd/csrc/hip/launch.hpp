@@ -43,6 +43,15 @@ int colr_shape(int bs, int L, int *tasks);
 size_t colr_lds(int bs, int L, int P, int f64);
 void launch_colr(const LaunchTab &t, int spec64, int P, int L, int bs, int nthr, size_t lds, hipStream_t st);
 void launch_irow(const LaunchTab &t, int mode, int rm, int rp, int slot, int osel, size_t lds, hipStream_t st);
+// LHS WFS pass with TT removal (ptt_lhs): gradients (tg: the WFS tile
+// grid), their TT projection (tt: one workgroup per WFS and system), then
+// W (g - TT tt) and GP' into q (tg again).
+void launch_wfs_lhs_tt(const LaunchTab &tg, const LaunchTab &tt, int pos, int rt64, int xsel, hipStream_t st);
+// Plain CG vector kernels over the row blocks (tomo.precond=0).
+enum { CGV_MODE_INIT_COLD = 0, CGV_MODE_INIT_WARM = 1, CGV_MODE_UPDATE = 2, CGV_MODE_LAST = 3 };
+enum { CGP_MODE_FIRST = 0, CGP_MODE_UPDATE = 1 };
+void launch_cgv(const LaunchTab &t, int mode, int rt64, int iter, hipStream_t st);
+void launch_cgp(const LaunchTab &t, int mode, int rt64, int slot, hipStream_t st);
 void launch_final(const LaunchTab &t, hipStream_t st);
 void launch_zero_stop(const LaunchTab &t, hipStream_t st);
 void launch_copy(const float *a, float *b, long n, hipStream_t st);

@@ -111,6 +111,8 @@ struct Batch {
     DevBuf<Job> jobs;
     int rt64 = 1, spec64 = 1, maxit = 30, warm = 0, precond = 1, anyttr = 0;
     Stage zero, ttr, wfs_r, layer, wfs_l, row, col, irow, fin;
+    Stage wfs_g, ttl;  // LHS with TT removal: gradient tiles, TT projection (ptt_lhs jobs)
+    Stage cg;          // plain CG vector updates over the row blocks (no preconditioner)
 };
 }  // namespace
 
@@ -146,10 +148,12 @@ struct gpu_tomo {
     DevBuf<int> grp_col, colkx, grp_blk, blkpos;
     std::vector<int> blk_order;  // half-spectrum block id of each block in group order
     std::vector<int> blkpos_host;  // blkpos, host copy
+    std::vector<int> grp_blk_host; // grp_blk, host copy (register column layout)
     DevBuf<int2> rowblk;
     // state
     DevBuf<float> x, z, p, g, tt;
     DevBuf<double> r, Ap, b, q;  // residual side: double, or float in the first half when !rt64
+    DevBuf<double> gl, ttl;      // LHS gradients and their TT projection (ptt_lhs plans only)
     int rt64 = 1;             // 1: r, Ap, b in FP64 (default); 0: FP32
     int spec64 = 1;           // 1: forward FDPCG spectrum in FP64; 0: FP32
     size_t lds_max = 64 * 1024;  // LDS one workgroup may use on this device
@@ -183,6 +187,7 @@ struct gpu_tomo {
     std::map<std::pair<int, int>, std::unique_ptr<Batch>> own;
     // systems per workgroup chosen by the autotuner (0: heuristic)
     int tune_wfs = 0, tune_layer = 0, tune_col = 0;
+    long gen = 0;         // bumped whenever the launch tables must be rebuilt (shape, precision)
     int tune_lty = 0;     // layer tile rows: 0 = heuristic (LTY where it fits), TILE or LTY
     int tune_colfft = 0;  // column kernel: 0 = registers where available, 1 = registers, 2 = LDS stages
     bool colr_ok = false; // the register column kernel applies (per-frequency blocks, ny = shL x shP)
@@ -411,12 +416,13 @@ std::vector<float2> col_blocks(const mt_recon_t *R, const gpu_tomo *g) {
 // ky = k1 + P bitrev_L(l), conjugated where the block's frequency is the
 // conjugate of its bin (conj(M) x = conj(M conj(x))).  Otherwise the plan
 // keeps the LDS column kernel only (colr_ok false, shL 0).
-void build_mreg(gpu_tomo *g, const std::vector<float2> &hM, const std::vector<int> &hgb) {
-    Geom &G = g->G;
-    g->colr_ok = false;
-    G.Mreg = nullptr;
+// Returns false (the layout does not apply) or fills Mr.  hgb: first block
+// of each column group (group order).
+bool mreg_layout(const gpu_tomo *g, const std::vector<float2> &hM, const std::vector<int> &hgb,
+                 std::vector<float2> &Mr) {
+    const Geom &G = g->G;
     const int L = G.shL, P = G.shP, ny = G.fny, bs = G.bs, ngr = G.ngroup;
-    auto none = [&] { G.shL = G.shP = 0; };
+    auto none = [&] { return false; };
     if (!L || G.maxcol != 1 || bs != G.npsr) return none();
     std::vector<int> jof((size_t)ngr * ny, -1);  // block of (column, ky)
     std::vector<char> cjof((size_t)ngr * ny, 0);
@@ -443,7 +449,7 @@ void build_mreg(gpu_tomo *g, const std::vector<float2> &hM, const std::vector<in
         return r;
     };
     const size_t nb2 = (size_t)bs * bs;
-    std::vector<float2> Mr((size_t)ngr * ny * nb2);
+    Mr.assign((size_t)ngr * ny * nb2, make_float2(0.f, 0.f));
     for (int c = 0; c < ngr; c++)
         for (int k1 = 0; k1 < P; k1++)
             for (int l = 0; l < L; l++) {
@@ -455,15 +461,26 @@ void build_mreg(gpu_tomo *g, const std::vector<float2> &hM, const std::vector<in
                     Mr[(((size_t)c * P + k1) * nb2 + e) * L + l] = v;
                 }
             }
+    return true;
+}
+void build_mreg(gpu_tomo *g, const std::vector<float2> &hM, const std::vector<int> &hgb) {
+    Geom &G = g->G;
+    g->colr_ok = false;
+    G.Mreg = nullptr;
+    std::vector<float2> Mr;
+    if (!mreg_layout(g, hM, hgb, Mr)) {
+        G.shL = G.shP = 0;
+        return;
+    }
     g->Mreg.upload(Mr);
     G.Mreg = g->Mreg.p;
     g->colr_ok = true;
+    g->grp_blk_host = hgb;
 }
 
 void build_plan(gpu_tomo *g, const mt_recon_t *R) {
     if (!R || R->npsr <= 0 || R->npsr > MAXL || R->nwfs <= 0) fail(MT_EINVAL, "invalid recon (npsr/nwfs)");
     if (R->pos != 1 && R->pos != 2) fail(MT_EINVAL, "tomo.pos must be 1 or 2");
-    if (R->ptt_lhs) fail(MT_EINVAL, "TT removal in the LHS (recon.split=0 or tomo.splitlrt=1) is not supported");
     Geom &G = g->G;
     G = Geom();
     g->npsr = R->npsr;
@@ -510,6 +527,10 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
     G.saoff[nw] = saoff;
     G.ngtot = goff;
     for (int l = 0; l < R->npsr; l++) G.l2c[l] = (float)R->l2c[l];
+    // TT removal inside the LHS: the same TTFR as the RHS, on the operator's
+    // own gradients (maos/recon_utils.c:461-465); the GPU reference removes
+    // it there too when !split || splitlrt (cuda/tomo.cu:183)
+    G.ptt_lhs = R->ptt_lhs ? 1 : 0;
 
     // --- ray tables -------------------------------------------------------
     const long pnx = R->pnx, pny = R->pny;
@@ -733,20 +754,23 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
 
     // --- FDPCG ---------------------------------------------------------------
     G.precond = R->precond ? 1 : 0;
-    // FFT geometry is needed for the row blocks even without preconditioner
-    // (the PCG update kernel walks rows).
+    // Row blocks (ROWS rows of one layer) carry the PCG vector updates: the
+    // row FFT kernels with FDPCG, the plain CG kernels without it (where
+    // layers may differ in size).
     const int nx0 = (int)R->xnx[0], ny0 = (int)R->xny[0];
-    for (int l = 0; l < R->npsr; l++)
-        if (R->xnx[l] != nx0 || R->xny[l] != ny0)
-            fail(MT_EINVAL, "all layers must share one grid size (equal atmr.os)");
-    if ((nx0 & 1) || (ny0 & 1)) fail(MT_EINVAL, "layer grids must have even size");
-    if (nx0 > MAXN || ny0 > MAXN) fail(MT_EINVAL, "layer grid larger than 1024");
+    if (G.precond) {
+        for (int l = 0; l < R->npsr; l++)
+            if (R->xnx[l] != nx0 || R->xny[l] != ny0)
+                fail(MT_EINVAL, "all layers must share one grid size (equal atmr.os)");
+        if ((nx0 & 1) || (ny0 & 1)) fail(MT_EINVAL, "layer grids must have even size");
+        if (nx0 > MAXN || ny0 > MAXN) fail(MT_EINVAL, "layer grid larger than 1024");
+    }
     G.fnx = nx0;
     G.fny = ny0;
     {
         std::vector<int2> rb;
         for (int l = 0; l < R->npsr; l++)
-            for (int r0 = 0; r0 < ny0; r0 += ROWS) rb.push_back(make_int2(l, r0));
+            for (int r0 = 0; r0 < R->xny[l]; r0 += ROWS) rb.push_back(make_int2(l, r0));
         G.nrowblk = (int)rb.size();
         g->rowblk.upload(rb);
         G.rowblk = g->rowblk.p;
@@ -754,7 +778,7 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
     long so = 0;
     for (int l = 0; l < R->npsr; l++) {
         G.soff[l] = so;
-        so += (long)(nx0 / 2 + 1) * ny0;
+        if (G.precond) so += (long)(nx0 / 2 + 1) * ny0;
     }
     G.soff[R->npsr] = so;
     G.nspec = so;
@@ -895,7 +919,7 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
         if (col_lds(G.maxcol, G.npsr, G.fny, 1, 1) > 64 * 1024)
             fail(MT_EINVAL, "FDPCG column group too large for LDS");
     }
-    if (row_lds(G.fnx, 1) > 64 * 1024) fail(MT_EINVAL, "layer grid too large for the row FFT kernel");
+    if (G.precond && row_lds(G.fnx, 1) > 64 * 1024) fail(MT_EINVAL, "layer grid too large for the row FFT kernel");
 }
 
 // Tiles of the layer passes (the largest layer decides; all layers are
@@ -908,6 +932,7 @@ int layer_tiles(const Geom &G) {
 }
 
 void upload_plandev(gpu_tomo *g);
+void upload_plandev(gpu_tomo *g, hipStream_t st);
 void drop_batches(gpu_tomo *g);
 // Apply the layer tile height choice (the plan's device record and
 // partial counts follow; launch tables are rebuilt on next use).
@@ -924,7 +949,7 @@ void apply_lty(gpu_tomo *g) {
     drop_batches(g);
 }
 
-void upload_plandev(gpu_tomo *g) {
+void upload_plandev(gpu_tomo *g, hipStream_t st) {
     PlanDev h{};
     h.G = g->G;
     h.x = g->x.p;
@@ -941,9 +966,13 @@ void upload_plandev(gpu_tomo *g) {
     h.SC = g->SC;
     h.stop = g->stop.p;
     h.ltiles = layer_tiles(g->G);
+    h.gl = g->gl.p;
+    h.ttl = g->ttl.p;
     if (!g->pdev.p) g->pdev.alloc(1);
-    HIPCHK(hipMemcpy(g->pdev.p, &h, sizeof(PlanDev), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(g->pdev.p, &h, sizeof(PlanDev), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
 }
+void upload_plandev(gpu_tomo *g) { upload_plandev(g, g->stream); }
 
 void alloc_state(gpu_tomo *g) {
     const Geom &G = g->G;
@@ -957,6 +986,10 @@ void alloc_state(gpu_tomo *g) {
     g->g.alloc(ns * G.ngtot);
     g->q.alloc(ns * G.nwfs * G.nploc);
     g->tt.alloc(ns * G.nwfs * 2);
+    if (G.ptt_lhs) {
+        g->gl.alloc(ns * G.ngtot);
+        g->ttl.alloc(ns * G.nwfs * 2);
+    }
     g->specf.alloc(ns * std::max<long>(G.nspec, 1));
     g->speci.alloc(ns * std::max<long>(G.nspec, 1));
     Geom g16 = G;  // partials sized for the smaller layer tiles (either height may be chosen later)
@@ -1191,7 +1224,15 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
     build_stage(b, b.wfs_r, [&](int j, std::vector<int> &k) { k = {G(j).pos}; return true; },
                 [&](const Sub &, int j) { return JobGrid{ptiles(j), G(j).nwfs, NS(j)}; },
                 [&](Sub &sb, const std::vector<int> &k, const std::vector<int> &) { sb.pos = k[0]; });
-    build_stage(b, b.wfs_l, [&](int j, std::vector<int> &k) { k = {G(j).matched, G(j).pos}; return true; },
+    // LHS with TT removal: three launches (gradients, TT projection, W GP')
+    // on the generic tables; other jobs take the one-pass kernels below
+    build_stage(b, b.wfs_g, [&](int j, std::vector<int> &k) { k = {G(j).pos}; return G(j).ptt_lhs != 0; },
+                [&](const Sub &, int j) { return JobGrid{ptiles(j), G(j).nwfs, NS(j)}; },
+                [&](Sub &sb, const std::vector<int> &k, const std::vector<int> &) { sb.pos = k[0]; });
+    build_stage(b, b.ttl, [&](int j, std::vector<int> &k) { k = {G(j).pos}; return G(j).ptt_lhs != 0; },
+                [&](const Sub &, int j) { return JobGrid{G(j).nwfs, 1, NS(j)}; },
+                [&](Sub &sb, const std::vector<int> &k, const std::vector<int> &) { sb.pos = k[0]; });
+    build_stage(b, b.wfs_l, [&](int j, std::vector<int> &k) { k = {G(j).matched, G(j).pos}; return !G(j).ptt_lhs; },
                 [&](const Sub &sb, int j) {
                     return JobGrid{ptiles(j), G(j).nwfs, sb.matched ? ceil_div(NS(j), sb.cs) : NS(j)};
                 },
@@ -1301,6 +1342,10 @@ std::unique_ptr<Batch> make_batch(const std::vector<gpu_tomo *> &plans, const st
                     },
                     [](const Sub &sb) { return sb.reg ? XCD_BY_Y : XCD_BY_X; });
     }
+    if (!b.precond)
+        build_stage(b, b.cg, [&](int, std::vector<int> &) { return true; },
+                    [&](const Sub &, int j) { return JobGrid{G(j).nrowblk, 1, NS(j)}; },
+                    [&](Sub &, const std::vector<int> &, const std::vector<int> &) {});
     build_stage(b, b.fin, [&](int, std::vector<int> &) { return true; },
                 [&](const Sub &, int j) { return JobGrid{1, 1, NS(j)}; },
                 [&](Sub &, const std::vector<int> &, const std::vector<int> &) {});
@@ -1318,6 +1363,7 @@ Batch &own_batch(gpu_tomo *g, int s0, int n) {
     return ref;
 }
 void drop_batches(gpu_tomo *g) {
+    g->gen++;
     g->own.clear();
     if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
@@ -1338,6 +1384,9 @@ LaunchTab tab(const Batch &B, const Sub &s) { return LaunchTab{B.jobs.p, s.wgi.p
 void enqueue_L(Batch &B, int vsel, int partsel, Rec &rec) {
     rec(KC_WFS, [&] {
         for (auto &s : B.wfs_l) launch_wfs(tab(B, *s), false, s->matched, s->pos, s->cs, B.rt64, vsel, rec.st);
+        // the two stages hold the same partitions (keyed by tomo.pos) in order
+        for (size_t i = 0; i < B.wfs_g.size(); i++)
+            launch_wfs_lhs_tt(tab(B, *B.wfs_g[i]), tab(B, *B.ttl[i]), B.wfs_g[i]->pos, B.rt64, vsel, rec.st);
     });
     rec(KC_LAYER, [&] {
         for (auto &s : B.layer)
@@ -1384,11 +1433,52 @@ void enqueue_P_tail(Batch &B, int irow_mode, int slot, int osel, Rec &rec) {
 // Kernels per iteration: wfs, layer (Ap, p.Ap partials), row (alpha, r,
 // forward row FFT), col (column FFTs, blocks, r.z partials), irow (inverse
 // row FFT, beta, x and p).  All jobs of the batch advance in lockstep.
+// Without a preconditioner (tomo.precond=0) z = r, as lib/pcg.c does with
+// Mmul = NULL: per iteration wfs, layer, cgv (alpha, x, r, r.r partials) and
+// cgp (beta, p); the start is cgv (r = b, or b - A x warm) and cgp (p = r).
+void enqueue_solve_cg(Batch &B, Rec &rec) {
+    const int maxit = B.maxit;
+    auto cgv = [&](int kc, int mode, int iter) {
+        rec(kc, [&] {
+            for (auto &s : B.cg) launch_cgv(tab(B, *s), mode, B.rt64, iter, rec.st);
+        });
+    };
+    auto cgp = [&](int kc, int mode, int slot) {
+        rec(kc, [&] {
+            for (auto &s : B.cg) launch_cgp(tab(B, *s), mode, B.rt64, slot, rec.st);
+        });
+    };
+    auto start = [&](int k, bool cold) {
+        if (cold) {
+            cgv(KC_ROW_START, CGV_MODE_INIT_COLD, k);
+        } else {
+            enqueue_L(B, BUF_X, PART_NONE, rec);
+            cgv(KC_ROW_START, CGV_MODE_INIT_WARM, k);
+        }
+        cgp(KC_IROW_START, CGP_MODE_FIRST, k);
+    };
+    start(0, !B.warm);
+    for (int k = 0; k < maxit; k++) {
+        if (k > 0 && k % 100 == 0) start(k, false);
+        enqueue_L(B, BUF_P, PART_PAP, rec);
+        const bool last = k + 1 == maxit;
+        cgv(last ? KC_ROW_LAST : KC_ROW, last ? CGV_MODE_LAST : CGV_MODE_UPDATE, k);
+        if (last) break;
+        cgp(KC_IROW, CGP_MODE_UPDATE, k);
+    }
+}
+
 void enqueue_solve(Batch &B, Rec &rec) {
     const int maxit = B.maxit;
-    if (!B.precond) fail(MT_EINVAL, "the GPU solve requires the FDPCG preconditioner (tomo.precond=1)");
     for (auto &s : B.zero) launch_zero_stop(tab(B, *s), rec.st);
     enqueue_R(B, rec);
+    if (!B.precond) {
+        enqueue_solve_cg(B, rec);
+        rec(KC_FINAL, [&] {
+            for (auto &s : B.fin) launch_final(tab(B, *s), rec.st);
+        });
+        return;
+    }
     auto start = [&](int k, bool cold) {
         if (cold) {
             enqueue_row(B, KC_ROW_START, ROW_MODE_INIT_COLD, k, BUF_X, rec);
@@ -1412,12 +1502,15 @@ void enqueue_solve(Batch &B, Rec &rec) {
     });
 }
 
-void prof_collect(gpu_tomo *g) {
+// Fold the recorded events into the per-class averages; accumulate: add to
+// the current sums (a call made of several solves) instead of starting anew.
+void prof_collect(gpu_tomo *g, bool accumulate = false) {
     HIPCHK(hipStreamSynchronize(g->stream));
-    for (int i = 0; i < KC_N; i++) {
-        g->prof_ms[i] = 0;
-        g->prof_cnt[i] = 0;
-    }
+    if (!accumulate)
+        for (int i = 0; i < KC_N; i++) {
+            g->prof_ms[i] = 0;
+            g->prof_cnt[i] = 0;
+        }
     for (auto &e : g->evs) {
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
@@ -1542,23 +1635,41 @@ int gpu_tomo_setup(const mt_recon_t *recon, int device, int nbatch, int maxit, i
     return rc;
 }
 
+// New regulariser and FDPCG blocks of a plan, written in place (the device
+// record, Minv and the register kernel's copy Mreg keep their addresses, so
+// launch tables and captured graphs that point at them stay valid).  Waits
+// for the plan's own streams only: the caller must not have a solve of this
+// plan in flight on another stream.
+static void update_cn2_plan(gpu_tomo *g, const mt_recon_t *recon) {
+    if (recon->npsr != g->npsr) fail(MT_EINVAL, "layer count changed");
+    if ((recon->precond ? 1 : 0) != g->G.precond) fail(MT_EINVAL, "preconditioner changed");
+    std::vector<float2> hM, Mr;
+    if (g->G.precond) {
+        const long nbh = recon->fd_nbhalf, bs = recon->fd_bs;
+        if (bs != g->G.bs || (long)g->Minv.n != nbh * bs * bs) fail(MT_EINVAL, "FDPCG layout changed");
+        hM = col_blocks(recon, g);
+        if (g->colr_ok) {
+            if (!mreg_layout(g, hM, g->grp_blk_host, Mr) || Mr.size() != g->Mreg.n)
+                fail(MT_EINVAL, "FDPCG block layout changed");
+        }
+    }
+    HIPCHK(hipStreamSynchronize(g->stream));
+    for (auto &f : g->fl)
+        if (f.st) HIPCHK(hipStreamSynchronize(f.st));
+    for (int l = 0; l < recon->npsr; l++) g->G.l2c[l] = (float)recon->l2c[l];
+    if (!hM.empty())
+        HIPCHK(hipMemcpyAsync(g->Minv.p, hM.data(), sizeof(float2) * hM.size(), hipMemcpyHostToDevice, g->stream));
+    if (!Mr.empty())
+        HIPCHK(hipMemcpyAsync(g->Mreg.p, Mr.data(), sizeof(float2) * Mr.size(), hipMemcpyHostToDevice, g->stream));
+    upload_plandev(g, g->stream);
+    HIPCHK(hipStreamSynchronize(g->stream));
+}
+
 int gpu_tomo_update_cn2(gpu_tomo_t *g, const mt_recon_t *recon) {
     if (!g || !recon) return MT_EINVAL;
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
-        HIPCHK(hipStreamSynchronize(g->stream));
-        HIPCHK(hipDeviceSynchronize());
-        if (recon->npsr != g->npsr) fail(MT_EINVAL, "layer count changed");
-        for (int l = 0; l < recon->npsr; l++) g->G.l2c[l] = (float)recon->l2c[l];
-        if (g->G.precond) {
-            const long nbh = recon->fd_nbhalf, bs = recon->fd_bs;
-            if (bs != g->G.bs || (long)g->Minv.n != nbh * bs * bs) fail(MT_EINVAL, "FDPCG layout changed");
-            std::vector<float2> hM = col_blocks(recon, g);
-            HIPCHK(hipMemcpy(g->Minv.p, hM.data(), sizeof(float2) * hM.size(), hipMemcpyHostToDevice));
-        }
-        // in-place update of the device plan: tables and captured graphs,
-        // which point at it, stay valid
-        upload_plandev(g);
+        update_cn2_plan(g, recon);
     });
 }
 
@@ -1777,6 +1888,13 @@ int gpu_tomo_mvm_tomo(gpu_tomo_t *g, long col0, long ncol, double *out) {
         if (idx.empty()) return;
         DevBuf<int> didx;
         didx.upload(idx);
+        // the chunks overwrite the slots' gradients and solutions: keep them
+        // (the warm-start state of a closed loop) and put them back at the end
+        DevBuf<float> keep_x, keep_g;
+        keep_x.alloc(g->x.n);
+        keep_g.alloc(g->g.n);
+        HIPCHK(hipMemcpyAsync(keep_x.p, g->x.p, sizeof(float) * g->x.n, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(keep_g.p, g->g.p, sizeof(float) * g->g.n, hipMemcpyDeviceToDevice, st));
         // chunks of nbatch columns; two staging buffers, so that the host
         // widens chunk t-1 to double while the GPU solves chunk t
         const int B = g->nbatch;
@@ -1817,7 +1935,7 @@ int gpu_tomo_mvm_tomo(gpu_tomo_t *g, long col0, long ncol, double *out) {
                 } else {  // profiled plans launch kernel by kernel, as gpu_tomo_solve_dev does
                     Rec rec{&g->evs, st, true, g->prof};
                     enqueue_solve(own_batch(g, 0, n), rec);
-                    prof_collect(g);
+                    prof_collect(g, t > 0);  // the averages cover every chunk
                 }
                 HIPCHK(hipMemcpy2DAsync(h[t % 2], sizeof(float) * nx, g->x.p, sizeof(float) * G.nxtot,
                                         sizeof(float) * nx, n, hipMemcpyDeviceToHost, st));
@@ -1827,12 +1945,17 @@ int gpu_tomo_mvm_tomo(gpu_tomo_t *g, long col0, long ncol, double *out) {
             widen(nch - 1);
         } catch (...) {
             (void)hipStreamSynchronize(st);
+            (void)hipMemcpy(g->x.p, keep_x.p, sizeof(float) * g->x.n, hipMemcpyDeviceToDevice);
+            (void)hipMemcpy(g->g.p, keep_g.p, sizeof(float) * g->g.n, hipMemcpyDeviceToDevice);
             for (int b = 0; b < 2; b++) {
                 if (h[b]) (void)hipHostFree(h[b]);
                 if (ev[b]) (void)hipEventDestroy(ev[b]);
             }
             throw;
         }
+        HIPCHK(hipMemcpyAsync(g->x.p, keep_x.p, sizeof(float) * g->x.n, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(g->g.p, keep_g.p, sizeof(float) * g->g.n, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipStreamSynchronize(st));
         for (int b = 0; b < 2; b++) {
             (void)hipHostFree(h[b]);
             (void)hipEventDestroy(ev[b]);
@@ -1914,6 +2037,7 @@ int gpu_tomo_get_x(gpu_tomo_t *g, int isys, double *x, double *cgres) {
 struct gpu_tomo_group {
     std::vector<gpu_tomo *> plans;
     std::vector<int> s0, nsys;
+    std::vector<long> gens;           // each member's gen when B was built
     std::unique_ptr<Batch> B;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
@@ -1940,16 +2064,35 @@ int gpu_tomo_group_create(int n, gpu_tomo_t *const *plans, const int *s0, const 
         grp->s0.assign(s0, s0 + n);
         grp->nsys.assign(nsys, nsys + n);
         grp->B = make_batch(pl, grp->s0, grp->nsys);
+        for (auto *g : pl) grp->gens.push_back(g->gen);
         HIPCHK(hipStreamCreateWithFlags(&grp->stream, hipStreamNonBlocking));
     });
     if (rc == MT_OK) *out = grp.release();
     return rc;
 }
 
+// The group's tables follow its members: a member whose launch shape or
+// precision changed since (gpu_tomo_tune_set, gpu_tomo_precision,
+// gpu_tomo_autotune) has a new gen, and the group rebuilds its batch and
+// captures its graph again before solving.
+static void group_refresh(gpu_tomo_group *grp) {
+    bool moved = false;
+    for (size_t j = 0; j < grp->plans.size(); j++) moved |= grp->plans[j]->gen != grp->gens[j];
+    if (!moved) return;
+    HIPCHK(hipStreamSynchronize(grp->stream));
+    if (grp->gexec) (void)hipGraphExecDestroy(grp->gexec);
+    if (grp->graph) (void)hipGraphDestroy(grp->graph);
+    grp->gexec = nullptr;
+    grp->graph = nullptr;
+    grp->B = make_batch(grp->plans, grp->s0, grp->nsys);
+    for (size_t j = 0; j < grp->plans.size(); j++) grp->gens[j] = grp->plans[j]->gen;
+}
+
 int gpu_tomo_group_solve(gpu_tomo_group_t *grp, int mode, void *stream) {
     if (!grp) return MT_EINVAL;
     return guarded([&] {
         HIPCHK(hipSetDevice(grp->device));
+        group_refresh(grp);
         hipStream_t st = stream ? (hipStream_t)stream : grp->stream;
         if (mode == 1) {
             if (!grp->gexec) capture_solve(*grp->B, &grp->graph, &grp->gexec, &grp->evs);
@@ -1981,6 +2124,7 @@ void group_apply(gpu_tomo_group *grp, const int *c) {
     grp->gexec = nullptr;
     grp->graph = nullptr;
     grp->B = make_batch(grp->plans, grp->s0, grp->nsys);
+    for (size_t j = 0; j < grp->plans.size(); j++) grp->gens[j] = grp->plans[j]->gen;
 }
 }  // namespace
 
@@ -2265,6 +2409,7 @@ struct SrvLoop {
     gpu_tomo *plan = nullptr;
     size_t bytes = 0;
     bool busy = false;                 // a request of this loop is on the GPU
+    bool hold = false;                 // its plan is being updated: admit none of its requests
     std::vector<double> xlast;         // last completed solution (host)
     double reslast = NAN;
 };
@@ -2315,6 +2460,9 @@ size_t plan_bytes(const gpu_tomo *g) {
 }
 }  // namespace
 
+// Ownership: `flights` (every field of every Flight) belongs to the worker
+// thread alone; other threads reach it only by asking the worker (rebuild
+// requests below).  Everything else is shared and guarded by `mu`.
 struct gpu_tomo_server {
     int device = 0, max_inflight = 2, max_batch = 64, use_graph = 1;
     size_t budget = 0, used = 0;
@@ -2323,6 +2471,8 @@ struct gpu_tomo_server {
     std::vector<SrvDone> done;
     std::vector<Flight> flights;
     long gen = 0;                      // bumped when loops are added or removed
+    long rebuild_gen = -1;             // ask the worker to rebuild every flight for this gen
+    long built_gen = -1;               // the gen it last rebuilt every flight for
     std::mutex mu;
     std::condition_variable cv, cv_done;
     std::thread worker;
@@ -2352,9 +2502,25 @@ void flight_drop(Flight &f) {
     f.hoff.clear();
     f.B.reset();
 }
-// (Re)build an idle flight's batch over all admitted loops (lock held).
+// (Re)build an idle flight's batch over all admitted loops (worker thread,
+// lock held).  The flight counts as current for S->gen only once the whole
+// build has succeeded: a build that throws leaves it dropped and stale, so
+// it is built again before any launch (a flight marked current with a
+// partial build would be launched with null host buffers).
+void flight_build_body(gpu_tomo_server *S, Flight &f);
 void flight_build(gpu_tomo_server *S, Flight &f) {
     flight_drop(f);
+    f.gen = -1;
+    try {
+        flight_build_body(S, f);
+    } catch (...) {
+        flight_drop(f);
+        f.gen = -1;
+        throw;
+    }
+    f.gen = S->gen;
+}
+void flight_build_body(gpu_tomo_server *S, Flight &f) {
     std::vector<gpu_tomo *> pl;
     f.job_of.assign(S->loops.size(), -1);
     for (size_t i = 0; i < S->loops.size(); i++)
@@ -2362,7 +2528,6 @@ void flight_build(gpu_tomo_server *S, Flight &f) {
             f.job_of[i] = (int)pl.size();
             pl.push_back(S->loops[i]->plan);
         }
-    f.gen = S->gen;
     if (pl.empty()) return;
     f.B = make_batch(pl, std::vector<int>(pl.size(), 0), std::vector<int>(pl.size(), 1));
     HIPCHK(hipHostMalloc((void **)&f.hjobs, sizeof(Job) * pl.size()));
@@ -2429,6 +2594,17 @@ void srv_worker(gpu_tomo_server *S) {
     try {
         while (!S->stop) {
             bool did = false;
+            // rebuild requests (server autotune): every flight is idle then
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                if (S->rebuild_gen > S->built_gen) {
+                    const long want = S->rebuild_gen;
+                    for (auto &f : S->flights)
+                        if (!f.active) flight_build(S, f);
+                    S->built_gen = want;
+                    S->cv_done.notify_all();
+                }
+            }
             // retire completed flights
             for (auto &f : S->flights) {
                 if (!f.active) continue;
@@ -2472,7 +2648,7 @@ void srv_worker(gpu_tomo_server *S) {
                         for (auto &L : S->loops) nl += L->plan != nullptr;
                         std::vector<int> seen;
                         for (auto &r : S->pending) {
-                            if (S->loops[r.loop]->busy ||
+                            if (S->loops[r.loop]->busy || S->loops[r.loop]->hold ||
                                 std::find(seen.begin(), seen.end(), r.loop) != seen.end())
                                 continue;
                             seen.push_back(r.loop);
@@ -2485,7 +2661,7 @@ void srv_worker(gpu_tomo_server *S) {
                     std::vector<int> taken;
                     for (auto it = S->pending.begin(); it != S->pending.end() && (int)f.reqs.size() < S->max_batch;) {
                         SrvLoop &L = *S->loops[it->loop];
-                        if (L.busy || std::find(taken.begin(), taken.end(), it->loop) != taken.end()) {
+                        if (L.busy || L.hold || std::find(taken.begin(), taken.end(), it->loop) != taken.end()) {
                             ++it;
                             continue;
                         }
@@ -2518,6 +2694,11 @@ void srv_worker(gpu_tomo_server *S) {
     } catch (const HipError &e) {
         std::lock_guard<std::mutex> lk(S->mu);
         S->err = e.msg;
+        S->stop = true;
+        S->cv_done.notify_all();
+    } catch (const std::exception &e) {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->err = e.what();
         S->stop = true;
         S->cv_done.notify_all();
     }
@@ -2597,7 +2778,7 @@ int gpu_tomo_server_remove_loop(gpu_tomo_server_t *S, int loop) {
     if (loop < 0 || loop >= (int)S->loops.size() || !S->loops[loop]->plan) return MT_EINVAL;
     for (auto &r : S->pending)
         if (r.loop == loop) return MT_ESTATE;
-    if (S->loops[loop]->busy) return MT_ESTATE;
+    if (S->loops[loop]->busy || S->loops[loop]->hold) return MT_ESTATE;
     SrvLoop &L = *S->loops[loop];
     // flights still holding a batch with this loop switch it off (its job is
     // never on again) and rebuild before their next launch
@@ -2610,9 +2791,10 @@ int gpu_tomo_server_remove_loop(gpu_tomo_server_t *S, int loop) {
 
 int gpu_tomo_server_autotune(gpu_tomo_server_t *S, int *choice) {
     if (!S) return MT_EINVAL;
-    // the lock is held throughout: the worker admits nothing meanwhile, and
-    // with no request pending or in flight no batch is on the GPU
-    std::lock_guard<std::mutex> lk(S->mu);
+    // the lock is held (but for the wait on the worker's rebuild): the worker
+    // admits nothing meanwhile, and with no request pending or in flight no
+    // batch is on the GPU
+    std::unique_lock<std::mutex> lk(S->mu);
     if (!S->pending.empty() || S->inflight_reqs) {
         mt::set_error("server autotune needs an idle server (drain first)");
         return MT_ESTATE;
@@ -2637,17 +2819,55 @@ int gpu_tomo_server_autotune(gpu_tomo_server_t *S, int *choice) {
     rc = gpu_tomo_group_create(n, pl.data(), s0.data(), ns.data(), &grp);
     if (rc == MT_OK) rc = gpu_tomo_group_autotune(grp, choice);
     gpu_tomo_group_free(grp);
-    // every flight's batch tables and graph predate the new shapes: rebuild
-    // them now rather than at their first launch, inside the request stream
+    // every flight's batch tables and graph predate the new shapes: the
+    // worker (which alone touches the flights) rebuilds them now rather than
+    // at their first launch, inside the request stream
     S->gen++;
-    const int rc2 = guarded([&] {
+    int rc2 = guarded([&] {
         for (int j = 0; j < n; j++)
             HIPCHK(hipMemcpy(pl[j]->x.p, keep[j].p, sizeof(float) * pl[j]->x.n, hipMemcpyDeviceToDevice));
         HIPCHK(hipDeviceSynchronize());
-        if (rc == MT_OK)
-            for (auto &f : S->flights) flight_build(S, f);
     });
+    if (rc == MT_OK && rc2 == MT_OK) {
+        const long want = S->gen;
+        S->rebuild_gen = want;
+        S->cv.notify_all();
+        S->cv_done.wait(lk, [&] { return S->built_gen >= want || !S->err.empty(); });
+        if (!S->err.empty()) {
+            mt::set_error(S->err);
+            rc2 = MT_EHIP;
+        }
+    }
     return rc != MT_OK ? rc : rc2;
+}
+
+int gpu_tomo_server_update_cn2(gpu_tomo_server_t *S, int loop, const mt_recon_t *recon) {
+    if (!S || !recon) return MT_EINVAL;
+    std::unique_lock<std::mutex> lk(S->mu);
+    if (loop < 0 || loop >= (int)S->loops.size() || !S->loops[loop]->plan || S->loops[loop]->hold)
+        return MT_EINVAL;
+    SrvLoop &L = *S->loops[loop];
+    // no new flight takes this loop; wait for the one that carries it, if any
+    // (other loops keep being served meanwhile)
+    L.hold = true;
+    S->cv_done.wait(lk, [&] { return !L.busy || !S->err.empty(); });
+    int rc = MT_OK;
+    if (!S->err.empty()) {
+        mt::set_error(S->err);
+        rc = MT_EHIP;
+    } else {
+        gpu_tomo *g = L.plan;
+        lk.unlock();
+        rc = guarded([&] {
+            HIPCHK(hipSetDevice(S->device));
+            update_cn2_plan(g, recon);
+        });
+        lk.lock();
+    }
+    L.hold = false;
+    lk.unlock();
+    S->cv.notify_all();
+    return rc;
 }
 
 int gpu_tomo_server_submit(gpu_tomo_server_t *S, int loop, const double *grad, long tag) {

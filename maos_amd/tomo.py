@@ -391,6 +391,12 @@ class Server:
         _abi.check(_abi.lib().gpu_tomo_server_autotune(self._h, ch), "gpu_tomo_server_autotune")
         return tuple(ch)
 
+    def update_cn2(self, lid, recon):
+        """gpu_tomo_server_update_cn2: loop lid takes recon's new regulariser
+        and FDPCG blocks (after recon.update_cn2); other loops keep running."""
+        _abi.check(_abi.lib().gpu_tomo_server_update_cn2(self._h, int(lid), C.byref(recon.rec)),
+                   "gpu_tomo_server_update_cn2")
+
     def usage(self):
         u, b = C.c_double(), C.c_double()
         _abi.lib().gpu_tomo_server_usage(self._h, C.byref(u), C.byref(b))
