@@ -245,6 +245,28 @@ int  gpu_tomo_solve_batch(gpu_tomo_t *g, int nsys, const double *grad,
  * results do not depend on k. */
 int  gpu_tomo_solve_dev(gpu_tomo_t *g, const float *d_grad, float *d_x,
                         int nbatch, int mode, void *stream);
+/* gpu_tomo_solve_dev (mode 0, plain launches) that allocates nothing on the
+ * device or the host at call time, for callers that cannot allocate: code
+ * under hipGraph stream capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  d_grad (nsys stacked FP32 gradient vectors in the active-WFS
+ * layout of gpu_tomo_solve_dev) is read, d_x (nsys stacked layer vectors:
+ * warm start in when the plan has warm=1, solution out) is written in place,
+ * and d_res (device, nsys doubles, may be NULL) receives the residuals.  The
+ * solve's other state (r, z, p, A p, the WFS fields, both FDPCG spectra,
+ * partial sums, scalars) lives in one library-owned device array of
+ * 16 MiB per GPU, fixed at load (the 30 m / 6 LGS / 6 layer system needs
+ * 2.9 MB); a plan whose one-system state exceeds it gets MT_EINVAL.  The
+ * systems are solved one after another with the plan's one-system launch
+ * tables (built by gpu_tomo_setup and by every call that changes the launch
+ * shape), with the same bits as gpu_tomo_solve_dev.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other streams, other
+ * threads, other plans) race and give wrong results; the array is cleared
+ * at the start of each system.  Profiling must be off (MT_ESTATE).
+ * No counterpart in the reference: MAOS allocates its CG scratch per
+ * reconstructor (CGTMP_T, cuda/pcg.h:24-37); this is for callers that
+ * cannot allocate. */
+int  gpu_tomo_solve_dev_noalloc(gpu_tomo_t *g, const float *d_grad, float *d_x,
+                                double *d_res, int nsys, void *stream);
 /* Tomography half of MVM assembly: columns [col0, col0 + ncol) of the
  * matrix T with x = T g, each the solve of a unit gradient vector (cold
  * start, maxit iterations, nbatch columns per graph solve), into out

@@ -125,6 +125,7 @@ def lib():
     L.gpu_tomo_server_free.argtypes = [vp]
     L.gpu_tomo_server_free.restype = None
     L.gpu_tomo_solve_dev.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp]
+    L.gpu_tomo_solve_dev_noalloc.argtypes = [vp, vp, vp, vp, C.c_int, vp]
     L.gpu_tomo_mvm_tomo.argtypes = [vp, C.c_long, C.c_long, dp]
     L.gpu_tomo_residuals.argtypes = [vp, dp, C.c_int]
     L.gpu_tomo_L.argtypes = [vp, dp, dp, C.c_double, C.c_double]
@@ -160,7 +161,7 @@ EXPORTS = [
     "mt_setup_free", "mt_setup_fdpcg_mhat", "mt_setup_sizes", "mt_fdpcg_prepare",
     "mt_fdpcg_free", "gpu_tomo_setup",
     "gpu_tomo_update_cn2", "gpu_tomo_reset", "gpu_tomo_free", "gpu_tomo_solve",
-    "gpu_tomo_solve_batch", "gpu_tomo_solve_dev", "gpu_tomo_mvm_tomo", "gpu_tomo_set_grad", "gpu_tomo_get_x",
+    "gpu_tomo_solve_batch", "gpu_tomo_solve_dev", "gpu_tomo_solve_dev_noalloc", "gpu_tomo_mvm_tomo", "gpu_tomo_set_grad", "gpu_tomo_get_x",
     "gpu_tomo_group_create", "gpu_tomo_group_solve", "gpu_tomo_group_autotune", "gpu_tomo_group_free",
     "gpu_tomo_server_create", "gpu_tomo_server_add_loop", "gpu_tomo_server_remove_loop",
     "gpu_tomo_server_submit", "gpu_tomo_server_poll", "gpu_tomo_server_drain", "gpu_tomo_server_get_x",

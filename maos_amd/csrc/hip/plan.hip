@@ -90,6 +90,21 @@ const char *kc_names[KC_N] = {"k_ttr", "k_wfs", "k_layer", "k_row", "k_col", "k_
 
 struct gpu_tomo;
 
+// ---------------------------------------------------------------------------
+// Workspace of gpu_tomo_solve_dev_noalloc: one file-scope device array per
+// GPU (held until the process ends), so that entry point needs no device
+// allocation.  Host code never uses the array's name as an address (on the
+// host it is only a stand-in): at set-up, with the device current,
+// hipGetSymbolAddress gives that device's copy, kept per device below.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_BYTES = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+namespace {
+constexpr int NOALLOC_MAXDEV = 64;
+std::mutex scratch_mu;
+void *scratch_ptr[NOALLOC_MAXDEV] = {nullptr};
+}  // namespace
+
 namespace {
 // One launch of a stage: a template variant over a subset of the jobs.
 struct Sub {
@@ -113,6 +128,9 @@ struct Batch {
     Stage zero, ttr, wfs_r, layer, wfs_l, row, col, irow, fin;
     Stage wfs_g, ttl;  // LHS with TT removal: gradient tiles, TT projection (ptt_lhs jobs)
     Stage cg;          // plain CG vector updates over the row blocks (no preconditioner)
+    // a job table to launch with instead of `jobs` (same jobs, other device
+    // records: gpu_tomo_solve_dev_noalloc's workspace); null normally
+    const Job *jobs_over = nullptr;
 };
 }  // namespace
 
@@ -1378,7 +1396,7 @@ void drop_batches(gpu_tomo *g) {
     g->fl_nsys = g->fl_k = 0;
 }
 
-LaunchTab tab(const Batch &B, const Sub &s) { return LaunchTab{B.jobs.p, s.wgi.p, s.nwg}; }
+LaunchTab tab(const Batch &B, const Sub &s) { return LaunchTab{B.jobs_over ? B.jobs_over : B.jobs.p, s.wgi.p, s.nwg}; }
 
 // Operator A v -> Ap for all jobs (WFS pass + layer pass).
 void enqueue_L(Batch &B, int vsel, int partsel, Rec &rec) {
@@ -1598,6 +1616,66 @@ void from_dev_x(gpu_tomo *g, double *dst, const float *src) {
     for (long i = 0; i < g->nxtot; i++) dst[i] = g->hstage[i];
 }
 
+// gpu_tomo_solve_dev_noalloc's set-up, done where allocating is allowed:
+// this device's address of the workspace array (looked up once per device,
+// with the device current) and the plan's one-system launch tables, which
+// that entry point reuses.  Called by gpu_tomo_setup and again by every call
+// that rebuilds the plan's launch tables.
+void noalloc_init(gpu_tomo *g) {
+    if (g->device < 0 || g->device >= NOALLOC_MAXDEV) fail(MT_EINVAL, "device index too large");
+    {
+        std::lock_guard<std::mutex> lk(scratch_mu);
+        if (!scratch_ptr[g->device]) {
+            void *ptr = nullptr;
+            HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+            if (!ptr) fail(MT_EHIP, "no device address for the workspace array");
+            scratch_ptr[g->device] = ptr;
+        }
+    }
+    (void)own_batch(g, 0, 1);
+}
+
+// Workspace layout of one system (byte offsets from the start of the
+// array, 256-byte aligned): the device record and job entry, then the
+// state buffers a solve writes besides x.
+struct NoallocLayout {
+    size_t pd, job, r, Ap, b, z, p, q, tt, gl, ttl, specf, speci, part[4], rz, alpha, res, stop, total;
+};
+NoallocLayout noalloc_layout(const gpu_tomo *g) {
+    const Geom &G = g->G;
+    const size_t rt = g->rt64 ? 8 : 4, sp = g->spec64 ? 16 : 8;
+    Geom g16 = G;
+    g16.lty = TILE;
+    const size_t maxpart = (size_t)std::max({layer_tiles(g16) * G.npsr, G.nrowblk, G.ngroup, 1});
+    NoallocLayout L{};
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o += (bytes + 255) & ~(size_t)255;
+        return at;
+    };
+    L.pd = take(sizeof(PlanDev));
+    L.job = take(sizeof(Job));
+    L.r = take(rt * G.nxtot);
+    L.Ap = take(rt * G.nxtot);
+    L.b = take(rt * G.nxtot);
+    L.z = take(4 * G.nxtot);
+    L.p = take(4 * G.nxtot);
+    L.q = take(rt * G.nwfs * G.nploc);
+    L.tt = take(4 * 2 * G.nwfs);
+    L.gl = G.ptt_lhs ? take(rt * G.ngtot) : 0;
+    L.ttl = G.ptt_lhs ? take(8 * 2 * G.nwfs) : 0;
+    L.specf = take(sp * std::max<long>(G.nspec, 1));
+    L.speci = take(8 * std::max<long>(G.nspec, 1));
+    for (int i = 0; i < 4; i++) L.part[i] = take(8 * maxpart);
+    L.rz = take(8 * (g->maxit + 2));
+    L.alpha = take(8);
+    L.res = take(8);
+    L.stop = take(4);
+    L.total = o;
+    return L;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1629,6 +1707,7 @@ int gpu_tomo_setup(const mt_recon_t *recon, int device, int nbatch, int maxit, i
         build_plan(g.get(), recon);
         alloc_state(g.get());
         HIPCHK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+        noalloc_init(g.get());
         HIPCHK(hipDeviceSynchronize());
     });
     if (rc == MT_OK) *out = g.release();
@@ -1859,6 +1938,101 @@ int gpu_tomo_solve_dev(gpu_tomo_t *g, const float *d_grad, float *d_x, int nbatc
             }
         }
         if (d_x && d_x != g->x.p) launch_copy(g->x.p, d_x, (long)nbatch * G.nxtot, st);
+    });
+}
+
+}  // extern "C"
+
+// gpu_tomo_solve_dev_noalloc's device side: the workspace is cleared by a
+// fill kernel (no memset node, so the call can be captured), and the device
+// record and job entry of the system are written from kernel arguments.
+__global__ __launch_bounds__(256) void k_noalloc_clear(uint4 *p, long n16) {
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (long)gridDim.x * blockDim.x) p[i] = z;
+}
+__global__ void k_noalloc_head(PlanDev h, Job j, PlanDev *dpd, Job *djob) {
+    if (threadIdx.x == 0) {
+        *dpd = h;
+        j.pd = dpd;
+        *djob = j;
+    }
+}
+static_assert(sizeof(PlanDev) + sizeof(Job) + 2 * sizeof(void *) <= 4096, "kernel argument size");
+
+extern "C" {
+
+int gpu_tomo_solve_dev_noalloc(gpu_tomo_t *g, const float *d_grad, float *d_x, double *d_res, int nsys,
+                               void *stream) {
+    if (!g || !d_grad || !d_x || nsys <= 0 || nsys > g->nbatch) return MT_EINVAL;
+    return guarded([&] {
+        HIPCHK(hipSetDevice(g->device));
+        if (g->prof) fail(MT_ESTATE, "gpu_tomo_solve_dev_noalloc: profiling is on (it allocates events)");
+        unsigned char *base = nullptr;
+        {
+            std::lock_guard<std::mutex> lk(scratch_mu);
+            if (g->device < NOALLOC_MAXDEV) base = (unsigned char *)scratch_ptr[g->device];
+        }
+        if (!base) fail(MT_ESTATE, "gpu_tomo_solve_dev_noalloc: workspace not set up on this device");
+        auto it = g->own.find(std::make_pair(0, 1));
+        if (it == g->own.end()) fail(MT_ESTATE, "gpu_tomo_solve_dev_noalloc: launch tables not built");
+        Batch &B = *it->second;
+        const NoallocLayout L = noalloc_layout(g);
+        if (L.total > NOALLOC_BYTES)
+            fail(MT_EINVAL, "gpu_tomo_solve_dev_noalloc: one system needs " + std::to_string(L.total) +
+                                " bytes of workspace, more than the " + std::to_string(NOALLOC_BYTES) +
+                                " bytes available");
+        hipStream_t st = stream ? (hipStream_t)stream : g->stream;
+        const Geom &G = g->G;
+        Geom g16 = G;
+        g16.lty = TILE;
+        const int maxpart = std::max({layer_tiles(g16) * G.npsr, G.nrowblk, G.ngroup, 1});
+        for (int s = 0; s < nsys; s++) {
+            // a system at a time through the one-system tables; the
+            // workspace is reused, in stream order
+            PlanDev h{};
+            h.G = G;
+            h.x = d_x + (long)s * G.nxtot;
+            h.g = const_cast<float *>(d_grad) + (long)s * G.ngtot;
+            h.z = (float *)(base + L.z);
+            h.p = (float *)(base + L.p);
+            h.tt = (float *)(base + L.tt);
+            h.r = base + L.r;
+            h.Ap = base + L.Ap;
+            h.b = base + L.b;
+            h.q = base + L.q;
+            h.specf = base + L.specf;
+            h.speci = (float2 *)(base + L.speci);
+            h.SC = g->SC;
+            h.SC.part_pap = (double *)(base + L.part[0]);
+            h.SC.part_rz = (double *)(base + L.part[1]);
+            h.SC.part_rr = (double *)(base + L.part[2]);
+            h.SC.part_bb = (double *)(base + L.part[3]);
+            h.SC.rz = (double *)(base + L.rz);
+            h.SC.alpha = (double *)(base + L.alpha);
+            h.SC.res = d_res ? d_res + s : (double *)(base + L.res);
+            h.SC.tt = h.tt;
+            h.SC.maxpart = maxpart;
+            h.stop = (int *)(base + L.stop);
+            h.ltiles = layer_tiles(G);
+            h.gl = G.ptt_lhs ? base + L.gl : nullptr;
+            h.ttl = G.ptt_lhs ? (double *)(base + L.ttl) : nullptr;
+            const long n16 = (long)(L.total / 16);
+            hipLaunchKernelGGL(k_noalloc_clear, dim3((unsigned)std::min<long>((n16 + 255) / 256, 2048)), dim3(256), 0,
+                               st, (uint4 *)base, n16);
+            hipLaunchKernelGGL(k_noalloc_head, dim3(1), dim3(64), 0, st, h, Job{nullptr, 0, 1, 1, 0},
+                               (PlanDev *)(base + L.pd), (Job *)(base + L.job));
+            HIPCHK(hipGetLastError());
+            B.jobs_over = (const Job *)(base + L.job);
+            try {
+                Rec rec{&g->evs, st, false};
+                enqueue_solve(B, rec);
+            } catch (...) {
+                B.jobs_over = nullptr;
+                throw;
+            }
+            B.jobs_over = nullptr;
+        }
+        HIPCHK(hipGetLastError());
     });
 }
 
@@ -2118,6 +2292,7 @@ void group_apply(gpu_tomo_group *grp, const int *c) {
         g->tune_colfft = c[4];
         drop_batches(g);
         apply_lty(g);
+        noalloc_init(g);
     }
     if (grp->gexec) (void)hipGraphExecDestroy(grp->gexec);
     if (grp->graph) (void)hipGraphDestroy(grp->graph);
@@ -2209,6 +2384,7 @@ int gpu_tomo_tune_set(gpu_tomo_t *g, const int *choice) {
         g->tune_flights = choice[5];
         drop_batches(g);
         apply_lty(g);
+        noalloc_init(g);
     });
 }
 
@@ -2343,6 +2519,7 @@ int gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice) {
             g->prof = save_pr;
             drop_batches(g);
         }
+        noalloc_init(g);
         if (choice) {
             choice[0] = g->tune_wfs;
             choice[1] = g->tune_layer;
@@ -2365,6 +2542,7 @@ int gpu_tomo_precision(gpu_tomo_t *g, int mode) {
         g->G.dot32 = mode == 3;
         upload_plandev(g);
         drop_batches(g);
+        noalloc_init(g);
     });
 }
 
