@@ -30,6 +30,11 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "tomo solves/s + p99 latency (30m, 6 LGS, 6 layers, 30 PCG it) vs plain GPU"
 
 
+def log(msg):
+    """Progress on stderr (the result is the one JSON line on stdout)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def algorithmic_bytes(r):
     """Unique HBM bytes each kernel must move in one launch of an UPDATE
     iteration: (per system, shared by all systems of the launch).  Vectors:
@@ -54,6 +59,32 @@ def algorithmic_bytes(r):
         # read the row spectrum, write the inverse-column spectrum; blocks
         "k_col": (16 * nspec + 8 * nspec, 8 * nbh * bs * bs),
         # read the spectrum, x and p; write x and p
+        "k_irow": (8 * nspec + 8 * nx + 8 * nx, 0),
+    }
+
+
+def algorithmic_bytes_fp32(r):
+    """The same per-launch figures with every vector stored in FP32 (and the
+    forward spectrum in complex64): the per-iteration byte model of the solve
+    with FP32 operators and vectors.  For the WFS pass (read p, write the
+    WFS fields) and the layer pass (read the WFS fields and p, write Ap) these
+    are the operator terms 4(Nx + W Np) + P_op and 4(W Np + 2 Nx) of that
+    model; the stored widths of algorithmic_bytes() add the FP64 residual
+    side of the default precision mode on top."""
+    rec = r.rec
+    nx = r.nx
+    nw = sum(1 for w in range(rec.nwfs) if not rec.skip[w])
+    nploc = rec.pnx * rec.pny
+    nsa = sum(rec.nsa[w] for w in range(rec.nwfs) if not rec.skip[w])
+    np2 = (rec.pos + 1) ** 2
+    nxl, nyl = rec.xnx[0], rec.xny[0]
+    nspec = rec.npsr * (nxl // 2 + 1) * nyl
+    bs, nbh = rec.fd_bs, rec.fd_nbhalf
+    return {
+        "k_wfs": (4 * nx + 4 * nw * nploc, nsa * (2 * np2 * 4 + 16) + 4 * nw * nploc),
+        "k_layer": (4 * nw * nploc + 4 * nx + 4 * nx, 0),
+        "k_row": (4 * nx + 4 * nx + 4 * nx + 8 * nspec, 0),
+        "k_col": (8 * nspec + 8 * nspec, 8 * nbh * bs * bs),
         "k_irow": (8 * nspec + 8 * nx + 8 * nx, 0),
     }
 
@@ -173,20 +204,25 @@ def copy_bandwidth():
     return 2 * 4 * n / best / 1e9
 
 
-def kernel_table(plan, ab, nb, d_g, d_x, stream):
+def kernel_table(plan, ab, nb, d_g, d_x, stream, ab32=None):
     """Per-kernel-class mean launch time (HIP events on the launch stream,
     read-only classes 8 launches per event pair) and achieved algorithmic
-    GB/s for one solve of nb systems."""
+    GB/s for one solve of nb systems: GBs on the FP32 byte model (ab32) and
+    GBs_stored on the widths the default precision mode stores (ab)."""
+    ab32 = ab32 or ab
     plan.profile(True, reps=8)
     plan.solve_dev(d_g, d_x, nbatch=nb, mode=0, stream=stream)
     prof = plan.profile_read()
     plan.profile(False)
     out = {}
     for k, (ms, calls) in prof.items():
-        per, shared = ab.get(k, (0, 0))
+        per, shared = ab32.get(k, (0, 0))
         by = per * nb + shared
+        per_s, shared_s = ab.get(k, (0, 0))
+        bys = per_s * nb + shared_s
         out[k] = dict(avg_us=1e3 * ms, calls=calls, total_ms=ms * calls, alg_MB=by / 1e6,
-                      GBs=(by / (ms * 1e-3) / 1e9) if ms > 0 else 0.0)
+                      GBs=(by / (ms * 1e-3) / 1e9) if ms > 0 else 0.0, stored_MB=bys / 1e6,
+                      GBs_stored=(bys / (ms * 1e-3) / 1e9) if ms > 0 else 0.0)
     return out
 
 
@@ -196,6 +232,7 @@ def bench_sweep(r, grads, maxit, sizes, copy_gbs):
     import torch
     from maos_amd import tomo
     ab = algorithmic_bytes(r)
+    ab32 = algorithmic_bytes_fp32(r)
     out = {}
     for B in sizes:
         g = tomo.GpuTomo(r, nbatch=B, maxit=maxit)
@@ -212,7 +249,7 @@ def bench_sweep(r, grads, maxit, sizes, copy_gbs):
             g.solve_dev(G.data_ptr(), X.data_ptr(), nbatch=B, mode=1, stream=st)
             torch.cuda.synchronize()
             lat.append(time.perf_counter() - t)
-        kt = kernel_table(g, ab, B, G.data_ptr(), X.data_ptr(), st)
+        kt = kernel_table(g, ab, B, G.data_ptr(), X.data_ptr(), st, ab32)
         torch.cuda.synchronize()
         g.close()
         del G, X
@@ -222,18 +259,54 @@ def bench_sweep(r, grads, maxit, sizes, copy_gbs):
                            flights=tuned[5],
                            kernels={k: dict(avg_us=round(v["avg_us"], 2), GBs=round(v["GBs"], 1),
                                             frac_of_peak=round(v["GBs"] / HBM_PEAK_GBS, 4),
+                                            frac_of_peak_stored=round(v["GBs_stored"] / HBM_PEAK_GBS, 4),
                                             frac_of_copy=round(v["GBs"] / copy_gbs, 4))
                                     for k, v in kt.items() if k in ab})
     return out
 
 
+def cpu_info():
+    """Host CPU as /proc/cpuinfo names it: model name(s), vendor, family /
+    model / stepping, the logical CPUs this process may run on and those the
+    machine shows."""
+    info = {}
+    names = []
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if ":" not in line:
+                    continue
+                k, v = (s.strip() for s in line.split(":", 1))
+                if k == "model name" and v not in names:
+                    names.append(v)
+                elif k in ("vendor_id", "cpu family", "model", "stepping") and k not in info:
+                    info[k] = v
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return dict(model_name=" / ".join(names), vendor=info.get("vendor_id"),
+                family_model_stepping="%s/%s/%s" % (info.get("cpu family"), info.get("model"), info.get("stepping")),
+                logical_cpus_usable=usable, logical_cpus_visible=os.cpu_count())
+
+
+def cpu_threads():
+    """Threads for the CPU baseline: the host CPU share of one GPU on a
+    shared GPU node (16 logical CPUs per GPU; OMP_NUM_THREADS is set to it
+    there), or fewer where fewer are usable."""
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    return max(1, min(usable, share, 16))
+
+
 def cpu_baseline(r, grads, maxit, seconds):
     from oracle.oracle import Oracle
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    cores = cpu_threads()
     o = Oracle(r, nthread=cores)
     x0, _ = o.solve(grads[0], maxit)  # warm-up + reference solution
     o.solve(grads[1 % len(grads)], maxit)
@@ -248,20 +321,14 @@ def cpu_baseline(r, grads, maxit, seconds):
         if time.perf_counter() - t0 > seconds and n >= 3:
             break
     el = time.perf_counter() - t0
-    model = ""
-    try:
-        with open("/proc/cpuinfo") as f:
-            for line in f:
-                if line.startswith("model name"):
-                    model = line.split(":", 1)[1].strip()
-                    break
-    except OSError:
-        pass
-    return dict(value=n / el, unit="solves/s", cores=cores, kind="port", cpu_model=model,
+    ci = cpu_info()
+    return dict(value=n / el, unit="solves/s", cores=cores, kind="port", cpu_model=ci["model_name"], cpu=ci,
+                threads_note="OpenMP threads = the host CPU share of one GPU (16 logical CPUs per GPU on the "
+                             "node; %s usable, %s visible)" % (ci["logical_cpus_usable"], ci["logical_cpus_visible"]),
                 p50_ms=1e3 * float(np.percentile(lat, 50)), p99_ms=1e3 * float(np.percentile(lat, 99)),
                 sample="%d cold 30-iteration FDPCG solves of the 30 m system (RHS + PCG, FP64, "
-                       "OpenMP over WFS/layers/blocks), %.1f s; p50 %.1f ms p99 %.1f ms"
-                       % (n, el, 1e3 * np.percentile(lat, 50), 1e3 * np.percentile(lat, 99))), x0
+                       "OpenMP over WFS/layers/blocks, %d threads), %.1f s; p50 %.1f ms p99 %.1f ms"
+                       % (n, cores, el, 1e3 * np.percentile(lat, 50), 1e3 * np.percentile(lat, 99))), x0
 
 
 def bench_uneven(steps, check=3):
@@ -431,32 +498,187 @@ def bench_update(cfg, r, maxit, reps=5):
                 host_threads=min(16, os.cpu_count() or 1))
 
 
-def bench_served(nsteps=40, cycles=3, rate_hz=800.0, seed=11, low_rate_hz=400.0):
-    """Served stream: 16 loops at rate_hz with Poisson-jittered arrivals,
-    time-correlated (frozen-flow, 10 m/s) gradients, warm start and 10
-    iterations per step, through the request loop (continuous batching over
-    several streams, graphs cached per loop set).  The comparator is the same
-    arrival schedule through one-at-a-time plain solves (one request per
-    launch sequence, one stream, no graph).  Latency = submit to completion."""
+def bench_fit(steps=20, batch=32, mvm_cols=512, mvm_batch=256):
+    """DM fitting after the large30 tomography (two DMs at 0 and 11.8 km,
+    the 9-direction 30" fitting ring, 100 cold CG iterations, FP64): one
+    system per call and `batch` systems per call (one workgroup per system,
+    one launch per call), against the FP64 fitting oracle on the host cores;
+    and MVM assembly (control-matrix columns: tomography then fit of each
+    unit gradient vector, on the device)."""
+    import torch
+    from maos_amd import configs, tomo, workload
+    from oracle.oracle import FitOracle
+    base = configs.large30()
+    fc = configs.fit_for(base)
+    cfg = configs.with_fit_dirs(base, fc)
+    r = tomo.Recon(cfg)
+    f = tomo.FitSetup(r, fc)
+    t = tomo.GpuTomo(r, nbatch=batch, maxit=cfg["maxit"])
+    g = tomo.GpuFit(r, f, nbatch=batch)
+    grads = [workload.make_inputs(cfg, r, 700 + k) for k in range(batch)]
+    X = t.tomo_batch(np.stack(grads))
+    dx = torch.tensor(X.astype(np.float32), device="cuda:0")
+    da = torch.zeros((batch, f.nact), dtype=torch.float64, device="cuda:0")
+    st = torch.cuda.current_stream().cuda_stream
+
+    def timed(n, reps):
+        lat = []
+        for i in range(reps + 3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            g.solve_dev(dx.data_ptr(), da.data_ptr(), nsys=n, stream=st)
+            torch.cuda.synchronize()
+            if i >= 3:
+                lat.append(time.perf_counter() - t0)
+        lat = np.array(lat)
+        return dict(solves_per_s=n * len(lat) / lat.sum(), p50_ms=1e3 * float(np.percentile(lat, 50)),
+                    p99_ms=1e3 * float(np.percentile(lat, 99)))
+
+    one = timed(1, max(100, steps))
+    many = timed(batch, max(100, steps))
+    a_gpu = da[0].cpu().numpy()
+    o = FitOracle(r, f, nthread=cpu_threads())
+    xo = X[0].astype(np.float32).astype(np.float64)
+    ao, reso = o.solve(xo, f.rec.maxit)
+    lat = []
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < 5.0 or len(lat) < 3:
+        t1 = time.perf_counter()
+        o.solve(xo, f.rec.maxit)
+        lat.append(time.perf_counter() - t1)
+    out = dict(workload="large30 fit: 2 DMs (0, 11.8 km; %d actuators), 9 fitting directions (30\" ring), "
+                        "floc %d points, %d CG iterations, cold, FP64" % (f.nact, f.nfloc, f.rec.maxit),
+               one_system=one, batch=dict(systems=batch, **many),
+               rel_l2_vs_fp64_oracle=float(np.linalg.norm(a_gpu - ao) / np.linalg.norm(ao)),
+               cpu_oracle=dict(solves_per_s=len(lat) / sum(lat), p50_ms=1e3 * float(np.median(lat)),
+                               threads=cpu_threads()))
+    t.close()
+    g.close()
+    # MVM assembly of control-matrix columns (tomography + fit per column)
+    tm = tomo.GpuTomo(r, nbatch=mvm_batch, maxit=cfg["maxit"])
+    tm.autotune(mvm_batch)
+    gm = tomo.GpuFit(r, f, nbatch=mvm_batch)
+    tm.mvm_assemble(gm, 0, mvm_batch)  # warm-up (graph capture)
+    t0 = time.perf_counter()
+    M = tm.mvm_assemble(gm, 0, mvm_cols)
+    dt = time.perf_counter() - t0
+    out["mvm_assembly"] = dict(workload="control matrix F T columns 0..%d of %d (nact %d), %d per chunk"
+                                        % (mvm_cols - 1, r.ng, f.nact, mvm_batch),
+                               columns_per_s=mvm_cols / dt, seconds=dt,
+                               full_matrix_seconds_projected=r.ng / (mvm_cols / dt),
+                               nonzero_columns=int(np.count_nonzero(np.abs(M).sum(0))))
+    tm.close()
+    gm.close()
+    return out
+
+
+_STREAM_JOBS = None
+
+
+def _stream_job(i):
     from maos_amd import tomo, workload
-    from oracle.oracle import Oracle
+    c, seed, nframes = _STREAM_JOBS[i]
+    return workload.frozen_flow_stream(c, tomo.Recon(c), seed, nframes)
+
+
+def served_streams(nframes, seed=11):
+    """The served workload's inputs: for each of the 16 loops, nframes
+    distinct time steps of its own frozen-flow atmosphere (no frame repeats).
+    Made before any GPU work, in forked worker processes (the host CPU share
+    of one GPU), since the 30 m loops' slopes take ~0.2 s per frame; this
+    process has not loaded the library (nor its GPU runtime) when it forks:
+    each worker sets up its own loop geometry."""
+    global _STREAM_JOBS
+    import multiprocessing as mp
     cfgs = served_loops()
-    recons, streams = {}, []
-    for i, c in enumerate(cfgs):
+    _STREAM_JOBS = [(c, seed + i, nframes) for i, c in enumerate(cfgs)]
+    # the 30 m loops first, so that the pool's tail is short
+    order = sorted(range(len(cfgs)), key=lambda i: -cfgs[i]["aper_d"] * len(cfgs[i]["wfs"]))
+    with mp.get_context("fork").Pool(cpu_threads()) as pool:
+        res = dict(zip(order, pool.map(_stream_job, order, chunksize=1)))
+    _STREAM_JOBS = None
+    return cfgs, [res[i] for i in range(len(cfgs))]
+
+
+def served_recons(cfgs):
+    from maos_amd import tomo
+    recons = {}
+    for c in cfgs:
         if c["name"] not in recons:
             recons[c["name"]] = tomo.Recon(c)
-        r = recons[c["name"]]
-        streams.append(workload.frozen_flow_stream(c, r, seed + i, nsteps))
+    return recons
+
+
+def served_accuracy(cfgs, recons, streams, which=(0, 4, 6), maxit=10):
+    """Per-step error of the served (warm-start, maxit-iteration) solutions
+    of three loops (an 8 m NGS, a 30 m 6-LGS and an uneven-batch loop) over
+    every step of their streams, against (a) a cold 30-iteration FP64 oracle
+    solve of the same step and (b) the FP64 oracle run through the same
+    warm-start history (maxit iterations per step from its own previous
+    solution).  The loops are replayed through a request loop of their own,
+    one step at a time: a loop's results are the same bits alone and in any
+    mix, so these are the served stream's solutions."""
+    from maos_amd import tomo
+    from oracle.oracle import Oracle
+    srv = tomo.Server(mem_budget_gb=16.0, max_inflight=2)
+    ids = [srv.add_loop(recons[cfgs[i]["name"]], maxit=maxit) for i in which]
+    nstep = len(streams[which[0]])
+    xs = {i: [] for i in which}
+    for k in range(nstep):
+        for j, i in enumerate(which):
+            srv.submit(ids[j], streams[i][k], tag=k)
+        srv.drain()
+        srv.poll(4096)
+        for j, i in enumerate(which):
+            xs[i].append(srv.get_x(ids[j])[0])
+    srv.close()
+    out = {}
+    for i in which:
+        r = recons[cfgs[i]["name"]]
+        o = Oracle(r, nthread=cpu_threads())
+        xw = None
+        e_cold, e_warm = [], []
+        for k in range(nstep):
+            g = streams[i][k]
+            xc, _ = o.solve(g, 30)
+            xw, _ = o.solve(g, maxit, x0=xw)
+            x = xs[i][k]
+            e_cold.append(float(np.linalg.norm(x - xc) / np.linalg.norm(xc)))
+            e_warm.append(float(np.linalg.norm(x - xw) / np.linalg.norm(xw)))
+        e_cold, e_warm = np.array(e_cold), np.array(e_warm)
+        out[cfgs[i]["name"]] = dict(
+            steps=nstep,
+            vs_cold30_fp64=dict(median=float(np.median(e_cold)), p99=float(np.percentile(e_cold, 99)),
+                                max=float(e_cold.max()), last10_median=float(np.median(e_cold[-10:]))),
+            vs_warm_fp64_same_history=dict(median=float(np.median(e_warm)), p99=float(np.percentile(e_warm, 99)),
+                                           max=float(e_warm.max()), first=float(e_warm[0]),
+                                           last=float(e_warm[-1])))
+    return out
+
+
+def bench_served(cfgs, recons, streams, rate_hz=800.0, seed=11, low_rate_hz=400.0, plain_rate_hz=50.0):
+    """Served stream: 16 loops at rate_hz with Poisson-jittered arrivals,
+    time-correlated (frozen-flow, 10 m/s) gradients (every request its own
+    time step), warm start and 10 iterations per step, through the request
+    loop (continuous batching over several streams, graphs cached per loop
+    set).  The comparator is the same arrival schedule through one-at-a-time
+    plain solves (one request per launch sequence, one stream, no graph).
+    Latency = submit to completion.  Both are also run at plain_rate_hz, a
+    rate the one-at-a-time comparator sustains, so that its latency there is
+    a solve's and not a queue's."""
+    from maos_amd import tomo
+    nframes = len(streams[0])
     T = 1.0 / rate_hz
     rng = np.random.default_rng(seed)
-    events = []
+    base = []
     for i in range(len(cfgs)):
         ph = rng.uniform(0, T)
-        for k in range(nsteps * cycles):
-            events.append((k * T + ph + min(rng.exponential(0.1 * T), 0.5 * T), i, k))
-    events.sort()
+        for k in range(nframes):
+            base.append((k + ph / T + min(rng.exponential(0.1), 0.5), i, k))
+    base.sort()
 
-    def run(max_inflight, max_batch, graph):
+    def run(max_inflight, max_batch, graph, hz):
+        events = [(t / hz, i, k) for t, i, k in base]
         srv = tomo.Server(mem_budget_gb=32.0, max_inflight=max_inflight, max_batch=max_batch, use_graph=graph)
         ids = [srv.add_loop(recons[c["name"]], maxit=10) for c in cfgs]
         # warm-up: one step per loop (graph capture, first touch); the
@@ -467,66 +689,44 @@ def bench_served(nsteps=40, cycles=3, rate_hz=800.0, seed=11, low_rate_hz=400.0)
         srv.poll(4096)
         tuned = srv.autotune() if graph else None
 
-        def stream(upto, tagged):
+        def stream(evs, tagged):
             t0 = time.perf_counter()
-            for t, i, k in events:
-                if t > upto:
-                    break
+            for t, i, k in evs:
                 while time.perf_counter() - t0 < t:
                     pass
-                srv.submit(ids[i], streams[i][k % nsteps], tag=k if tagged else -1)
+                srv.submit(ids[i], streams[i][k], tag=k if tagged else -1)
             srv.drain()
             return t0
 
         # warm-up: the first 50 ms of the schedule, untimed (the first
         # stream of a process otherwise pays a one-off stall of 15-30 ms)
-        stream(0.05, False)
+        stream([e for e in events if e[0] <= 0.05], False)
         srv.poll(4096)
         done = []
-        t0 = stream(float("inf"), True)
+        t0 = stream(events, True)
         t1 = time.perf_counter()
         while True:
             d = srv.poll(4096)
             if not d:
                 break
             done.extend(d)
+        srv.close()
         lat = np.array([d[3] for d in done]) * 1e-3
-        out = dict(requests=len(done), solves_per_s=len(done) / (t1 - t0), p50_ms=float(np.percentile(lat, 50)),
-                   p99_ms=float(np.percentile(lat, 99)), max_ms=float(lat.max()), wall_s=t1 - t0)
+        out = dict(offered_per_s=len(cfgs) * hz, requests=len(done), solves_per_s=len(done) / (t1 - t0),
+                   p50_ms=float(np.percentile(lat, 50)), p99_ms=float(np.percentile(lat, 99)),
+                   max_ms=float(lat.max()), wall_s=t1 - t0)
         if tuned:
             out["autotuned"] = dict(wfs=tuned[0], layer=tuned[1], layer_tile_rows=tuned[3],
                                     column_fft="registers" if tuned[4] != 2 else "lds")
-        return srv, ids, out
+        return out
 
-    srv, ids, served = run(3, 64, True)
-    # per-step error of the warm 10-iteration solution against a cold
-    # 30-iteration FP64 oracle solve of the same step (three loops, next steps)
-    errs = []
-    for i in (0, 4, 6):
-        r = recons[cfgs[i]["name"]]
-        g = streams[i][(nsteps * cycles) % nsteps]
-        srv.submit(ids[i], g, tag=-2)
-        srv.drain()
-        x, _ = srv.get_x(ids[i])
-        xo, _ = Oracle(r, nthread=8).solve(g, 30)
-        errs.append(float(np.linalg.norm(x - xo) / np.linalg.norm(xo)))
-    srv.close()
-    srv_p, _, plain = run(1, 1, False)
-    srv_p.close()
-    out = dict(workload="16 loops at %g Hz (4 x 8 m NGS, 2 x 30 m 6-LGS, 10 x uneven-batch systems), "
-                        "frozen flow 10 m/s, Poisson-jittered arrivals, warm start, 10 iterations" % rate_hz,
-               offered_per_s=len(cfgs) * rate_hz, served=served, one_at_a_time_plain=plain,
-               rel_l2_warm10_vs_cold30_fp64=errs)
-    if low_rate_hz:
-        # the same stream at a rate the request loop sustains: latency
-        # without a queue building up (the plain comparator still saturates)
-        events[:] = [(t * rate_hz / low_rate_hz, i, k) for t, i, k in events]
-        srv, _, served_lo = run(3, 64, True)
-        srv.close()
-        srv_p, _, plain_lo = run(1, 1, False)
-        srv_p.close()
-        out["at_%g_hz" % low_rate_hz] = dict(offered_per_s=len(cfgs) * low_rate_hz, served=served_lo,
-                                             one_at_a_time_plain=plain_lo)
+    out = dict(workload="16 loops (4 x 8 m NGS, 2 x 30 m 6-LGS, 10 x uneven-batch systems), frozen flow 10 m/s, "
+                        "%d distinct time steps per loop, Poisson-jittered arrivals, warm start, 10 iterations"
+                        % nframes)
+    for hz in (rate_hz, low_rate_hz, plain_rate_hz):
+        if not hz:
+            continue
+        out["at_%g_hz" % hz] = dict(served=run(3, 64, True, hz), one_at_a_time_plain=run(1, 1, False, hz))
     return out
 
 
@@ -540,7 +740,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", type=int, default=None,
                     help="solve precision mode (0, 1, 2; default: the library default)")
-    ap.add_argument("--modes", default="plain,graph,batched,uneven,served,mvm,update,precision,sweep",
+    ap.add_argument("--modes", default="plain,graph,batched,uneven,served,mvm,fit,update,precision,sweep",
                     help="comma list of modes to time (profiling runs use one)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 3 device copies of 1 GiB (known HBM traffic) for counter scaling")
@@ -569,6 +769,12 @@ def main():
               "independent solves would shard across GPUs with no communication). "
               "Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
+
+    modes = set(args.modes.split(","))
+    # the served stream's inputs are made first, in forked workers, before
+    # this process loads the library and its GPU runtime
+    served_inputs = served_streams(120) if "served" in modes else None
+    log("served stream inputs ready")
 
     import torch
 
@@ -601,7 +807,6 @@ def main():
         return dict(solves_per_s=nb * len(lat) / lat.sum(), p50_ms=1e3 * float(np.percentile(lat, 50)),
                     p99_ms=1e3 * float(np.percentile(lat, 99)), ms_per_step=1e3 * float(lat.mean()))
 
-    modes = set(args.modes.split(","))
     plain = tomo.GpuTomo(r, nbatch=1, maxit=maxit)
     batched = tomo.GpuTomo(r, nbatch=B, maxit=maxit)
     if args.precision is not None:
@@ -613,13 +818,20 @@ def main():
     else:
         tuned = batched.autotune(B)
     empty = dict(solves_per_s=float("nan"), p50_ms=float("nan"), p99_ms=float("nan"), ms_per_step=float("nan"))
-    res_plain = run_mode(plain, 1, 0, args.steps, args.warmup) if "plain" in modes else empty
-    res_graph = run_mode(plain, 1, 1, args.steps, args.warmup) if "graph" in modes else empty
-    nsteps_b = max(10, args.steps // 4)
-    # timed region of the headline mode: barrier + synchronize on both sides
+    # latency percentiles need enough calls: at least 100 per mode
+    ncall = max(100, args.steps)
+    res_plain = run_mode(plain, 1, 0, ncall, args.warmup) if "plain" in modes else empty
+    res_graph = run_mode(plain, 1, 1, ncall, args.warmup) if "graph" in modes else empty
+    # headline: exactly --warmup untimed and --steps timed calls, the timed
+    # region bracketed by a device synchronize on both sides (one GPU: no
+    # other rank to wait for)
     torch.cuda.synchronize()
-    res_batch = run_mode(batched, B, 1, nsteps_b, max(2, args.warmup // 4))
+    res_batch = run_mode(batched, B, 1, args.steps, args.warmup)
     torch.cuda.synchronize()
+    # p50 / p99 of the batched call over >= 100 calls (the headline's own
+    # calls when --steps is at least 100)
+    res_lat = res_batch if args.steps >= 100 else run_mode(batched, B, 1, ncall, 0)
+    log("headline: %.1f solves/s" % res_batch["solves_per_s"])
 
     # accuracy of system 0 (batched path) against the FP64 CPU oracle
     x_gpu = d_x[0].double().cpu().numpy()
@@ -629,16 +841,23 @@ def main():
     # the launch stream; the read-only kernel classes launched 8 times back
     # to back per event pair, so the average excludes the event overhead)
     ab = algorithmic_bytes(r)
-    kernels = kernel_table(batched, ab, B, d_g.data_ptr(), d_x.data_ptr(), sp)
+    ab32 = algorithmic_bytes_fp32(r)
+    kernels = kernel_table(batched, ab, B, d_g.data_ptr(), d_x.data_ptr(), sp, ab32)
     torch.cuda.synchronize()
     dom = max((k for k in kernels if k in ab), key=lambda k: kernels[k]["total_ms"])
     kd = kernels[dom]
     copy_gbs = copy_bandwidth()
+    # achieved = the FP32 byte model of the dominant kernel's launch / its
+    # mean duration; the bytes the default precision mode stores (FP64
+    # residual side) are reported beside it
     roofline = dict(bound="hbm", achieved=round(kd["GBs"], 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(kd["GBs"] / HBM_PEAK_GBS, 4), traffic=measured_traffic(dom, B), kernel=dom,
-                    avg_us=round(kd["avg_us"], 2), alg_bytes_per_launch=int(ab[dom][0] * B + ab[dom][1]),
+                    avg_us=round(kd["avg_us"], 2), alg_bytes_per_launch=int(ab32[dom][0] * B + ab32[dom][1]),
+                    stored_bytes_per_launch=int(ab[dom][0] * B + ab[dom][1]),
+                    achieved_stored=round(kd["GBs_stored"], 1), frac_stored=round(kd["GBs_stored"] / HBM_PEAK_GBS, 4),
                     copy_GBs=round(copy_gbs, 1), frac_of_copy=round(kd["GBs"] / copy_gbs, 4))
 
+    log("kernel table and copy bandwidth done")
     sweep = None
     if "sweep" in modes:
         try:
@@ -646,6 +865,7 @@ def main():
         except Exception as e:  # reported, not fatal for the headline line
             sweep = {"error": repr(e)}
 
+    log("sweep done")
     uneven = None
     if "uneven" in modes:
         try:
@@ -653,6 +873,7 @@ def main():
         except Exception as e:  # reported, not fatal for the headline line
             uneven = {"error": repr(e)}
 
+    log("uneven done")
     precision = None
     if "precision" in modes:
         try:
@@ -660,6 +881,7 @@ def main():
         except Exception as e:  # reported, not fatal for the headline line
             precision = {"error": repr(e)}
 
+    log("precision done")
     update = None
     if "update" in modes:
         try:
@@ -667,6 +889,7 @@ def main():
         except Exception as e:
             update = {"error": repr(e)}
 
+    log("update done")
     mvm = None
     if "mvm" in modes:
         try:
@@ -674,13 +897,26 @@ def main():
         except Exception as e:
             mvm = {"error": repr(e)}
 
+    log("mvm done")
+    fit = None
+    if "fit" in modes:
+        try:
+            fit = bench_fit(args.steps)
+        except Exception as e:  # reported, not fatal for the headline line
+            fit = {"error": repr(e)}
+
+    log("fit done")
     served = None
     if "served" in modes:
         try:
-            served = bench_served()
+            cfgs_s, streams_s = served_inputs
+            recons_s = served_recons(cfgs_s)
+            served = bench_served(cfgs_s, recons_s, streams_s)
+            served["per_step_error"] = served_accuracy(cfgs_s, recons_s, streams_s)
         except Exception as e:  # reported, not fatal for the headline line
             served = {"error": repr(e)}
 
+    log("served done")
     cpu = None
     err = None
     if not args.no_cpu:
@@ -692,8 +928,8 @@ def main():
         "value": round(res_batch["solves_per_s"], 2),
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": nsteps_b,
-        "warmup": max(2, args.warmup // 4),
+        "steps": args.steps,
+        "warmup": args.warmup,
         "ms_per_step": round(res_batch["ms_per_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
@@ -707,9 +943,11 @@ def main():
                    "layer_tile_rows": tuned[3] if len(tuned) > 3 else None,
                    "column_fft": {1: "registers", 2: "lds"}.get(tuned[4]) if len(tuned) > 4 else None,
                    "flights": tuned[5] if len(tuned) > 5 else None},
-        "p50_ms": round(res_batch["p50_ms"], 4),
-        "p99_ms": round(res_batch["p99_ms"], 4),
-        "modes": {"plain": res_plain, "graph": res_graph, "batched": res_batch},
+        "p50_ms": round(res_lat["p50_ms"], 4),
+        "p99_ms": round(res_lat["p99_ms"], 4),
+        "latency_calls": ncall,
+        "modes": {"plain": res_plain, "graph": res_graph, "batched": res_batch,
+                  "batched_latency": res_lat},
         "speedup_vs_plain": round(res_batch["solves_per_s"] / res_plain["solves_per_s"], 2),
         "accuracy": {"rel_l2_vs_fp64_oracle": err, "cg_residual": float(res_gpu)},
         "roofline": roofline,
@@ -718,6 +956,7 @@ def main():
         "uneven64": uneven,
         "served16": served,
         "mvm_tomography": mvm,
+        "dm_fit": fit,
         "cn2_update": update,
         "precision_study": precision,
         "batch_sweep": sweep,

@@ -438,6 +438,170 @@ int  gpu_tomo_profile_read(gpu_tomo_t *g, const char **names, double *ms,
                            double *calls, int *n);
 
 /* ------------------------------------------------------------------ */
+/* 3. DM fitting (the solve after tomography)                          */
+/* ------------------------------------------------------------------ */
+/*
+ * The fitting step of MAOS's minimum-variance reconstructor
+ * (recon.alg=0, fit.alg=1 CG): the DM commands a minimise the aperture-
+ * weighted piston-removed residual between the tomography estimate x seen
+ * through the fitting directions and the DMs,
+ *
+ *   (sum_f wt_f HA_f' W HA_f + NW NW' + S + tikcr*4/nact I) a
+ *       = sum_f wt_f HA_f' W HX_f x,                W = W0 - W1 W1',
+ *
+ * as FitR / FitL (maos/recon_utils.c:483-560, with the assembled matrix's
+ * Tikhonov term, maos/setup_recon_fit.c:270-283) and the GPU's grid form
+ * cufit_grid (cuda/fit.cu:47-230, gpu.fit=2: fit.square=1, tomo.square=1).
+ * HA_f: bilinear interpolation from each DM's square actuator grid to the
+ * fitting grid floc along fitting direction f (mkh, lib/mkh.c:47-135, the
+ * linear influence function dm.iac=0); HX_f: the same from the tomography
+ * layers; W0, W1: the bilinear-influence aperture weighting on floc
+ * (mkw_circular, lib/mkw.c:363-439); NW: the piston and upper-DM tip/tilt
+ * low-rank terms (setup_fit_lrt, maos/setup_recon_fit.c:104-171); S: the
+ * slaving regulariser of weakly coupled actuators (slaving,
+ * lib/slaving.c:63-227, with the coupling of genactcpl, :24-55).
+ * Not supported (MT_EINVAL): dm.iac != 0 (cubic influence functions) and
+ * fit.actinterp=1 (HA replaced by HA*actinterp); the reference's grid path
+ * (gpu.fit=2) does not apply actinterp either.
+ */
+typedef struct {
+    double ht;       /* dm.ht (m)                                          */
+    double dx;       /* dm.dx (m); dm.ar = 1                               */
+    double offset;   /* dm.offset (fraction of dx)                         */
+    double guard;    /* dm.guard (actuator rings)                          */
+    double iac;      /* dm.iac; must be 0                                  */
+} mt_dm_t;
+
+typedef struct {
+    int    ndm;
+    const mt_dm_t *dm;
+    int    nfit;                   /* fit directions                        */
+    const double *thetax, *thetay; /* fit.thetax/thetay (radian)            */
+    const double *wt;              /* fit.wt (normalised to sum |wt| = 1)   */
+    const double *hs;              /* fit.hs (m); INFINITY for science      */
+    int    pos;                    /* fit.pos; <= 0: tomo.pos               */
+    int    maxit;                  /* fit.maxit; 0: 10 (warm) / 100 (cold) as
+                                      maos/setup_parms.c:2381-2385          */
+    int    warm;                   /* recon.warm_restart                    */
+    int    lrt_piston, lrt_tt;     /* fit.lrt_piston, fit.lrt_tt            */
+    int    actslave;               /* fit.actslave                          */
+    double actthres;               /* fit.actthres                          */
+    double tikcr;                  /* fit.tikcr                             */
+    int    actinterp;              /* fit.actinterp; must be 0              */
+    int    annular_W;              /* dbg.annular_W                         */
+} mt_fit_parms_t;
+
+/* POD view of the fitting operators (RECON_T->fit fields on this path).
+ * Grids are square maps (fit.square=1); vectors are stacked per DM,
+ * index ix + iy*anx[dm]. */
+typedef struct {
+    int    ndm;
+    const long   *anx, *any;       /* actuator grids (recon->amap)           */
+    const double *aox, *aoy, *adx, *aht;
+    long   fnx, fny;               /* fitting grid floc (recon->fmap)        */
+    double fox, foy, fdx;
+    int    nfit;
+    const double *thetax, *thetay, *wt, *hs;
+    const double *W0;              /* 9 entries per floc point: W0(i, j) for
+                                      j at offsets (dx, dy) in {-1,0,1}^2,
+                                      index (dy+1)*3 + dx+1                  */
+    const double *W1;              /* per floc point                         */
+    int    nnw;                    /* low-rank columns (NW)                  */
+    const double *NW;              /* nact_total x nnw, column-major         */
+    /* S in CSR over all actuators (stacked DMs; block diagonal) or NULL     */
+    const long   *sl_p, *sl_i;
+    const double *sl_x;
+    double tikdiag;                /* tikcr * 4 / nact_total                 */
+    const double *actcpl;          /* per actuator coupling (genactcpl)      */
+    int    maxit, warm;
+} mt_fit_recon_t;
+
+typedef struct mt_fit mt_fit_t;
+
+/* Fitting geometry and operators for the tomography geometry of s
+ * (setup_recon_floc/aloc, setup_recon_fit: maos/setup_recon_prep.c:112-165,
+ * 271-360, maos/setup_recon_fit.c:404-453). */
+int  mt_fit_create(const mt_setup_t *s, const mt_fit_parms_t *p, mt_fit_t **out);
+int  mt_fit_recon(const mt_fit_t *f, mt_fit_recon_t *out);
+/* Actuator count (sum over DMs) and floc points. */
+int  mt_fit_sizes(const mt_fit_t *f, long *nact, long *nfloc);
+void mt_fit_free(mt_fit_t *f);
+
+typedef struct gpu_fit gpu_fit_t;
+
+/* Upload the fitting operators for nbatch independent systems; the
+ * tomography layers x come in the layout of recon (the tomography's
+ * mt_recon_t).  Replaces the fitting half of gpu_setup_recon
+ * (curecon_t::update -> cufit_grid, cuda/recon.cu:104-125, cuda/fit.cu:47-117). */
+int  gpu_fit_setup(const mt_recon_t *recon, const mt_fit_recon_t *fit, int device,
+                   int nbatch, gpu_fit_t **out);
+/* Fit of one system (slot isys, host doubles): opdr (tomography layers, as
+ * gpu_tomo_solve returns them) -> RHS -> CG (maxit iterations, warm start
+ * from the slot's previous DM commands when warm) -> dmfit (stacked DM
+ * vectors); cgres = sqrt(r'r / b'b).  Replaces curecon_t::fit
+ * (cuda/recon.cu:277-288). */
+int  gpu_fit_solve(gpu_fit_t *g, int isys, const double *opdr, double *dmfit,
+                   double *cgres);
+/* nsys <= nbatch systems with device buffers (FP32 layer vectors in, FP64
+ * DM vectors out), enqueued on stream (0: the plan's own); no host sync.
+ * One workgroup per system runs the whole CG (one launch). */
+int  gpu_fit_solve_dev(gpu_fit_t *g, const float *d_opdr, double *d_dmfit,
+                       int nsys, void *stream);
+/* Operators for parity checks (host doubles, one system):
+ *   R : out = alpha * FitR x              (cufit_grid::R)
+ *   L : out = beta * out + alpha * FitL a (cufit_grid::L) */
+int  gpu_fit_R(gpu_fit_t *g, const double *opdr, double *out, double alpha);
+int  gpu_fit_L(gpu_fit_t *g, const double *a, double *out, double alpha, double beta);
+/* Residuals of the last solve. */
+int  gpu_fit_residuals(gpu_fit_t *g, double *cgres, int nsys);
+/* MVM assembly: columns [col0, col0 + ncol) of the control matrix
+ * M = F T (DM commands per gradient), each the cold tomography solve of a
+ * unit gradient vector (t's maxit, nbatch columns per graph solve) followed
+ * by the cold fit of that solution (f's maxit) on the device, into out
+ * (host, nact x ncol doubles, column-major).  Columns of WFS the
+ * tomography does not use are zero.  t's warm-start state is kept.  The
+ * tomography-then-fitting assembly of mvm_direct (cuda/mvm_direct.cu:
+ * 107-134; per column, as cuda/mvm_trans.cu:98-148 distributes them). */
+int  gpu_mvm_assemble(gpu_tomo_t *t, gpu_fit_t *f, long col0, long ncol,
+                      double *out);
+void gpu_fit_free(gpu_fit_t *g);
+
+/* ------------------------------------------------------------------ */
+/* MAOS .conf files                                                    */
+/* ------------------------------------------------------------------ */
+/* The key store MAOS fills from its .conf files and command-line
+ * overrides (open_config, lib/readcfg.c:236-456) and the value parsers it
+ * reads keys with (readcfg_str / readcfg_dmat / readcfg_strarr,
+ * lib/readcfg.c:523-640; readstr_numarr, sys/readstr.c:193-420), so that a
+ * MAOS configuration can set up this library (maos_amd.conf maps the keys
+ * of this path to mt_parms_t / mt_fit_parms_t as setup_parms does).
+ * mt_conf_read takes a file name ending in .conf (found as given, then in
+ * the directories of mt_conf_add_path, most recent first) or key=value text
+ * (lines; what MAOS accepts on its command line).  A key read again at the
+ * same or a higher priority replaces the earlier value; key += [..]
+ * appends to an array.  Every get marks the key as used; mt_conf_unused
+ * lists the keys never read (MAOS's close_config refuses them). */
+typedef struct mt_conf mt_conf_t;
+int  mt_conf_new(mt_conf_t **out);
+void mt_conf_free(mt_conf_t *c);
+int  mt_conf_add_path(mt_conf_t *c, const char *dir);
+int  mt_conf_read(mt_conf_t *c, const char *file_or_text, const char *prefix,
+                  int priority);
+int  mt_conf_has(const mt_conf_t *c, const char *key);
+/* The value with paired quotes removed (empty: n = 0); buf may be NULL to
+ * query the length n. */
+int  mt_conf_str(mt_conf_t *c, const char *key, char *buf, long len, long *n);
+/* String array element i (i < 0: the count into n). */
+int  mt_conf_strarr(mt_conf_t *c, const char *key, long i, char *buf, long len,
+                    long *n);
+/* Numeric array (column-major, shape nrow x ncol; a value naming a .bin or
+ * .bin.gz file is that file's double array); maxn = 0 queries the count. */
+int  mt_conf_dbl(mt_conf_t *c, const char *key, double *out, long maxn,
+                 long *n, long *nrow, long *ncol);
+/* Keys never read, space separated, and their count. */
+int  mt_conf_unused(const mt_conf_t *c, char *buf, long len, long *n);
+
+/* ------------------------------------------------------------------ */
 /* MAOS .bin files                                                     */
 /* ------------------------------------------------------------------ */
 /* Double arrays and cells of them in MAOS's .bin layout (sys/bin.c:513-811,

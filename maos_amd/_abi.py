@@ -71,6 +71,26 @@ class MtFdpcgOut(C.Structure):
                 ("Mhat", dp), ("Minv", dp), ("perm", lp), ("permhf", lp)]
 
 
+class MtDm(C.Structure):
+    _fields_ = [("ht", C.c_double), ("dx", C.c_double), ("offset", C.c_double), ("guard", C.c_double),
+                ("iac", C.c_double)]
+
+
+class MtFitParms(C.Structure):
+    _fields_ = [("ndm", C.c_int), ("dm", C.POINTER(MtDm)), ("nfit", C.c_int), ("thetax", dp), ("thetay", dp),
+                ("wt", dp), ("hs", dp), ("pos", C.c_int), ("maxit", C.c_int), ("warm", C.c_int),
+                ("lrt_piston", C.c_int), ("lrt_tt", C.c_int), ("actslave", C.c_int), ("actthres", C.c_double),
+                ("tikcr", C.c_double), ("actinterp", C.c_int), ("annular_W", C.c_int)]
+
+
+class MtFitRecon(C.Structure):
+    _fields_ = [("ndm", C.c_int), ("anx", lp), ("any", lp), ("aox", dp), ("aoy", dp), ("adx", dp), ("aht", dp),
+                ("fnx", C.c_long), ("fny", C.c_long), ("fox", C.c_double), ("foy", C.c_double),
+                ("fdx", C.c_double), ("nfit", C.c_int), ("thetax", dp), ("thetay", dp), ("wt", dp), ("hs", dp),
+                ("W0", dp), ("W1", dp), ("nnw", C.c_int), ("NW", dp), ("sl_p", lp), ("sl_i", lp),
+                ("sl_x", dp), ("tikdiag", C.c_double), ("actcpl", dp), ("maxit", C.c_int), ("warm", C.c_int)]
+
+
 _lib = None
 
 
@@ -135,6 +155,30 @@ def lib():
     L.gpu_tomo_precision.argtypes = [vp, C.c_int]
     L.gpu_tomo_profile.argtypes = [vp, C.c_int]
     L.gpu_tomo_profile_read.argtypes = [vp, C.POINTER(C.c_char_p), dp, dp, ip]
+    L.mt_fit_create.argtypes = [vp, C.POINTER(MtFitParms), C.POINTER(vp)]
+    L.mt_fit_recon.argtypes = [vp, C.POINTER(MtFitRecon)]
+    L.mt_fit_sizes.argtypes = [vp, lp, lp]
+    L.mt_fit_free.argtypes = [vp]
+    L.mt_fit_free.restype = None
+    L.gpu_fit_setup.argtypes = [C.POINTER(MtRecon), C.POINTER(MtFitRecon), C.c_int, C.c_int, C.POINTER(vp)]
+    L.gpu_fit_solve.argtypes = [vp, C.c_int, dp, dp, dp]
+    L.gpu_fit_solve_dev.argtypes = [vp, vp, vp, C.c_int, vp]
+    L.gpu_fit_R.argtypes = [vp, dp, dp, C.c_double]
+    L.gpu_fit_L.argtypes = [vp, dp, dp, C.c_double, C.c_double]
+    L.gpu_fit_residuals.argtypes = [vp, dp, C.c_int]
+    L.gpu_mvm_assemble.argtypes = [vp, vp, C.c_long, C.c_long, dp]
+    L.gpu_fit_free.argtypes = [vp]
+    L.gpu_fit_free.restype = None
+    L.mt_conf_new.argtypes = [C.POINTER(vp)]
+    L.mt_conf_free.argtypes = [vp]
+    L.mt_conf_free.restype = None
+    L.mt_conf_add_path.argtypes = [vp, C.c_char_p]
+    L.mt_conf_read.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_int]
+    L.mt_conf_has.argtypes = [vp, C.c_char_p]
+    L.mt_conf_str.argtypes = [vp, C.c_char_p, C.c_char_p, C.c_long, lp]
+    L.mt_conf_strarr.argtypes = [vp, C.c_char_p, C.c_long, C.c_char_p, C.c_long, lp]
+    L.mt_conf_dbl.argtypes = [vp, C.c_char_p, dp, C.c_long, lp, lp, lp]
+    L.mt_conf_unused.argtypes = [vp, C.c_char_p, C.c_long, lp]
     L.mt_bin_read.argtypes = [C.c_char_p, C.POINTER(vp)]
     L.mt_bin_write.argtypes = [C.c_char_p, vp]
     L.mt_bin_free.argtypes = [vp]
@@ -167,6 +211,10 @@ EXPORTS = [
     "gpu_tomo_server_submit", "gpu_tomo_server_poll", "gpu_tomo_server_drain", "gpu_tomo_server_get_x",
     "gpu_tomo_server_usage", "gpu_tomo_server_autotune", "gpu_tomo_server_update_cn2", "gpu_tomo_server_free", "gpu_tomo_autotune", "gpu_tomo_tune_set", "gpu_tomo_info", "gpu_tomo_residuals", "gpu_tomo_L", "gpu_tomo_R", "gpu_tomo_P",
     "gpu_tomo_sizes", "gpu_tomo_profile", "gpu_tomo_profile_read", "gpu_tomo_precision",
+    "mt_fit_create", "mt_fit_recon", "mt_fit_sizes", "mt_fit_free", "gpu_fit_setup", "gpu_fit_solve",
+    "gpu_fit_solve_dev", "gpu_fit_R", "gpu_fit_L", "gpu_fit_residuals", "gpu_fit_free", "gpu_mvm_assemble",
+    "mt_conf_new", "mt_conf_free", "mt_conf_add_path", "mt_conf_read", "mt_conf_has", "mt_conf_str",
+    "mt_conf_strarr", "mt_conf_dbl", "mt_conf_unused",
     "mt_bin_read", "mt_bin_write", "mt_bin_free", "mt_bin_info", "mt_bin_data", "mt_bin_child",
     "mt_bin_header", "mt_bin_new_dbl", "mt_bin_new_cell", "mt_bin_set", "mt_bin_grad", "mt_bin_opdr",
 ]

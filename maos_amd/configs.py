@@ -89,6 +89,62 @@ def batch64(seed=3):
     return out
 
 
+def fit_for(cfg, **kw):
+    """DM fitting parameters for a tomography config (MAOS fit.* / dm.* keys,
+    config/maos/recon.conf:56-72, sim.conf:181-187): on-axis fitting on
+    8-10 m apertures with one ground DM (dm_single.conf), and on larger ones
+    the 9-direction 30" ring of fit_cir30.conf with two DMs at 0 and 11.8 km
+    (dm_dual.conf, offset 0 and 1/2); dm.dx = 0.5 m, dm.guard = 1, linear
+    influence functions (dm.iac = 0), fit.actinterp = 0, cold start
+    (fit.maxit 0 -> 100 iterations)."""
+    if cfg["aper_d"] <= 10:
+        dms = [dict(ht=0.0, dx=0.5, offset=0.0, guard=1.0, iac=0.0)]
+        th = [(0.0, 0.0, 1.0)]
+    else:
+        dms = [dict(ht=0.0, dx=0.5, offset=0.0, guard=1.0, iac=0.0),
+               dict(ht=11800.0, dx=0.5, offset=0.5, guard=1.0, iac=0.0)]
+        r = [0, 0.5, 0.3536, 0, -0.3536, -0.5, -0.3536, 0, 0.3536]
+        q = [0, 0, 0.3536, 0.5, 0.3536, 0, -0.3536, -0.5, -0.3536]
+        w = [1 / 2, 1 / 12, 1 / 24, 1 / 12, 1 / 24, 1 / 12, 1 / 24, 1 / 12, 1 / 24]
+        th = [(a * 30, b * 30, c) for a, b, c in zip(r, q, w)]
+    c = dict(dm=dms, fit=th, fit_hs=[INF] * len(th), fit_pos=0, fit_maxit=0, warm=0, lrt_piston=1,
+             lrt_tt=1, actslave=1, actthres=0.1, tikcr=1e-7, actinterp=0, annular_W=0)
+    c.update(kw)
+    return c
+
+
+def with_fit_dirs(cfg, fc):
+    """The tomography config with the fitting directions among its beam
+    directions, as MAOS lists them in parms->dirs (maos/setup_parms.c:
+    1945-1984), so that the layer grids cover every fitting direction."""
+    dirs = list(cfg.get("dirs", []))
+    for (tx, ty, _), hs in zip(fc["fit"], fc["fit_hs"]):
+        if (tx, ty, hs) not in dirs:
+            dirs.append((tx, ty, hs))
+    c = dict(cfg)
+    c["dirs"] = dirs
+    c["name"] = cfg["name"] + "_fit"
+    return c
+
+
+class FitParms:
+    """Owns the ctypes arrays behind one mt_fit_parms_t."""
+
+    def __init__(self, fc):
+        self.fc = fc
+        dms = fc["dm"]
+        self.dm = (_abi.MtDm * len(dms))(*[_abi.MtDm(d["ht"], d["dx"], d["offset"], d["guard"], d["iac"])
+                                             for d in dms])
+        n = len(fc["fit"])
+        self.tx = (C.c_double * n)(*[f[0] * AS2RAD for f in fc["fit"]])
+        self.ty = (C.c_double * n)(*[f[1] * AS2RAD for f in fc["fit"]])
+        self.wt = (C.c_double * n)(*[f[2] for f in fc["fit"]])
+        self.hs = (C.c_double * n)(*fc["fit_hs"])
+        self.p = _abi.MtFitParms(len(dms), self.dm, n, self.tx, self.ty, self.wt, self.hs, fc["fit_pos"],
+                                 fc["fit_maxit"], fc["warm"], fc["lrt_piston"], fc["lrt_tt"], fc["actslave"],
+                                 fc["actthres"], fc["tikcr"], fc["actinterp"], fc["annular_W"])
+
+
 CONFIGS = {"small8": small8, "large30": large30, "lgs8_flat": lgs8_flat}
 
 

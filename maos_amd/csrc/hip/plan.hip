@@ -30,6 +30,7 @@
 #include <thread>
 #include <vector>
 
+#include "fit_internal.hpp"
 #include "kernels.hpp"
 #include "launch.hpp"
 #include "maos_tomo.h"
@@ -2036,104 +2037,140 @@ int gpu_tomo_solve_dev_noalloc(gpu_tomo_t *g, const float *d_grad, float *d_x, d
     });
 }
 
-int gpu_tomo_mvm_tomo(gpu_tomo_t *g, long col0, long ncol, double *out) {
-    if (!g || !out || col0 < 0 || ncol <= 0 || col0 + ncol > g->ngtot_all) return MT_EINVAL;
-    return guarded([&] {
-        HIPCHK(hipSetDevice(g->device));
-        const Geom &G = g->G;
-        const long nx = g->nxtot;
-        hipStream_t st = g->stream;
-        // device gradient index of each requested column; the columns of a
-        // WFS the solve does not use are zero
-        std::vector<int> kof(g->goff_all.size() - 1, -1);
-        for (int k = 0; k < G.nwfs; k++) kof[g->wfs_active[k]] = k;
-        std::vector<int> idx;
-        std::vector<long> col;
-        for (long j = col0; j < col0 + ncol; j++) {
-            const int w = (int)(std::upper_bound(g->goff_all.begin(), g->goff_all.end(), j) - g->goff_all.begin()) - 1;
-            const int k = kof[w];
-            if (k < 0) {
-                std::fill(out + (j - col0) * nx, out + (j - col0 + 1) * nx, 0.0);
-            } else {
-                idx.push_back((int)(G.goff[k] + (j - g->goff_all[w])));
-                col.push_back(j - col0);
-            }
+}  // extern "C"
+
+// Columns [col0, col0 + ncol) of MVM assembly, each the cold solve of a unit
+// gradient vector e_j: with f null the tomography solution x = T e_j (nx
+// per column), else the DM commands a = F T e_j of the fit of that solution
+// (nact per column), written to out column-major as doubles.  Chunks of
+// nbatch columns: the unit vectors are written on the device from an index
+// table, the tomography runs as a graph solve, the fit (if any) reads its x
+// in place; two pinned staging buffers let the host copy chunk t-1 out
+// while the GPU solves chunk t.  The slots' gradients and solutions (the
+// warm-start state) are restored at the end.
+static void mvm_columns(gpu_tomo *g, gpu_fit_t *f, long col0, long ncol, double *out) {
+    HIPCHK(hipSetDevice(g->device));
+    const Geom &G = g->G;
+    const long nx = g->nxtot;
+    const long nrow = f ? fit_nact(f) : nx;  // output rows per column
+    hipStream_t st = g->stream;
+    // device gradient index of each requested column; the columns of a
+    // WFS the solve does not use are zero
+    std::vector<int> kof(g->goff_all.size() - 1, -1);
+    for (int k = 0; k < G.nwfs; k++) kof[g->wfs_active[k]] = k;
+    std::vector<int> idx;
+    std::vector<long> col;
+    for (long j = col0; j < col0 + ncol; j++) {
+        const int w = (int)(std::upper_bound(g->goff_all.begin(), g->goff_all.end(), j) - g->goff_all.begin()) - 1;
+        const int k = kof[w];
+        if (k < 0) {
+            std::fill(out + (j - col0) * nrow, out + (j - col0 + 1) * nrow, 0.0);
+        } else {
+            idx.push_back((int)(G.goff[k] + (j - g->goff_all[w])));
+            col.push_back(j - col0);
         }
-        if (idx.empty()) return;
-        DevBuf<int> didx;
-        didx.upload(idx);
-        // the chunks overwrite the slots' gradients and solutions: keep them
-        // (the warm-start state of a closed loop) and put them back at the end
-        DevBuf<float> keep_x, keep_g;
-        keep_x.alloc(g->x.n);
-        keep_g.alloc(g->g.n);
-        HIPCHK(hipMemcpyAsync(keep_x.p, g->x.p, sizeof(float) * g->x.n, hipMemcpyDeviceToDevice, st));
-        HIPCHK(hipMemcpyAsync(keep_g.p, g->g.p, sizeof(float) * g->g.n, hipMemcpyDeviceToDevice, st));
-        // chunks of nbatch columns; two staging buffers, so that the host
-        // widens chunk t-1 to double while the GPU solves chunk t
-        const int B = g->nbatch;
-        const long nch = ((long)idx.size() + B - 1) / B;
-        float *h[2] = {nullptr, nullptr};
-        hipEvent_t ev[2] = {nullptr, nullptr};
-        auto widen = [&](long t) {
-            HIPCHK(hipEventSynchronize(ev[t % 2]));
+    }
+    if (idx.empty()) return;
+    DevBuf<int> didx;
+    didx.upload(idx);
+    // the chunks overwrite the slots' gradients and solutions: keep them
+    // (the warm-start state of a closed loop) and put them back at the end
+    DevBuf<float> keep_x, keep_g;
+    keep_x.alloc(g->x.n);
+    keep_g.alloc(g->g.n);
+    HIPCHK(hipMemcpyAsync(keep_x.p, g->x.p, sizeof(float) * g->x.n, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(keep_g.p, g->g.p, sizeof(float) * g->g.n, hipMemcpyDeviceToDevice, st));
+    const int B = g->nbatch;
+    DevBuf<double> dm;  // fit output of one chunk
+    if (f) dm.alloc((size_t)B * nrow);
+    const long nch = ((long)idx.size() + B - 1) / B;
+    void *h[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    const size_t esz = f ? sizeof(double) : sizeof(float);
+    auto widen = [&](long t) {
+        HIPCHK(hipEventSynchronize(ev[t % 2]));
+        const long c = t * B;
+        const int n = (int)std::min<long>(B, (long)idx.size() - c);
+        // columns over up to 8 host threads (the output's first touch and
+        // the widening are host-bound otherwise)
+        auto cols = [&](int k0, int k1) {
+            for (int k = k0; k < k1; k++) {
+                double *o = out + col[c + k] * nrow;
+                if (f) {
+                    std::memcpy(o, (const double *)h[t % 2] + (long)k * nrow, sizeof(double) * nrow);
+                } else {
+                    const float *hk = (const float *)h[t % 2] + (long)k * nrow;
+                    for (long i = 0; i < nrow; i++) o[i] = hk[i];
+                }
+            }
+        };
+        const int nt = std::min(n, 8);
+        std::vector<std::thread> th;
+        for (int q = 1; q < nt; q++) th.emplace_back(cols, q * n / nt, (q + 1) * n / nt);
+        cols(0, n / nt);
+        for (auto &x : th) x.join();
+    };
+    try {
+        for (int b = 0; b < 2; b++) {
+            HIPCHK(hipHostMalloc(&h[b], esz * nrow * B));
+            HIPCHK(hipEventCreateWithFlags(&ev[b], hipEventDisableTiming));
+        }
+        for (long t = 0; t < nch; t++) {
             const long c = t * B;
             const int n = (int)std::min<long>(B, (long)idx.size() - c);
-            // columns over up to 8 host threads (the output's first touch
-            // and the widening are host-bound otherwise)
-            auto cols = [&](int k0, int k1) {
-                for (int k = k0; k < k1; k++) {
-                    double *o = out + col[c + k] * nx;
-                    const float *hk = h[t % 2] + (long)k * nx;
-                    for (long i = 0; i < nx; i++) o[i] = hk[i];
-                }
-            };
-            const int nt = std::min(n, 8);
-            std::vector<std::thread> th;
-            for (int q = 1; q < nt; q++) th.emplace_back(cols, q * n / nt, (q + 1) * n / nt);
-            cols(0, n / nt);
-            for (auto &x : th) x.join();
-        };
-        try {
-            for (int b = 0; b < 2; b++) {
-                HIPCHK(hipHostMalloc((void **)&h[b], sizeof(float) * nx * B));
-                HIPCHK(hipEventCreateWithFlags(&ev[b], hipEventDisableTiming));
+            launch_unit(g->g.p, G.ngtot, didx.p + c, n, st);
+            if (g->warm) launch_unit(g->x.p, G.nxtot, nullptr, n, st);  // cold start
+            if (!g->prof) {
+                graph_solve(g, n, st);
+            } else {  // profiled plans launch kernel by kernel, as gpu_tomo_solve_dev does
+                Rec rec{&g->evs, st, true, g->prof};
+                enqueue_solve(own_batch(g, 0, n), rec);
+                prof_collect(g, t > 0);  // the averages cover every chunk
             }
-            for (long t = 0; t < nch; t++) {
-                const long c = t * B;
-                const int n = (int)std::min<long>(B, (long)idx.size() - c);
-                launch_unit(g->g.p, G.ngtot, didx.p + c, n, st);
-                if (g->warm) launch_unit(g->x.p, G.nxtot, nullptr, n, st);  // cold start
-                if (!g->prof) {
-                    graph_solve(g, n, st);
-                } else {  // profiled plans launch kernel by kernel, as gpu_tomo_solve_dev does
-                    Rec rec{&g->evs, st, true, g->prof};
-                    enqueue_solve(own_batch(g, 0, n), rec);
-                    prof_collect(g, t > 0);  // the averages cover every chunk
-                }
+            if (f) {
+                const int rc = fit_enqueue_cold(f, g->x.p, G.nxtot, dm.p, n, st);
+                if (rc != MT_OK) fail(rc, mt_last_error());
+                HIPCHK(hipMemcpyAsync(h[t % 2], dm.p, sizeof(double) * nrow * n, hipMemcpyDeviceToHost, st));
+            } else {
                 HIPCHK(hipMemcpy2DAsync(h[t % 2], sizeof(float) * nx, g->x.p, sizeof(float) * G.nxtot,
                                         sizeof(float) * nx, n, hipMemcpyDeviceToHost, st));
-                HIPCHK(hipEventRecord(ev[t % 2], st));
-                if (t > 0) widen(t - 1);
             }
-            widen(nch - 1);
-        } catch (...) {
-            (void)hipStreamSynchronize(st);
-            (void)hipMemcpy(g->x.p, keep_x.p, sizeof(float) * g->x.n, hipMemcpyDeviceToDevice);
-            (void)hipMemcpy(g->g.p, keep_g.p, sizeof(float) * g->g.n, hipMemcpyDeviceToDevice);
-            for (int b = 0; b < 2; b++) {
-                if (h[b]) (void)hipHostFree(h[b]);
-                if (ev[b]) (void)hipEventDestroy(ev[b]);
-            }
-            throw;
+            HIPCHK(hipEventRecord(ev[t % 2], st));
+            if (t > 0) widen(t - 1);
         }
-        HIPCHK(hipMemcpyAsync(g->x.p, keep_x.p, sizeof(float) * g->x.n, hipMemcpyDeviceToDevice, st));
-        HIPCHK(hipMemcpyAsync(g->g.p, keep_g.p, sizeof(float) * g->g.n, hipMemcpyDeviceToDevice, st));
-        HIPCHK(hipStreamSynchronize(st));
+        widen(nch - 1);
+    } catch (...) {
+        (void)hipStreamSynchronize(st);
+        (void)hipMemcpy(g->x.p, keep_x.p, sizeof(float) * g->x.n, hipMemcpyDeviceToDevice);
+        (void)hipMemcpy(g->g.p, keep_g.p, sizeof(float) * g->g.n, hipMemcpyDeviceToDevice);
         for (int b = 0; b < 2; b++) {
-            (void)hipHostFree(h[b]);
-            (void)hipEventDestroy(ev[b]);
+            if (h[b]) (void)hipHostFree(h[b]);
+            if (ev[b]) (void)hipEventDestroy(ev[b]);
         }
+        throw;
+    }
+    HIPCHK(hipMemcpyAsync(g->x.p, keep_x.p, sizeof(float) * g->x.n, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipMemcpyAsync(g->g.p, keep_g.p, sizeof(float) * g->g.n, hipMemcpyDeviceToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int b = 0; b < 2; b++) {
+        (void)hipHostFree(h[b]);
+        (void)hipEventDestroy(ev[b]);
+    }
+}
+
+extern "C" {
+
+int gpu_tomo_mvm_tomo(gpu_tomo_t *g, long col0, long ncol, double *out) {
+    if (!g || !out || col0 < 0 || ncol <= 0 || col0 + ncol > g->ngtot_all) return MT_EINVAL;
+    return guarded([&] { mvm_columns(g, nullptr, col0, ncol, out); });
+}
+
+int gpu_mvm_assemble(gpu_tomo_t *g, gpu_fit_t *f, long col0, long ncol, double *out) {
+    if (!g || !f || !out || col0 < 0 || ncol <= 0 || col0 + ncol > g->ngtot_all) return MT_EINVAL;
+    return guarded([&] {
+        if (fit_nxtot(f) != g->G.nxtot) fail(MT_EINVAL, "gpu_mvm_assemble: the fit plan's layers are not the tomography's");
+        if (fit_device(f) != g->device) fail(MT_EINVAL, "gpu_mvm_assemble: plans on different devices");
+        mvm_columns(g, f, col0, ncol, out);
     });
 }
 

@@ -32,42 +32,6 @@ const char *last_error_cstr() { return g_last_error.c_str(); }
 
 using namespace mt;
 
-struct mt_setup {
-    // parameters
-    double D = 0, Din = 0;
-    int npsr = 0;
-    std::vector<double> ht, wt0;
-    std::vector<int> os;
-    double atmr_dx = 0, atmr_hs = INFINITY, r0 = 0, L0 = 0, za = 0;
-    int pos = 1, guard = 1, precond = 1, ptt_lhs = 0, split = 1;
-    double cxxscale = 1, tomoscale = 1e-12;
-    std::vector<mt_powfs_t> powfs;
-    std::vector<mt_wfs_t> wfs;
-    std::vector<Dir> dirs;
-    // geometry
-    std::vector<double> wt;  // clipped + normalised layer weights
-    Grid pg;
-    std::vector<Grid> xg;
-    std::vector<SaGeom> sa;        // per powfs
-    std::vector<SparseRows> gprow; // per powfs (rows = gradients)
-    // per-powfs CSC export of GP (2nsa x nploc)
-    std::vector<std::vector<long>> gp_p, gp_i;
-    std::vector<std::vector<double>> gp_x;
-    // per WFS
-    std::vector<int> skip, ttr;
-    std::vector<long> nsa;
-    std::vector<double> dsa, thetax, thetay, hs, neam;
-    std::vector<std::vector<double>> saloc, neai, ptt;
-    // regularisation
-    std::vector<double> l2c, l2cf;
-    // FDPCG
-    FdpcgOut fd;
-    // export views
-    std::vector<long> xnx, xny;
-    std::vector<double> xox, xoy, xdx;
-    std::vector<const double *> v_saloc, v_saa, v_gpx, v_neai, v_ptt;
-    std::vector<const long *> v_gpp, v_gpi;
-};
 
 namespace {
 

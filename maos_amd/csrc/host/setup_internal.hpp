@@ -116,3 +116,42 @@ void dft2(std::vector<cplx> &a, long nx, long ny, int sign);
 void fftshift2(std::vector<cplx> &a, long nx, long ny);
 
 }  // namespace mt
+
+// The host setup of one tomography geometry (the RECON_T fields of this path).
+struct mt_setup {
+    // parameters
+    double D = 0, Din = 0;
+    int npsr = 0;
+    std::vector<double> ht, wt0;
+    std::vector<int> os;
+    double atmr_dx = 0, atmr_hs = INFINITY, r0 = 0, L0 = 0, za = 0;
+    int pos = 1, guard = 1, precond = 1, ptt_lhs = 0, split = 1;
+    double cxxscale = 1, tomoscale = 1e-12;
+    std::vector<mt_powfs_t> powfs;
+    std::vector<mt_wfs_t> wfs;
+    std::vector<mt::Dir> dirs;
+    // geometry
+    std::vector<double> wt;  // clipped + normalised layer weights
+    mt::Grid pg;
+    std::vector<mt::Grid> xg;
+    std::vector<mt::SaGeom> sa;        // per powfs
+    std::vector<mt::SparseRows> gprow; // per powfs (rows = gradients)
+    // per-powfs CSC export of GP (2nsa x nploc)
+    std::vector<std::vector<long>> gp_p, gp_i;
+    std::vector<std::vector<double>> gp_x;
+    // per WFS
+    std::vector<int> skip, ttr;
+    std::vector<long> nsa;
+    std::vector<double> dsa, thetax, thetay, hs, neam;
+    std::vector<std::vector<double>> saloc, neai, ptt;
+    // regularisation
+    std::vector<double> l2c, l2cf;
+    // FDPCG
+    mt::FdpcgOut fd;
+    // export views
+    std::vector<long> xnx, xny;
+    std::vector<double> xox, xoy, xdx;
+    std::vector<const double *> v_saloc, v_saa, v_gpx, v_neai, v_ptt;
+    std::vector<const long *> v_gpp, v_gpi;
+};
+

@@ -17,7 +17,7 @@ import ctypes as C
 import numpy as np
 
 from . import _abi
-from .configs import Parms
+from .configs import FitParms, Parms
 
 
 def _arr(ptr, n, dtype=np.float64):
@@ -266,6 +266,16 @@ class GpuTomo:
         _abi.check(_abi.lib().gpu_tomo_mvm_tomo(self._h, int(col0), int(ncol), _dp(out)), "gpu_tomo_mvm_tomo")
         return out.T
 
+    def mvm_assemble(self, fit, col0=0, ncol=None):
+        """gpu_mvm_assemble: columns [col0, col0 + ncol) of the control
+        matrix F T (fit of the tomography of each unit gradient vector);
+        (nact, ncol) array.  fit: a GpuFit on the same layers."""
+        ng = self.recon.ng
+        ncol = ng - col0 if ncol is None else ncol
+        out = np.zeros((ncol, fit.fit.nact))
+        _abi.check(_abi.lib().gpu_mvm_assemble(self._h, fit._h, int(col0), int(ncol), _dp(out)), "gpu_mvm_assemble")
+        return out.T
+
     def solve_dev(self, d_grad, d_x, nbatch=None, mode=0, stream=None):
         """Batched solve on device buffers (int device pointers, float32)."""
         nb = self.nbatch if nbatch is None else nbatch
@@ -412,3 +422,127 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class FitSetup:
+    """DM fitting geometry and operators (mt_fit_create) for a Recon."""
+
+    def __init__(self, recon, fc):
+        self.recon, self.fc = recon, fc
+        self._parms = FitParms(fc)
+        L = _abi.lib()
+        h = C.c_void_p()
+        _abi.check(L.mt_fit_create(recon._h, C.byref(self._parms.p), C.byref(h)), "mt_fit_create")
+        self._h = h
+        self.rec = _abi.MtFitRecon()
+        _abi.check(L.mt_fit_recon(h, C.byref(self.rec)), "mt_fit_recon")
+        na, nf = C.c_long(), C.c_long()
+        L.mt_fit_sizes(h, C.byref(na), C.byref(nf))
+        self.nact, self.nfloc = na.value, nf.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _abi.lib().mt_fit_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def W0(self):
+        """W0 as a scipy.sparse.csr_matrix (nfloc x nfloc) from the 9-point rows."""
+        import scipy.sparse as sp
+        r = self.rec
+        n, nx, ny = self.nfloc, r.fnx, r.fny
+        w = _arr(r.W0, 9 * n).reshape(n, 9)
+        rows, cols, vals = [], [], []
+        for k in range(9):
+            dx, dy = k % 3 - 1, k // 3 - 1
+            idx = np.arange(n)
+            ix, iy = idx % nx, idx // nx
+            jx, jy = ix + dx, iy + dy
+            ok = (w[:, k] != 0) & (jx >= 0) & (jx < nx) & (jy >= 0) & (jy < ny)
+            rows.append(idx[ok])
+            cols.append((jx + jy * nx)[ok])
+            vals.append(w[ok, k])
+        return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+
+    def W1(self):
+        return _arr(self.rec.W1, self.nfloc)
+
+    def NW(self):
+        return _arr(self.rec.NW, self.nact * self.rec.nnw).reshape(self.rec.nnw, self.nact).T
+
+    def slaving(self):
+        """S as scipy.sparse.csr_matrix, or None."""
+        import scipy.sparse as sp
+        r = self.rec
+        if not r.sl_p:
+            return None
+        p = _arr(r.sl_p, self.nact + 1, np.int64)
+        return sp.csr_matrix((_arr(r.sl_x, p[-1]), _arr(r.sl_i, p[-1], np.int64), p), shape=(self.nact, self.nact))
+
+    def actcpl(self):
+        return _arr(self.rec.actcpl, self.nact)
+
+    def grids(self):
+        r = self.rec
+        return [dict(nx=r.anx[d], ny=r.any[d], ox=r.aox[d], oy=r.aoy[d], dx=r.adx[d], ht=r.aht[d])
+                for d in range(r.ndm)]
+
+
+class GpuFit:
+    """GPU DM fitting plan for nbatch systems (gpu_fit_*)."""
+
+    def __init__(self, recon, fit, device=0, nbatch=1):
+        self.recon, self.fit = recon, fit
+        h = C.c_void_p()
+        _abi.check(_abi.lib().gpu_fit_setup(C.byref(recon.rec), C.byref(fit.rec), device, nbatch, C.byref(h)),
+                   "gpu_fit_setup")
+        self._h = h
+        self.nbatch = nbatch
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _abi.lib().gpu_fit_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def solve(self, opdr, isys=0):
+        """curecon_t::fit: (dmfit, cgres, status) from tomography layers."""
+        x = np.ascontiguousarray(opdr, dtype=np.float64)
+        out = np.zeros(self.fit.nact)
+        res = C.c_double()
+        rc = _abi.check(_abi.lib().gpu_fit_solve(self._h, int(isys), _dp(x), _dp(out), C.byref(res)),
+                        "gpu_fit_solve")
+        return out, res.value, rc
+
+    def solve_dev(self, d_opdr, d_dm, nsys=None, stream=None):
+        n = self.nbatch if nsys is None else nsys
+        _abi.check(_abi.lib().gpu_fit_solve_dev(self._h, C.c_void_p(d_opdr), C.c_void_p(d_dm), int(n),
+                                                C.c_void_p(stream or 0)), "gpu_fit_solve_dev")
+
+    def R(self, opdr, alpha=1.0):
+        x = np.ascontiguousarray(opdr, dtype=np.float64)
+        out = np.zeros(self.fit.nact)
+        _abi.check(_abi.lib().gpu_fit_R(self._h, _dp(x), _dp(out), float(alpha)), "gpu_fit_R")
+        return out
+
+    def L(self, a, alpha=1.0, beta=0.0, out=None):
+        x = np.ascontiguousarray(a, dtype=np.float64)
+        o = np.zeros(self.fit.nact) if out is None else np.array(out, dtype=np.float64)
+        _abi.check(_abi.lib().gpu_fit_L(self._h, _dp(x), _dp(o), float(alpha), float(beta)), "gpu_fit_L")
+        return o
+
+    def residuals(self, nsys=None):
+        n = self.nbatch if nsys is None else nsys
+        out = np.zeros(n)
+        _abi.check(_abi.lib().gpu_fit_residuals(self._h, _dp(out), n), "gpu_fit_residuals")
+        return out

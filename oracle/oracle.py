@@ -45,6 +45,19 @@ def lib():
         L.or_l2.restype = None
         L.or_fft1d.argtypes = [dp, dp, C.c_int, C.c_int]
         L.or_fft1d.restype = None
+        L.orf_create.argtypes = [vp, vp, C.c_int]
+        L.orf_create.restype = vp
+        L.orf_free.argtypes = [vp]
+        L.orf_free.restype = None
+        L.orf_nact.argtypes = [vp]
+        L.orf_nact.restype = C.c_long
+        for f in ("orf_fit_r", "orf_fit_l"):
+            getattr(L, f).argtypes = [vp, dp, dp, C.c_double]
+            getattr(L, f).restype = None
+        L.orf_pcg.argtypes = [vp, dp, dp, C.c_int, C.c_int]
+        L.orf_pcg.restype = C.c_double
+        L.orf_solve.argtypes = [vp, dp, dp, C.c_int, C.c_int]
+        L.orf_solve.restype = C.c_double
         _lib = L
     return _lib
 
@@ -123,6 +136,45 @@ class Oracle:
         oi = np.ascontiguousarray(opd, dtype=np.float64)
         lib().or_hxt(self._h, int(w), _dp(oi), _dp(out))
         return out
+
+
+class FitOracle:
+    """FP64 CPU DM fitting (oracle/fit_oracle.c) on a Recon and its FitSetup."""
+
+    def __init__(self, recon, fit, nthread=1):
+        self.recon, self.fit = recon, fit
+        self._h = lib().orf_create(C.cast(C.pointer(recon.rec), C.c_void_p), C.cast(C.pointer(fit.rec), C.c_void_p),
+                                   int(nthread))
+        self.nact = lib().orf_nact(self._h)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().orf_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def fit_r(self, x, alpha=1.0):
+        xi = np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(self.nact)
+        lib().orf_fit_r(self._h, _dp(xi), _dp(out), alpha)
+        return out
+
+    def fit_l(self, a, alpha=1.0):
+        ai = np.ascontiguousarray(a, dtype=np.float64)
+        out = np.zeros(self.nact)
+        lib().orf_fit_l(self._h, _dp(ai), _dp(out), alpha)
+        return out
+
+    def solve(self, x, maxit, a0=None):
+        xi = np.ascontiguousarray(x, dtype=np.float64)
+        a = np.zeros(self.nact) if a0 is None else np.array(a0, dtype=np.float64)
+        res = lib().orf_solve(self._h, _dp(xi), _dp(a), 0 if a0 is None else 1, int(maxit))
+        return a, res
 
 
 def fft1d(x, sign=-1):

@@ -10,6 +10,7 @@ import pytest
 
 from maos_amd import configs, tomo, workload
 from oracle.oracle import Oracle
+from parity_util import fp32_spread
 
 pytestmark = pytest.mark.gpu
 
@@ -67,46 +68,6 @@ def test_P(sys_):
     for l in range(r.npsr):
         x[off[l]:off[l + 1]] -= x[off[l]:off[l + 1]].mean()
     assert rel(g.P(x), o.fdpcg(x)) < 2e-5
-
-
-def fp32_spread(o, grad, maxit, draws=2, eps=3e-7):
-    """How far the oracle's own PCG (the steps of or_pcg, lib/pcg.c:28-113)
-    moves when each preconditioner output is rounded to FP32 with a relative
-    error of eps added, the size of the rounding of an FP32 inverse FFT and
-    block multiply.  The largest distance over `draws` draws.
-
-    On the small LGS batch members this spread grows quickly with the
-    iteration count, although the system is far from converged: b8_w2_l4
-    gives 2e-7 at 20 iterations, 2e-6 at 25, 4e-5 to 3e-4 at 30 and 3e-3
-    at 40.  The same PCG with eps = 0 reproduces the oracle to 3e-9."""
-    b = o.tomo_r(grad)
-    xo, _ = o.solve(grad, maxit)
-    worst = 0.0
-    for d in range(draws):
-        rng = np.random.default_rng(100 + d)
-
-        def z_of(v):
-            z = o.fdpcg(v)
-            return (z * (1 + eps * rng.standard_normal(z.shape))).astype(np.float32).astype(np.float64)
-
-        x = np.zeros_like(b)
-        rr = b.copy()
-        z = z_of(rr)
-        p = z.copy()
-        rz = rr @ z
-        for k in range(maxit):
-            ap = o.tomo_l(p)
-            a = rz / (p @ ap)
-            x += a * p
-            rr -= a * ap
-            if k + 1 == maxit:
-                break
-            z = z_of(rr)
-            rz2 = rr @ z
-            p = z + (rz2 / rz) * p
-            rz = rz2
-        worst = max(worst, rel(x, xo))
-    return worst
 
 
 def test_solve(sys_):
