@@ -159,6 +159,12 @@ int  mt_setup_update_cn2(mt_setup_t *s, const double *wt, double r0z);
 /* Fill a POD view; pointers stay valid until the next update or free. */
 int  mt_setup_recon(const mt_setup_t *s, mt_recon_t *out);
 void mt_setup_free(mt_setup_t *s);
+/* Replace the GP operators with MAOS-saved ones (load.GP,
+ * maos/setup_recon_prep.c:485-505): a cell (mt_bin_read of MAOS's GP.bin)
+ * of one sparse 2 nsa x nploc matrix per WFS or per powfs; WFS of one
+ * powfs must carry the same matrix.  Call before gpu_tomo_setup. */
+typedef struct mt_bin mt_bin_t;
+int  mt_setup_load_gp(mt_setup_t *s, const mt_bin_t *gp);
 /* Pre-inversion FDPCG blocks (Mhat, same layout as fd_Minv), for checking. */
 int  mt_setup_fdpcg_mhat(const mt_setup_t *s, const double **mhat);
 /* Recon-grid sizes: nx (sum of layer sizes), ng (sum of gradient sizes). */
@@ -609,20 +615,47 @@ int  mt_conf_unused(const mt_conf_t *c, char *buf, long len, long *n);
  * (gradlastcl, the input of gpu_tomo) and solutions (opdr) pass through
  * unchanged.  A node is a double matrix (nx rows, ny columns, column-major)
  * or a cell of nodes (an empty element is NULL); each may carry the header
- * text MAOS stores before it (key=value lines).  Other data types are
- * refused (MT_EINVAL).  No reference counterpart as a C API: MAOS does this
+ * text MAOS stores before it (key=value lines); the other element types
+ * MAOS writes (complex, single precision, integer and sparse matrices) are
+ * read and written too.  No reference counterpart as a C API: MAOS does this
  * I/O itself with readbin / writebin (math/cell.c). */
-typedef struct mt_bin mt_bin_t;
 
 int  mt_bin_read(const char *path, mt_bin_t **out);       /* .bin or .bin.gz */
 int  mt_bin_write(const char *path, const mt_bin_t *b);   /* gzip if the path ends in .gz */
 void mt_bin_free(mt_bin_t *b);
 int  mt_bin_info(const mt_bin_t *b, int *is_cell, long *nx, long *ny);
-const double   *mt_bin_data(const mt_bin_t *b);            /* NULL for a cell */
+/* Element types (sys/bin.h:34-51): double, cell, complex double, single
+ * float, single complex, integer (64- or 32-bit on file), sparse double,
+ * sparse complex (64- or 32-bit indices on file; CSC). */
+#define MT_BIN_DOUBLE   0
+#define MT_BIN_CELL     1
+#define MT_BIN_COMPLEX  2
+#define MT_BIN_FLOAT    3
+#define MT_BIN_FCOMPLEX 4
+#define MT_BIN_INT      5
+#define MT_BIN_SPARSE   6
+#define MT_BIN_CSPARSE  7
+int  mt_bin_type(const mt_bin_t *b);
+/* Dense values as doubles: nx*ny (real) or 2*nx*ny (complex: re, im pairs;
+ * single precision widened); NULL for cells, integers and sparse. */
+const double   *mt_bin_data(const mt_bin_t *b);
+const long     *mt_bin_idata(const mt_bin_t *b);           /* integer arrays */
+/* A sparse matrix in CSC form: p (ny+1 column pointers), i (row indices),
+ * x (values; complex: re, im pairs), nzmax = p[ny]. */
+int  mt_bin_sparse(const mt_bin_t *b, const long **p, const long **i,
+                   const double **x, long *nzmax);
 const mt_bin_t *mt_bin_child(const mt_bin_t *b, long i);   /* NULL: empty or not a cell */
 const char     *mt_bin_header(const mt_bin_t *b);
 int  mt_bin_new_dbl(long nx, long ny, const double *data, const char *header, mt_bin_t **out);
 int  mt_bin_new_cell(long nx, long ny, const char *header, mt_bin_t **out);
+/* A dense node of another type (MT_BIN_COMPLEX / FLOAT / FCOMPLEX / INT;
+ * data as mt_bin_data returns it; integers given as doubles). */
+int  mt_bin_new_typed(int type, long nx, long ny, const double *data,
+                      const char *header, mt_bin_t **out);
+/* A sparse matrix (CSC; cplx: x holds re, im pairs). */
+int  mt_bin_new_sparse(long nx, long ny, const long *p, const long *i,
+                       const double *x, int cplx, const char *header,
+                       mt_bin_t **out);
 /* Put child at element i of cell (column-major); the cell takes ownership
  * (and frees any previous element). */
 int  mt_bin_set(mt_bin_t *cell, long i, mt_bin_t *child);

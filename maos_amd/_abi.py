@@ -18,6 +18,9 @@ MT_ESTATE = -5
 MT_EFULL = -6
 MT_NOTCONV = 1
 
+MT_BIN_DOUBLE, MT_BIN_CELL, MT_BIN_COMPLEX, MT_BIN_FLOAT, MT_BIN_FCOMPLEX, MT_BIN_INT, MT_BIN_SPARSE, \
+    MT_BIN_CSPARSE = range(8)
+
 dp = C.POINTER(C.c_double)
 lp = C.POINTER(C.c_long)
 ip = C.POINTER(C.c_int)
@@ -179,6 +182,13 @@ def lib():
     L.mt_conf_strarr.argtypes = [vp, C.c_char_p, C.c_long, C.c_char_p, C.c_long, lp]
     L.mt_conf_dbl.argtypes = [vp, C.c_char_p, dp, C.c_long, lp, lp, lp]
     L.mt_conf_unused.argtypes = [vp, C.c_char_p, C.c_long, lp]
+    L.mt_setup_load_gp.argtypes = [vp, vp]
+    L.mt_bin_type.argtypes = [vp]
+    L.mt_bin_idata.argtypes = [vp]
+    L.mt_bin_idata.restype = lp
+    L.mt_bin_sparse.argtypes = [vp, C.POINTER(lp), C.POINTER(lp), C.POINTER(dp), lp]
+    L.mt_bin_new_typed.argtypes = [C.c_int, C.c_long, C.c_long, dp, C.c_char_p, C.POINTER(vp)]
+    L.mt_bin_new_sparse.argtypes = [C.c_long, C.c_long, lp, lp, dp, C.c_int, C.c_char_p, C.POINTER(vp)]
     L.mt_bin_read.argtypes = [C.c_char_p, C.POINTER(vp)]
     L.mt_bin_write.argtypes = [C.c_char_p, vp]
     L.mt_bin_free.argtypes = [vp]
@@ -217,6 +227,7 @@ EXPORTS = [
     "mt_conf_strarr", "mt_conf_dbl", "mt_conf_unused",
     "mt_bin_read", "mt_bin_write", "mt_bin_free", "mt_bin_info", "mt_bin_data", "mt_bin_child",
     "mt_bin_header", "mt_bin_new_dbl", "mt_bin_new_cell", "mt_bin_set", "mt_bin_grad", "mt_bin_opdr",
+    "mt_setup_load_gp", "mt_bin_type", "mt_bin_idata", "mt_bin_sparse", "mt_bin_new_typed", "mt_bin_new_sparse",
 ]
 
 

@@ -13,6 +13,14 @@
 //   for MCC_ANY the nx*ny elements, each with its own header; an empty
 //   element, and a cell whose elements are all empty, is written as
 //   MCC_ANY 0 0.  Little-endian, as MAOS writes on x86.
+// Other element types (magics of sys/bin.h:34-51): M_CMP complex double
+// (re, im pairs), M_FLT / M_ZMP single-precision real / complex, M_INT64 /
+// M_INT32 integers, and sparse matrices (math/spbin.c:32-89): after the
+// header, nzmax(u64), then the ny+1 column pointers and the nzmax row
+// indices (64-bit for M_DSP64 / M_CSP64, 32-bit for M_DSP32 / M_CSP32), then
+// the nzmax values (double, or complex double for M_CSP*); an empty sparse
+// matrix (nx or ny zero) has no nzmax.  Sparse matrices are written with
+// 64-bit indices, the form MAOS writes on a 64-bit host.
 // Files may be gzip-compressed (sys/bin.c:250-256 detects the gzip magic);
 // zlib's gzread reads both forms, and a path ending in ".gz" is written
 // compressed.
@@ -31,18 +39,23 @@
 using namespace mt;
 
 namespace {
-constexpr uint32_t M_DBL = 0x6402;
+constexpr uint32_t M_CSP64 = 0x6400, M_DSP64 = 0x6401, M_DBL = 0x6402, M_INT64 = 0x6403, M_CMP = 0x6404,
+                   M_INT32 = 0x6405, M_CSP32 = 0x6406, M_DSP32 = 0x6407, M_FLT = 0x6408, M_ZMP = 0x6409;
 constexpr uint32_t MCC_ANY = 0x6421;
 constexpr uint32_t M_HEADER = 0x6500;
 constexpr uint32_t M_SKIP = 0x6600;
 }  // namespace
 
 struct mt_bin {
-    uint32_t magic = M_DBL;  // M_DBL or MCC_ANY
+    uint32_t magic = M_DBL;  // one of the M_* above (sparse: M_DSP64 / M_CSP64) or MCC_ANY
     uint64_t nx = 0, ny = 0;
     std::string header;
-    std::vector<double> data;                    // M_DBL
+    std::vector<double> data;                    // values (complex: re, im pairs; float: widened)
+    std::vector<long> idata;                     // integer arrays
+    std::vector<long> sp_p, sp_i;                // sparse: column pointers, row indices
     std::vector<std::unique_ptr<mt_bin>> child;  // MCC_ANY (null: empty element)
+    bool is_sparse() const { return magic == M_DSP64 || magic == M_CSP64; }
+    bool is_complex() const { return magic == M_CMP || magic == M_ZMP || magic == M_CSP64; }
 };
 
 namespace {
@@ -113,6 +126,59 @@ std::unique_ptr<mt_bin> read_node(GzIn &in, bool eof_ok) {
         b->magic = M_DBL;
         b->data.resize(n);
         in.get(b->data.data(), n * sizeof(double));
+    } else if (magic == M_CMP) {
+        b->magic = M_CMP;
+        b->data.resize(2 * n);
+        in.get(b->data.data(), 2 * n * sizeof(double));
+    } else if (magic == M_FLT || magic == M_ZMP) {
+        b->magic = magic;
+        const uint64_t m = magic == M_ZMP ? 2 * n : n;
+        std::vector<float> f(m);
+        in.get(f.data(), m * sizeof(float));
+        b->data.assign(f.begin(), f.end());
+    } else if (magic == M_INT64 || magic == M_INT32) {
+        b->magic = magic;
+        b->idata.resize(n);
+        if (magic == M_INT64) {
+            std::vector<int64_t> v(n);
+            in.get(v.data(), n * 8);
+            b->idata.assign(v.begin(), v.end());
+        } else {
+            std::vector<int32_t> v(n);
+            in.get(v.data(), n * 4);
+            b->idata.assign(v.begin(), v.end());
+        }
+    } else if (magic == M_DSP64 || magic == M_DSP32 || magic == M_CSP64 || magic == M_CSP32) {
+        const bool cplx = magic == M_CSP64 || magic == M_CSP32, i64 = magic == M_DSP64 || magic == M_CSP64;
+        b->magic = cplx ? M_CSP64 : M_DSP64;
+        b->sp_p.assign(b->ny + 1, 0);
+        if (n) {
+            uint64_t nz;
+            in.get(&nz, 8);
+            if (nz > (1ull << 34)) throw Error(MT_EINVAL, "implausible sparse size in " + in.path);
+            auto ints = [&](std::vector<long> &dst, uint64_t cnt) {
+                dst.resize(cnt);
+                if (i64) {
+                    std::vector<int64_t> v(cnt);
+                    in.get(v.data(), cnt * 8);
+                    dst.assign(v.begin(), v.end());
+                } else {
+                    std::vector<int32_t> v(cnt);
+                    in.get(v.data(), cnt * 4);
+                    dst.assign(v.begin(), v.end());
+                }
+            };
+            ints(b->sp_p, b->ny + 1);
+            ints(b->sp_i, nz);
+            b->data.resize(cplx ? 2 * nz : nz);
+            in.get(b->data.data(), b->data.size() * sizeof(double));
+            if (b->sp_p[0] != 0 || (uint64_t)b->sp_p[b->ny] > nz)
+                throw Error(MT_EINVAL, "invalid sparse column pointers in " + in.path);
+            for (uint64_t c = 0; c < b->ny; c++)
+                if (b->sp_p[c + 1] < b->sp_p[c]) throw Error(MT_EINVAL, "invalid sparse column pointers in " + in.path);
+            for (long r : b->sp_i)
+                if (r < 0 || (uint64_t)r >= b->nx) throw Error(MT_EINVAL, "sparse row index out of range in " + in.path);
+        }
     } else if (magic == MCC_ANY) {
         b->magic = MCC_ANY;
         b->child.resize(n);
@@ -157,8 +223,32 @@ void write_node(GzOut &out, const mt_bin *b) {
         write_head(out, b->header, MCC_ANY, any ? b->nx : 0, any ? b->ny : 0);
         if (any)
             for (auto &c : b->child) write_node(out, c.get());
+    } else if (b->is_sparse()) {
+        const bool empty = b->nx * b->ny == 0;
+        write_head(out, b->header, b->magic, b->nx, b->ny);
+        if (!empty) {
+            const uint64_t nz = (uint64_t)b->sp_p[b->ny];
+            out.put(&nz, 8);
+            std::vector<int64_t> p(b->sp_p.begin(), b->sp_p.end()), i(b->sp_i.begin(), b->sp_i.begin() + nz);
+            out.put(p.data(), p.size() * 8);
+            out.put(i.data(), i.size() * 8);
+            out.put(b->data.data(), (b->is_complex() ? 2 : 1) * nz * sizeof(double));
+        }
+    } else if (b->magic == M_FLT || b->magic == M_ZMP) {
+        write_head(out, b->header, b->magic, b->nx, b->ny);
+        std::vector<float> f(b->data.begin(), b->data.end());
+        out.put(f.data(), f.size() * sizeof(float));
+    } else if (b->magic == M_INT64 || b->magic == M_INT32) {
+        write_head(out, b->header, b->magic, b->nx, b->ny);
+        if (b->magic == M_INT64) {
+            std::vector<int64_t> v(b->idata.begin(), b->idata.end());
+            out.put(v.data(), v.size() * 8);
+        } else {
+            std::vector<int32_t> v(b->idata.begin(), b->idata.end());
+            out.put(v.data(), v.size() * 4);
+        }
     } else {
-        write_head(out, b->header, M_DBL, b->nx, b->ny);
+        write_head(out, b->header, b->magic, b->nx, b->ny);
         out.put(b->data.data(), b->data.size() * sizeof(double));
     }
 }
@@ -234,7 +324,97 @@ int mt_bin_info(const mt_bin_t *b, int *is_cell, long *nx, long *ny) {
     return MT_OK;
 }
 
-const double *mt_bin_data(const mt_bin_t *b) { return b && b->magic == M_DBL ? b->data.data() : nullptr; }
+const double *mt_bin_data(const mt_bin_t *b) {
+    return b && b->magic != MCC_ANY && !b->is_sparse() && b->magic != M_INT64 && b->magic != M_INT32 ? b->data.data()
+                                                                                                       : nullptr;
+}
+
+int mt_bin_type(const mt_bin_t *b) {
+    if (!b) return MT_EINVAL;
+    switch (b->magic) {
+        case MCC_ANY: return MT_BIN_CELL;
+        case M_CMP: return MT_BIN_COMPLEX;
+        case M_FLT: return MT_BIN_FLOAT;
+        case M_ZMP: return MT_BIN_FCOMPLEX;
+        case M_INT64:
+        case M_INT32: return MT_BIN_INT;
+        case M_DSP64: return MT_BIN_SPARSE;
+        case M_CSP64: return MT_BIN_CSPARSE;
+        default: return MT_BIN_DOUBLE;
+    }
+}
+
+const long *mt_bin_idata(const mt_bin_t *b) {
+    return b && (b->magic == M_INT64 || b->magic == M_INT32) ? b->idata.data() : nullptr;
+}
+
+int mt_bin_sparse(const mt_bin_t *b, const long **p, const long **i, const double **x, long *nzmax) {
+    if (!b || !b->is_sparse()) return MT_EINVAL;
+    if (p) *p = b->sp_p.data();
+    if (i) *i = b->sp_i.data();
+    if (x) *x = b->data.data();
+    if (nzmax) *nzmax = b->sp_p.empty() ? 0 : b->sp_p.back();
+    return MT_OK;
+}
+
+int mt_bin_new_typed(int type, long nx, long ny, const double *data, const char *header, mt_bin_t **out) {
+    if (!out || nx < 0 || ny < 0) return MT_EINVAL;
+    uint32_t magic;
+    long m = nx * ny;
+    switch (type) {
+        case MT_BIN_DOUBLE: magic = M_DBL; break;
+        case MT_BIN_COMPLEX: magic = M_CMP; m *= 2; break;
+        case MT_BIN_FLOAT: magic = M_FLT; break;
+        case MT_BIN_FCOMPLEX: magic = M_ZMP; m *= 2; break;
+        case MT_BIN_INT: magic = M_INT64; break;
+        default: return MT_EINVAL;
+    }
+    if (m > 0 && !data) return MT_EINVAL;
+    return guard([&] {
+        std::unique_ptr<mt_bin> b(new mt_bin());
+        b->magic = magic;
+        b->nx = (uint64_t)nx;
+        b->ny = (uint64_t)ny;
+        b->header = header ? header : "";
+        if (type == MT_BIN_INT) {
+            b->idata.resize(m);
+            for (long k = 0; k < m; k++) {
+                if (data[k] != std::floor(data[k])) throw Error(MT_EINVAL, "integer array holds a non-integer");
+                b->idata[k] = (long)data[k];
+            }
+        } else {
+            b->data.assign(data, data + m);
+            if (type == MT_BIN_FLOAT || type == MT_BIN_FCOMPLEX)
+                for (double &v : b->data) v = (double)(float)v;
+        }
+        *out = b.release();
+    });
+}
+
+int mt_bin_new_sparse(long nx, long ny, const long *p, const long *i, const double *x, int cplx, const char *header,
+                      mt_bin_t **out) {
+    if (!out || nx < 0 || ny < 0 || (nx * ny > 0 && !p)) return MT_EINVAL;
+    return guard([&] {
+        std::unique_ptr<mt_bin> b(new mt_bin());
+        b->magic = cplx ? M_CSP64 : M_DSP64;
+        b->nx = (uint64_t)nx;
+        b->ny = (uint64_t)ny;
+        b->header = header ? header : "";
+        b->sp_p.assign(ny + 1, 0);
+        if (nx * ny > 0) {
+            b->sp_p.assign(p, p + ny + 1);
+            const long nz = p[ny];
+            if (p[0] != 0 || nz < 0 || (nz > 0 && (!i || !x))) throw Error(MT_EINVAL, "invalid sparse matrix");
+            for (long c = 0; c < ny; c++)
+                if (p[c + 1] < p[c]) throw Error(MT_EINVAL, "invalid sparse column pointers");
+            b->sp_i.assign(i, i + nz);
+            for (long r : b->sp_i)
+                if (r < 0 || r >= nx) throw Error(MT_EINVAL, "sparse row index out of range");
+            b->data.assign(x, x + (cplx ? 2 : 1) * nz);
+        }
+        *out = b.release();
+    });
+}
 
 const mt_bin_t *mt_bin_child(const mt_bin_t *b, long i) {
     if (!b || b->magic != MCC_ANY || i < 0 || (uint64_t)i >= b->child.size()) return nullptr;

@@ -433,6 +433,83 @@ int mt_setup_sizes(const mt_setup_t *s, long *nx, long *ng) {
 
 void mt_setup_free(mt_setup_t *s) { delete s; }
 
+int mt_setup_load_gp(mt_setup_t *s, const mt_bin_t *gp) {
+    if (!s || !gp) return MT_EINVAL;
+    return guarded([&] {
+        // setup_recon_GP with load.GP (maos/setup_recon_prep.c:485-505): a
+        // cell of one sparse matrix per WFS or per powfs, each 2 nsa x nploc
+        int cell = 0;
+        long nx = 0, ny = 0;
+        mt_bin_info(gp, &cell, &nx, &ny);
+        const long nel = nx * ny, nwfs = (long)s->wfs.size(), npowfs = (long)s->powfs.size();
+        if (!cell || (nel != nwfs && nel != npowfs)) throw Error(MT_EINVAL, "load.GP: a cell of nwfs or npowfs sparse matrices");
+        const long nploc = s->pg.nx * s->pg.ny;
+        struct Csc {
+            const long *p = nullptr, *i = nullptr;
+            const double *x = nullptr;
+            long nz = 0;
+        };
+        auto get = [&](long k, long nsa) {
+            const mt_bin_t *e = mt_bin_child(gp, k);
+            if (!e || mt_bin_type(e) != MT_BIN_SPARSE) throw Error(MT_EINVAL, "load.GP: element " + std::to_string(k) + " is not a sparse double matrix");
+            long ex = 0, ey = 0;
+            mt_bin_info(e, nullptr, &ex, &ey);
+            if (ex != 2 * nsa || ey != nploc)
+                throw Error(MT_EINVAL, "load.GP: element " + std::to_string(k) + " is " + std::to_string(ex) + " x " +
+                                           std::to_string(ey) + ", need " + std::to_string(2 * nsa) + " x " +
+                                           std::to_string(nploc));
+            Csc c;
+            mt_bin_sparse(e, &c.p, &c.i, &c.x, &c.nz);
+            return c;
+        };
+        for (size_t ip = 0; ip < s->powfs.size(); ip++) {
+            if (s->powfs[ip].skip) continue;
+            const long nsa = (long)s->sa[ip].locx.size();
+            Csc c;
+            if (nel == npowfs) {
+                c = get((long)ip, nsa);
+            } else {
+                // per WFS: the plan keeps one GP per powfs, so the WFS of a
+                // powfs must carry the same matrix
+                bool first = true;
+                for (long w = 0; w < nwfs; w++) {
+                    if (s->wfs[w].powfs != (int)ip) continue;
+                    const Csc cw = get(w, nsa);
+                    if (first) {
+                        c = cw;
+                        first = false;
+                    } else if (cw.nz != c.nz || !std::equal(cw.p, cw.p + nploc + 1, c.p) ||
+                               !std::equal(cw.i, cw.i + cw.nz, c.i) || !std::equal(cw.x, cw.x + cw.nz, c.x)) {
+                        throw Error(MT_EINVAL, "load.GP: WFS of one powfs with different GP are not supported");
+                    }
+                }
+                if (first) continue;
+            }
+            s->gp_p[ip].assign(c.p, c.p + nploc + 1);
+            s->gp_i[ip].assign(c.i, c.i + c.nz);
+            s->gp_x[ip].assign(c.x, c.x + c.nz);
+            // row form (gradient rows) for the rest of the setup
+            SparseRows r;
+            r.nrow = 2 * nsa;
+            r.ncol = nploc;
+            r.p.assign(r.nrow + 1, 0);
+            for (long k = 0; k < c.nz; k++) r.p[c.i[k] + 1]++;
+            for (long q = 0; q < r.nrow; q++) r.p[q + 1] += r.p[q];
+            r.i.resize(c.nz);
+            r.x.resize(c.nz);
+            std::vector<long> pos(r.p.begin(), r.p.end() - 1);
+            for (long col = 0; col < nploc; col++)
+                for (long k = c.p[col]; k < c.p[col + 1]; k++) {
+                    r.i[pos[c.i[k]]] = col;
+                    r.x[pos[c.i[k]]] = c.x[k];
+                    pos[c.i[k]]++;
+                }
+            s->gprow[ip] = std::move(r);
+        }
+        fill_views(s);
+    });
+}
+
 int mt_fdpcg_prepare(const mt_fdpcg_in_t *in, mt_fdpcg_out_t *out) {
     if (!in || !out || in->nps <= 0 || !in->nx || !in->ny || !in->os || !in->ht || !in->l2cf ||
         in->nsa <= 0 || !in->salocx || !in->salocy || !in->saa || in->nwfs <= 0 || !in->thetax ||

@@ -120,16 +120,102 @@ def test_empty_cell_is_written_empty(tmp_path):
     assert binio.read(p).shape == (0, 0)
 
 
-def test_refuses_other_types_and_truncation(tmp_path):
+def test_other_element_types(tmp_path):
+    """The element types of sys/bin.h:34-51 and the sparse layout of
+    math/spbin.c:32-89, from records built byte by byte here: single float,
+    complex double, 64- and 32-bit integers, sparse double with 64- and
+    32-bit indices, sparse complex; our writer gives back the same bytes."""
+    import scipy.sparse as sp
     p = tmp_path / "f.bin"
-    open(p, "wb").write(struct.pack("<IIQQ", M_SKIP, 0x6408, 2, 1) + struct.pack("<2f", 1, 2))  # M_FLT
+    # M_FLT 2 x 1
+    open(p, "wb").write(head("", 0x6408, 2, 1) + struct.pack("<2f", 1.5, -2))
+    a = binio.read(p)
+    assert a.dtype == np.float32 and a.shape == (2, 1) and list(a[:, 0]) == [1.5, -2]
+    # M_CMP 1 x 2
+    open(p, "wb").write(head("", 0x6404, 1, 2) + struct.pack("<4d", 1, 2, -3, 4))
+    a = binio.read(p)
+    assert a.dtype == np.complex128 and np.array_equal(a, np.array([[1 + 2j, -3 + 4j]]))
+    # M_INT64 / M_INT32 3 x 1
+    for magic, fmt in ((0x6403, "<3q"), (0x6405, "<3i")):
+        open(p, "wb").write(head("", magic, 3, 1) + struct.pack(fmt, 7, -8, 9))
+        a = binio.read(p)
+        assert a.dtype == np.int64 and list(a[:, 0]) == [7, -8, 9]
+    # sparse double 3 x 2: (0,0)=1.5 (2,0)=-2 (1,1)=4
+    P, I, X = [0, 2, 3], [0, 2, 1], [1.5, -2.0, 4.0]
+    want = sp.csc_matrix((X, I, P), shape=(3, 2))
+    rec64 = head("", 0x6401, 3, 2) + struct.pack("<Q3q3q3d", 3, *P, *I, *X)
+    open(p, "wb").write(rec64)
+    got = binio.read(p)
+    assert sp.issparse(got) and (got != want).nnz == 0 and got.dtype == np.float64
+    rec32 = head("", 0x6407, 3, 2) + struct.pack("<Q3i3i3d", 3, *P, *I, *X)
+    open(p, "wb").write(rec32)
+    assert (binio.read(p) != want).nnz == 0
+    q = tmp_path / "w.bin"
+    binio.write(q, want)
+    assert open(q, "rb").read() == rec64
+    # sparse complex
+    Xc = [1 + 1j, 2, -3j]
+    recc = head("", 0x6400, 3, 2) + struct.pack("<Q3q3q6d", 3, *P, *I, 1, 1, 2, 0, 0, -3)
+    open(p, "wb").write(recc)
+    got = binio.read(p)
+    assert got.dtype == np.complex128 and np.array_equal(got.toarray(), sp.csc_matrix((Xc, I, P), shape=(3, 2)).toarray())
+    binio.write(q, got)
+    assert open(q, "rb").read() == recc
+    # round trips of every type, with a header and inside a cell
+    cell = [np.arange(4.0).reshape(2, 2), np.array([[1 + 2j]]), np.arange(3, dtype=np.float32),
+            np.arange(5).reshape(5, 1), want, None]
+    binio.write(tmp_path / "c.bin.gz", cell, header="k=v")
+    back, hdr = binio.read(tmp_path / "c.bin.gz", with_header=True)
+    assert hdr == "k=v"
+    assert np.array_equal(back[0, 0], cell[0]) and np.array_equal(back[1, 0], cell[1])
+    assert back[2, 0].dtype == np.float32 and np.array_equal(back[2, 0][:, 0], cell[2])
+    assert np.array_equal(back[3, 0], cell[3]) and (back[4, 0] != want).nnz == 0 and back[5, 0] is None
+
+
+def test_refuses_unknown_types_and_truncation(tmp_path):
+    p = tmp_path / "f.bin"
+    open(p, "wb").write(struct.pack("<IIQQ", M_SKIP, 0x640A, 2, 1) + struct.pack("<2b", 1, 2))  # M_INT8
     with pytest.raises(_abi.MtError):
         binio.read(p)
     open(p, "wb").write(head("", M_DBL, 4, 1) + struct.pack("<2d", 1, 2))  # 2 of 4 values
     with pytest.raises(_abi.MtError):
         binio.read(p)
+    # sparse row index out of range
+    open(p, "wb").write(head("", 0x6401, 3, 2) + struct.pack("<Q3q3q3d", 3, 0, 2, 3, 0, 5, 1, 1, 2, 3))
+    with pytest.raises(_abi.MtError):
+        binio.read(p)
     with pytest.raises(_abi.MtError):
         binio.read(tmp_path / "missing.bin")
+
+
+def test_load_gp(tmp_path):
+    """load.GP: the recon's own GP written as MAOS's GP.bin (a cell of sparse
+    matrices, per powfs or per WFS) and loaded back gives the same
+    operators; a scaled GP scales the right-hand side accordingly."""
+    import scipy.sparse as sp
+    from oracle.oracle import Oracle
+    cfg = configs.small8()
+    r = tomo.Recon(cfg)
+    grad = np.random.default_rng(2).standard_normal(r.ng)
+    rhs = Oracle(r).tomo_r(grad)
+    gps = [r.gp(w) for w in range(r.nwfs)]
+    binio.write(tmp_path / "GP.bin", [gps[0]])              # one per powfs
+    r2 = tomo.Recon(cfg)
+    binio.load_gp(r2, tmp_path / "GP.bin")
+    for w in range(r.nwfs):
+        assert (r2.gp(w) != gps[w]).nnz == 0
+    assert np.array_equal(Oracle(r2).tomo_r(grad), rhs)
+    binio.write(tmp_path / "GPw.bin", [2.0 * g for g in gps])  # one per WFS
+    r3 = tomo.Recon(cfg)
+    binio.load_gp(r3, tmp_path / "GPw.bin")
+    assert np.allclose(Oracle(r3).tomo_r(grad), 2.0 * rhs, rtol=1e-12, atol=0)
+    # the wrong shape, or WFS of one powfs with different GP, are refused
+    binio.write(tmp_path / "bad.bin", [sp.csc_matrix((3, 3))])
+    with pytest.raises(_abi.MtError):
+        binio.load_gp(tomo.Recon(cfg), tmp_path / "bad.bin")
+    binio.write(tmp_path / "mix.bin", [gps[0], 2.0 * gps[1], gps[2], gps[3]])
+    with pytest.raises(_abi.MtError):
+        binio.load_gp(tomo.Recon(cfg), tmp_path / "mix.bin")
 
 
 def test_grad_and_opdr_layouts(tmp_path):
