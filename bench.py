@@ -419,7 +419,8 @@ def bench_precision(seeds=8, iters=(10, 30, 60)):
     o = Oracle(r, nthread=16)
     grads = [workload.make_inputs(cfg, r, 500 + k) for k in range(seeds)]
     ref = {(it, k): o.solve(grads[k], it) for it in iters for k in range(seeds)}
-    modes = {1: "FP32 operators, FP64 residual recurrence + forward spectrum + reductions (default)",
+    modes = {4: "FP64 everywhere: vectors, operators, FDPCG transforms, reductions (all-FP64 GPU plan)",
+             1: "FP32 operators, FP64 residual recurrence + forward spectrum + reductions (default)",
              2: "FP32 operators and spectrum, FP64 residual recurrence + reductions",
              0: "FP32 bulk, FP64 reductions and alpha/beta",
              3: "FP32 everywhere (reductions too)"}
@@ -430,15 +431,18 @@ def bench_precision(seeds=8, iters=(10, 30, 60)):
         for it in iters:
             g = tomo.GpuTomo(r, maxit=it)
             g.precision(mode)
-            errs, ratios = [], []
+            errs, ratios, ms = [], [], []
             for k in range(seeds):
+                t0 = time.perf_counter()
                 x, res, _ = g.tomo(grads[k])
+                ms.append(1e3 * (time.perf_counter() - t0))
                 xo, reso = ref[(it, k)]
                 errs.append(float(np.linalg.norm(x - xo) / np.linalg.norm(xo)))
                 ratios.append(float(res / reso))
             g.close()
             rows[str(it)] = dict(rel_l2_median=float(np.median(errs)), rel_l2_max=float(np.max(errs)),
-                                 residual_ratio_median=float(np.median(ratios)))
+                                 residual_ratio_median=float(np.median(ratios)),
+                                 ms_per_host_solve_median=float(np.median(ms)))
         out["modes"][str(mode)] = dict(description=desc, iterations=rows)
     return out
 
@@ -501,8 +505,9 @@ def bench_update(cfg, r, maxit, reps=5):
 def bench_fit(steps=20, batch=32, mvm_cols=512, mvm_batch=256):
     """DM fitting after the large30 tomography (two DMs at 0 and 11.8 km,
     the 9-direction 30" fitting ring, 100 cold CG iterations, FP64): one
-    system per call and `batch` systems per call (one workgroup per system,
-    one launch per call), against the FP64 fitting oracle on the host cores;
+    system per call, `batch` systems per call and 8 x `batch` per call (the
+    same inputs repeated; five launches per CG iteration shared by the
+    systems of a call), against the FP64 fitting oracle on the host cores;
     and MVM assembly (control-matrix columns: tomography then fit of each
     unit gradient vector, on the device)."""
     import torch
@@ -514,11 +519,12 @@ def bench_fit(steps=20, batch=32, mvm_cols=512, mvm_batch=256):
     r = tomo.Recon(cfg)
     f = tomo.FitSetup(r, fc)
     t = tomo.GpuTomo(r, nbatch=batch, maxit=cfg["maxit"])
-    g = tomo.GpuFit(r, f, nbatch=batch)
+    big = 8 * batch
+    g = tomo.GpuFit(r, f, nbatch=big)
     grads = [workload.make_inputs(cfg, r, 700 + k) for k in range(batch)]
     X = t.tomo_batch(np.stack(grads))
-    dx = torch.tensor(X.astype(np.float32), device="cuda:0")
-    da = torch.zeros((batch, f.nact), dtype=torch.float64, device="cuda:0")
+    dx = torch.tensor(np.tile(X.astype(np.float32), (8, 1)), device="cuda:0")
+    da = torch.zeros((big, f.nact), dtype=torch.float64, device="cuda:0")
     st = torch.cuda.current_stream().cuda_stream
 
     def timed(n, reps):
@@ -536,6 +542,7 @@ def bench_fit(steps=20, batch=32, mvm_cols=512, mvm_batch=256):
 
     one = timed(1, max(100, steps))
     many = timed(batch, max(100, steps))
+    bigr = timed(big, max(20, steps // 4))
     a_gpu = da[0].cpu().numpy()
     o = FitOracle(r, f, nthread=cpu_threads())
     xo = X[0].astype(np.float32).astype(np.float64)
@@ -548,7 +555,7 @@ def bench_fit(steps=20, batch=32, mvm_cols=512, mvm_batch=256):
         lat.append(time.perf_counter() - t1)
     out = dict(workload="large30 fit: 2 DMs (0, 11.8 km; %d actuators), 9 fitting directions (30\" ring), "
                         "floc %d points, %d CG iterations, cold, FP64" % (f.nact, f.nfloc, f.rec.maxit),
-               one_system=one, batch=dict(systems=batch, **many),
+               one_system=one, batch=dict(systems=batch, **many), batch_large=dict(systems=big, **bigr),
                rel_l2_vs_fp64_oracle=float(np.linalg.norm(a_gpu - ao) / np.linalg.norm(ao)),
                cpu_oracle=dict(solves_per_s=len(lat) / sum(lat), p50_ms=1e3 * float(np.median(lat)),
                                threads=cpu_threads()))

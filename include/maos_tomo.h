@@ -306,6 +306,14 @@ int  gpu_tomo_P(gpu_tomo_t *g, const double *xin, double *xout);
  *   2  r, A*p, b FP64; forward spectrum FP32
  *   3  as 0, and the dot products, alpha and beta in FP32 too (FP32
  *      everywhere; for the precision study)
+ *   4  everything FP64: vectors, operator coefficients, the full-spectrum
+ *      FDPCG transforms and the reductions (the reference's CUDA_DOUBLE
+ *      build, cuda/common.h:39-45), on a separate FP64 plan made when the
+ *      mode is selected; it serves gpu_tomo_solve, gpu_tomo_solve_batch,
+ *      gpu_tomo_L/R/P, gpu_tomo_get_x and gpu_tomo_residuals, and the FP32
+ *      device-buffer, group and MVM entry points refuse the plan (MT_EINVAL)
+ *      while it is active.  Entering and leaving mode 4 carries the
+ *      warm-start solutions over (rounded to FP32 on leaving).
  * A 30-iteration FDPCG solve of the 30 m system drifts ~2e-3 (relative L2)
  * from the FP64 solve with mode 0 and stays ~1e-5 with mode 1. */
 int  gpu_tomo_precision(gpu_tomo_t *g, int mode);
@@ -550,7 +558,8 @@ int  gpu_fit_solve(gpu_fit_t *g, int isys, const double *opdr, double *dmfit,
                    double *cgres);
 /* nsys <= nbatch systems with device buffers (FP32 layer vectors in, FP64
  * DM vectors out), enqueued on stream (0: the plan's own); no host sync.
- * One workgroup per system runs the whole CG (one launch). */
+ * Five launches per CG iteration, shared by the nsys systems.  d_dmfit
+ * holds the warm start when the plan has warm = 1. */
 int  gpu_fit_solve_dev(gpu_fit_t *g, const float *d_opdr, double *d_dmfit,
                        int nsys, void *stream);
 /* Operators for parity checks (host doubles, one system):

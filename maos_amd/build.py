@@ -19,7 +19,7 @@ LIB = os.path.join(BUILD, "libmaos_tomo.so")
 
 HOST_SRC = ["csrc/host/setup.cpp", "csrc/host/geom.cpp", "csrc/host/fdpcg_setup.cpp", "csrc/host/binio.cpp",
             "csrc/host/fit.cpp", "csrc/host/conf.cpp"]
-HIP_SRC = ["csrc/hip/kernels.hip", "csrc/hip/plan.hip", "csrc/hip/fit.hip"]
+HIP_SRC = ["csrc/hip/kernels.hip", "csrc/hip/plan.hip", "csrc/hip/fit.hip", "csrc/hip/solve64.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed FP32 instructions are disabled for all device code (wrong results
 # have been observed from some of their forms on this hardware).

@@ -10,11 +10,18 @@
 //
 // Design.  A fit is small (the 30 m system with two DMs: 8.5k actuators,
 // 4k fitting-grid points per direction) and its CG is a chain of dependent
-// passes, so the whole solve of one system runs in ONE workgroup of 1024
-// threads (16 waves) that loops over the CG iterations with workgroup
-// barriers between the passes: one launch per batch of systems instead of
-// ~6 launches per iteration, each system on its own CU (a batch of n systems
-// fills n CUs; MVM assembly solves thousands).  Every operator is a gather:
+// passes.  Each pass is one launch over the whole GPU, with work items
+// (point, direction, system) or (actuator, direction, system), so that a
+// thread does one short chain of dependent loads per pass; an iteration is
+// five launches (HA, W, HA' + low-rank terms, x/r update, p update) and a
+// batch of systems shares every launch.  Scalars (alpha, beta, the W1 and NW
+// projections) are fixed-order sums of per-workgroup partials, summed again
+// by every workgroup that needs them, so no launch waits on the host and a
+// system's bits do not depend on its batch neighbours.  (Running the whole
+// solve of a system inside one 1024-thread workgroup, with workgroup
+// barriers between the passes, was 20x slower for one system: every thread
+// then walks ~35 dependent load chains per pass.)  Every operator is a
+// gather:
 //   HX_f, HA_f  bilinear interpolation of the layers / DM grids at the
 //               fitting-grid points along direction f, from per-axis tap
 //               tables (the grids are square and axis-aligned, so a 2-D tap
@@ -80,7 +87,6 @@ struct DBuf {
 };
 
 constexpr int FMAXDM = 8, FMAXFIT = 32, FMAXL = 16, FMAXNW = 32;
-constexpr int FTHREADS = 1024;
 constexpr double FEPS = 1e-15;  // tap weights not above this are dropped (lib/mkh.c:109, sys/common.h:88)
 
 // One axis tap pair of a forward interpolation: nodes n and n+1 with
@@ -112,8 +118,9 @@ struct FitDev {
     const Tap2 *hax, *hay;  // [(f*ndm + d)*fnx + px], [(f*ndm + d)*fny + py]
     const Tap2 *hxx, *hxy;  // [(f*npsr + l)*fnx + px], ...
     // transposed HA lists: per (f, d) and actuator column ia (row ja) the
-    // entries [tp[b + ia], tp[b + ia + 1]) of te, b = txb[f*ndm + d]
-    const int *txb, *tyb;
+    // entries [tp[b + ia], tp[b + ia + 1]) of te, b = f tpx + tdx[d] (the
+    // pointer arrays of the (f, d) pairs in order, anx[d] + 1 entries each)
+    int tpx, tpy, tdx[FMAXDM], tdy[FMAXDM];
     const int *txp, *typ;
     const TapT *txe, *tye;
     const double *W0, *W1;  // 9 per point (row form), per point
@@ -121,221 +128,325 @@ struct FitDev {
     const long *slp, *sli;  // S in CSR (slp null: none)
     const double *slx;
     double tikdiag;
-    // per-system workspace, stride given by the fields below
+    // launch shape: point chunks of 256 per direction (nc1), actuator chunks
+    // of hab actuators x nfit directions (nc3), actuator chunks of 256 (nc4)
+    int nc1, nc3, nc4, hab;
+    // per-system partials, pstride doubles each: W1 . opd_f [nfit][nc1] at
+    // o1, p . Ap [nc3] at o2, NW' p [nnw][nc4] at o3, r . r in two slots
+    // [2][nc4] at o4 (iteration parity), b . b [nc4] at o5
+    int o1, o2, o3, o4, o5;
+    long pstride;
+    // per-system workspace
     double *opd, *opd2;     // [s][nfit][nfloc]
     double *r, *p, *Ap, *b; // [s][nact]
+    double *part;           // [s][pstride]
+    int *stop;              // [s]: 1 once a step size was not finite
     double *res;            // [s]
 };
 
-enum { FIT_SOLVE = 0, FIT_R = 1, FIT_L = 2 };
+constexpr int FB = 256;  // threads per workgroup of every fit kernel
 
-__device__ double fit_bsum(double v, double *sh) {
+// Sum of v over the 256 threads of a workgroup in a fixed order (lane tree,
+// then the four waves in order); the result is valid in thread 0.  Every
+// thread of the workgroup must call it.
+__device__ double fit_bred(double v, double *wsum) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (lane == 0) sh[wid] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0;
-        for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += sh[i];
-        sh[FTHREADS / 64] = s;
-    }
-    __syncthreads();
-    return sh[FTHREADS / 64];
+    double s = 0;
+    if (threadIdx.x == 0) s = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
+    return s;
 }
 
-// opd_f = sum over sources (DMs or layers) of the bilinear interpolation
-// at the fitting-grid points; v: stacked source vector (double or float).
-template <typename V>
-__device__ void fwd_interp(const FitDev &D, const Tap2 *tx, const Tap2 *ty, int nsrc, const int *snx, const long *soff,
-                           const V *v, double *opd) {
-    const int fnx = D.fnx;
+// Sum of the n partials at q in index order.
+__device__ __forceinline__ double psum(const double *q, int n) {
+    double s = 0;
+    for (int i = 0; i < n; i++) s += q[i];
+    return s;
+}
+
+__device__ __forceinline__ double *fpart(const FitDev &D, int s) { return D.part + (long)s * D.pstride; }
+
+// opd_f = sum over sources (DMs or layers) of the bilinear interpolation at
+// the fitting-grid point, and the workgroup's partial of W1 . opd_f.
+// Grid (point chunks, directions, systems).  SRC 0: actuator vectors
+// (double), 1: layer vectors (float).  Sources are taken GS at a time with
+// clamped indexes so that the tap loads of a group issue together.
+template <int SRC, int GS>
+__global__ __launch_bounds__(FB) void kf_fwd(const FitDev *__restrict__ Dp, const void *__restrict__ srcv,
+                                              long sstride, int ws0) {
+    const FitDev &D = *Dp;
+    __shared__ double wsum[4];
+    const int f = blockIdx.y, s = ws0 + blockIdx.z;
     const long nf = D.nfloc;
-    for (long t = threadIdx.x; t < (long)D.nfit * nf; t += blockDim.x) {
-        const int f = (int)(t / nf);
-        const long pt = t - (long)f * nf;
-        const int py = (int)(pt / fnx), px = (int)(pt - (long)py * fnx);
-        double acc = 0;
-        for (int d = 0; d < nsrc; d++) {
-            const Tap2 a = tx[((long)f * nsrc + d) * fnx + px];
-            const Tap2 b = ty[((long)f * nsrc + d) * D.fny + py];
-            const V *src = v + soff[d];
+    const long pt = (long)blockIdx.x * FB + threadIdx.x;
+    const bool on = pt < nf;
+    const long ptc = on ? pt : nf - 1;
+    const int py = (int)(ptc / D.fnx), px = (int)(ptc - (long)py * D.fnx);
+    const int nsrc = SRC ? D.npsr : D.ndm;
+    const Tap2 *tx = SRC ? D.hxx : D.hax, *ty = SRC ? D.hxy : D.hay;
+    const int *snx = SRC ? D.xnx : D.anx;
+    const long *soff = SRC ? D.xoff : D.aoff;
+    double acc = 0;
+    for (int d0 = 0; d0 < nsrc; d0 += GS) {
+        Tap2 a[GS], b[GS];
+#pragma unroll
+        for (int j = 0; j < GS; j++) {
+            const int d = min(d0 + j, nsrc - 1);
+            a[j] = tx[((long)f * nsrc + d) * D.fnx + px];
+            b[j] = ty[((long)f * nsrc + d) * D.fny + py];
+        }
+        double v[GS][4];
+#pragma unroll
+        for (int j = 0; j < GS; j++) {
+            const int d = min(d0 + j, nsrc - 1);
+            const long base = soff[d] + (long)blockIdx.z * sstride;
             const int nx = snx[d];
-            const double wx[2] = {a.w0, a.w1}, wy[2] = {b.w0, b.w1};
+            const long i00 = base + (long)b[j].n * nx + a[j].n;
+            // taps of zero weight point at node 0 or -1 of an axis; clamp the
+            // address into the source so the load stays in bounds
+            const long lo = base, hi = base + (long)snx[d] * (SRC ? D.xny[d] : D.any[d]) - 1;
+            const long k[4] = {i00, i00 + 1, i00 + nx, i00 + nx + 1};
 #pragma unroll
-            for (int j = 0; j < 2; j++)
-#pragma unroll
-                for (int i = 0; i < 2; i++) {
-                    const double w = wx[i] * wy[j];
-                    if (w > FEPS) acc += w * (double)src[(long)(b.n + j) * nx + a.n + i];
-                }
-        }
-        opd[t] = acc;
-    }
-}
-
-// opd2_f = wt_f (W0 opd_f - W1 (W1' opd_f)) (applyW, maos/recon_utils.c:155-180)
-__device__ void apply_w(const FitDev &D, const double *opd, double *opd2, double *sh) {
-    const int fnx = D.fnx, fny = D.fny;
-    const long nf = D.nfloc;
-    for (int f = 0; f < D.nfit; f++) {
-        const double *o = opd + (long)f * nf;
-        double dot = 0;
-        for (long i = threadIdx.x; i < nf; i += blockDim.x) dot += D.W1[i] * o[i];
-        dot = fit_bsum(dot, sh);
-        const double wf = D.wt[f];
-        for (long i = threadIdx.x; i < nf; i += blockDim.x) {
-            const int iy = (int)(i / fnx), ix = (int)(i - (long)iy * fnx);
-            const double *w0 = D.W0 + 9 * i;
-            double acc = 0;
-#pragma unroll
-            for (int k = 0; k < 9; k++) {
-                const int jx = ix + k % 3 - 1, jy = iy + k / 3 - 1;
-                const double c = w0[k];
-                if (c != 0.0 && jx >= 0 && jx < fnx && jy >= 0 && jy < fny) acc += c * o[(long)jy * fnx + jx];
+            for (int q = 0; q < 4; q++) {
+                const long kk = k[q] < lo ? lo : (k[q] > hi ? hi : k[q]);
+                v[j][q] = SRC ? (double)((const float *)srcv)[kk] : ((const double *)srcv)[kk];
             }
-            opd2[(long)f * nf + i] = wf * (acc - D.W1[i] * dot);
+        }
+#pragma unroll
+        for (int j = 0; j < GS; j++) {
+            if (d0 + j >= nsrc) break;
+            const double wx[2] = {a[j].w0, a[j].w1}, wy[2] = {b[j].w0, b[j].w1};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const double w = wx[q & 1] * wy[q >> 1];
+                if (w > FEPS) acc += w * v[j][q];
+            }
         }
     }
+    if (on) D.opd[((long)s * D.nfit + f) * nf + pt] = acc;
+    const double pw = fit_bred(on ? D.W1[pt] * acc : 0.0, wsum);
+    if (threadIdx.x == 0) fpart(D, s)[D.o1 + f * D.nc1 + blockIdx.x] = pw;
 }
 
-// out[a] = alpha * sum_f HA_f' opd2_f at actuator a (+ beta * out[a])
-__device__ void hat_gather(const FitDev &D, const double *opd2, double *out, double alpha) {
+// opd2_f = wt_f (W0 opd_f - W1 (W1' opd_f)) (applyW, maos/recon_utils.c:155-180).
+__global__ __launch_bounds__(FB) void kf_w(const FitDev *__restrict__ Dp, int ws0) {
+    const FitDev &D = *Dp;
+    const int f = blockIdx.y, s = ws0 + blockIdx.z;
     const long nf = D.nfloc;
-    const int fnx = D.fnx;
-    for (long a = threadIdx.x; a < D.nact; a += blockDim.x) {
+    const long i = (long)blockIdx.x * FB + threadIdx.x;
+    if (i >= nf) return;
+    const double dot = psum(fpart(D, s) + D.o1 + f * D.nc1, D.nc1);
+    const int fnx = D.fnx, fny = D.fny;
+    const int iy = (int)(i / fnx), ix = (int)(i - (long)iy * fnx);
+    const double *o = D.opd + ((long)s * D.nfit + f) * nf;
+    const double *w0 = D.W0 + 9 * i;
+    double acc = 0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+        const int jx = ix + k % 3 - 1, jy = iy + k / 3 - 1;
+        const double c = w0[k];
+        if (c != 0.0 && jx >= 0 && jx < fnx && jy >= 0 && jy < fny) acc += c * o[(long)jy * fnx + jx];
+    }
+    D.opd2[((long)s * D.nfit + f) * nf + i] = D.wt[f] * (acc - D.W1[i] * dot);
+}
+
+enum { HAT_R = 0, HAT_L = 1 };
+
+// out[a] = sum_f HA_f' opd2_f at actuator a; HAT_L adds NW (NW' p) + S p +
+// tikdiag p and writes the workgroup's partial of p . out.  A workgroup holds
+// hab actuators x nfit directions, one thread per (actuator, direction),
+// gathering through the transposed per-axis lists (no atomics); the
+// directions of an actuator are summed in index order.  Grid (actuator
+// chunks, systems); p and out are per-system vectors of stride nact.
+template <int MODE>
+__global__ __launch_bounds__(FB) void kf_hat(const FitDev *__restrict__ Dp, const double *__restrict__ p,
+                                              double *__restrict__ out, int ws0) {
+    const FitDev &D = *Dp;
+    __shared__ double red[FB];
+    __shared__ double tnw[FMAXNW];
+    __shared__ double wsum[4];
+    const int s = ws0 + blockIdx.y;
+    const int nfit = D.nfit, t = threadIdx.x;
+    const int al = t / nfit, f = t - al * nfit;
+    const long na = D.nact, nf = D.nfloc;
+    const long a = (long)blockIdx.x * D.hab + al;
+    const bool act = al < D.hab && a < na;
+    if (MODE == HAT_L && t < D.nnw) tnw[t] = psum(fpart(D, s) + D.o3 + t * D.nc4, D.nc4);
+    double acc = 0;
+    if (act) {
         int d = 0;
         while (a >= D.aoff[d + 1]) d++;
         const long k = a - D.aoff[d];
         const int ja = (int)(k / D.anx[d]), ia = (int)(k - (long)ja * D.anx[d]);
-        double acc = 0;
-        for (int f = 0; f < D.nfit; f++) {
-            const int fd = f * D.ndm + d;
-            const int bx = D.txb[fd] + ia, by = D.tyb[fd] + ja;
-            const int x0 = D.txp[bx], x1 = D.txp[bx + 1];
-            const int y0 = D.typ[by], y1 = D.typ[by + 1];
-            const double *o = opd2 + (long)f * nf;
-            for (int ey = y0; ey < y1; ey++) {
-                const TapT ty = D.tye[ey];
-                for (int ex = x0; ex < x1; ex++) {
-                    const TapT tx = D.txe[ex];
-                    const double w = tx.w * ty.w;
-                    if (w > FEPS) acc += w * o[(long)ty.i * fnx + tx.i];
-                }
+        const int fd = f * D.ndm + d;
+        const int bx = f * D.tpx + D.tdx[d] + ia, by = f * D.tpy + D.tdy[d] + ja;
+        const int x0 = D.txp[bx], x1 = D.txp[bx + 1];
+        const int y0 = D.typ[by], y1 = D.typ[by + 1];
+        const double *o = D.opd2 + ((long)s * nfit + f) * nf;
+        const int fnx = D.fnx;
+        for (int ey = y0; ey < y1; ey++) {
+            const TapT ty = D.tye[ey];
+            for (int ex = x0; ex < x1; ex++) {
+                const TapT tx = D.txe[ex];
+                const double w = tx.w * ty.w;
+                if (w > FEPS) acc += w * o[(long)ty.i * fnx + tx.i];
             }
         }
-        out[a] = alpha * acc;
     }
-}
-
-// Ap = FitL p: HA, W, HA', then the low-rank, slaving and Tikhonov terms.
-// Returns p.Ap.
-__device__ double apply_L(const FitDev &D, const double *p, double *Ap, double *opd, double *opd2, double *sh,
-                          double *tnw) {
-    fwd_interp(D, D.hax, D.hay, D.ndm, D.anx, D.aoff, p, opd);
-    __syncthreads();
-    apply_w(D, opd, opd2, sh);
-    __syncthreads();
-    hat_gather(D, opd2, Ap, 1.0);
-    // NW t, t = NW' p (dots over all actuators, in a fixed order; t in LDS)
-    for (int k = 0; k < D.nnw; k++) {
-        double s = 0;
-        const double *c = D.NW + (long)k * D.nact;
-        for (long a = threadIdx.x; a < D.nact; a += blockDim.x) s += c[a] * p[a];
-        s = fit_bsum(s, sh);
-        if (threadIdx.x == 0) tnw[k] = s;
-    }
+    red[t] = acc;
     __syncthreads();
     double pap = 0;
-    for (long a = threadIdx.x; a < D.nact; a += blockDim.x) {
-        double v = Ap[a];
-        for (int k = 0; k < D.nnw; k++) v += D.NW[(long)k * D.nact + a] * tnw[k];
-        if (D.slp) {
-            double s = 0;
-            for (long e = D.slp[a]; e < D.slp[a + 1]; e++) s += D.slx[e] * p[D.sli[e]];
-            v += s;
+    if (act && f == 0) {
+        double v = 0;
+        for (int j = 0; j < nfit; j++) v += red[al * nfit + j];
+        if (MODE == HAT_L) {
+            const double *ps = p + (long)blockIdx.y * na;
+            for (int q = 0; q < D.nnw; q++) v += D.NW[(long)q * na + a] * tnw[q];
+            if (D.slp) {
+                double sl = 0;
+                for (long e = D.slp[a]; e < D.slp[a + 1]; e++) sl += D.slx[e] * ps[D.sli[e]];
+                v += sl;
+            }
+            v += D.tikdiag * ps[a];
+            pap = ps[a] * v;
         }
-        v += D.tikdiag * p[a];
-        Ap[a] = v;
-        pap += p[a] * v;
+        out[(long)blockIdx.y * na + a] = v;
     }
-    return fit_bsum(pap, sh);
+    if (MODE == HAT_L) {
+        pap = fit_bred(pap, wsum);
+        if (t == 0) fpart(D, s)[D.o2 + blockIdx.x] = pap;
+    }
 }
 
-// One workgroup per system.  MODE FIT_R: b = FitR xin (into b, and out if
-// given); FIT_L: out = FitL ain; FIT_SOLVE: the whole CG into x.
-template <int MODE>
-__global__ __launch_bounds__(FTHREADS) void k_fit(const FitDev *__restrict__ Dp, const float *__restrict__ xin,
-                                                  long xstride, double *__restrict__ xout,
-                                                  const double *__restrict__ ain, int s0, int warm, int maxit) {
+// Partials of NW' v (into o3) for a per-system actuator vector v; with
+// INIT also zeroes v first when the start is cold, writes the partials of
+// b . b (o5) and clears the system's stop flag.  Grid (actuator chunks of
+// 256, systems).
+template <bool INIT>
+__global__ __launch_bounds__(FB) void kf_init(const FitDev *__restrict__ Dp, double *__restrict__ v, int ws0,
+                                               int warm) {
     const FitDev &D = *Dp;
-    __shared__ double sh[FTHREADS / 64 + 1];
-    __shared__ double tnw[FMAXNW];
-    const int s = s0 + blockIdx.x;
-    const long nf = D.nfit * D.nfloc, na = D.nact;
-    double *opd = D.opd + (long)s * nf, *opd2 = D.opd2 + (long)s * nf;
-    double *r = D.r + (long)s * na, *p = D.p + (long)s * na, *Ap = D.Ap + (long)s * na, *b = D.b + (long)s * na;
-    if (MODE == FIT_L) {
-        (void)apply_L(D, ain + (long)blockIdx.x * na, xout + (long)blockIdx.x * na, opd, opd2, sh, tnw);
-        return;
+    __shared__ double wsum[4];
+    const int s = ws0 + blockIdx.y;
+    const long na = D.nact;
+    const long a = (long)blockIdx.x * FB + threadIdx.x;
+    const bool on = a < na;
+    double *vs = v + (long)blockIdx.y * na;
+    double val = 0;
+    if (on) {
+        if (INIT && !warm) vs[a] = 0;
+        else val = vs[a];
     }
-    // RHS: b = sum_f HA_f' W HX_f x
-    fwd_interp(D, D.hxx, D.hxy, D.npsr, D.xnx, D.xoff, xin + (long)blockIdx.x * xstride, opd);
-    __syncthreads();
-    apply_w(D, opd, opd2, sh);
-    __syncthreads();
-    hat_gather(D, opd2, b, 1.0);
-    __syncthreads();
-    if (MODE == FIT_R) {
-        for (long a = threadIdx.x; a < na; a += blockDim.x) xout[(long)blockIdx.x * na + a] = b[a];
-        return;
-    }
-    // CG (lib/pcg.c:28-113, Mmul = NULL: z = r), restart every 100 iterations
-    double *x = xout + (long)blockIdx.x * na;
-    if (!warm)
-        for (long a = threadIdx.x; a < na; a += blockDim.x) x[a] = 0;
-    double bb = 0;
-    for (long a = threadIdx.x; a < na; a += blockDim.x) bb += b[a] * b[a];
-    bb = fit_bsum(bb, sh);
-    double rr = 0, res = NAN;
-    for (int k = 0; k < maxit; k++) {
-        if (k % 100 == 0) {
-            // r = b - A x; p = r; rr = r.r
-            __syncthreads();
-            (void)apply_L(D, x, Ap, opd, opd2, sh, tnw);
-            double t = 0;
-            for (long a = threadIdx.x; a < na; a += blockDim.x) {
-                const double v = b[a] - Ap[a];
-                r[a] = v;
-                p[a] = v;
-                t += v * v;
-            }
-            rr = fit_bsum(t, sh);
-        }
-        const double pap = apply_L(D, p, Ap, opd, opd2, sh, tnw);
-        const double ak = rr / pap;
-        if (!isfinite(ak)) {
-            res = NAN;
-            break;
-        }
-        double t = 0;
-        for (long a = threadIdx.x; a < na; a += blockDim.x) {
-            x[a] += ak * p[a];
-            const double v = r[a] - ak * Ap[a];
-            r[a] = v;
-            t += v * v;
-        }
-        const double rr2 = fit_bsum(t, sh);
-        if (k + 1 == maxit) {
-            res = sqrt(rr2 / bb);
-            break;
-        }
-        const double bk = rr2 / rr;
-        for (long a = threadIdx.x; a < na; a += blockDim.x) p[a] = bk * p[a] + r[a];
-        rr = rr2;
+    double *q = fpart(D, s);
+    for (int k = 0; k < D.nnw; k++) {
+        const double c = on ? D.NW[(long)k * na + a] * val : 0.0;
+        const double t = fit_bred(c, wsum);
+        if (threadIdx.x == 0) q[D.o3 + k * D.nc4 + blockIdx.x] = t;
         __syncthreads();
     }
-    if (threadIdx.x == 0) D.res[s] = res;
+    if (INIT) {
+        const double bv = on ? D.b[(long)s * na + a] : 0.0;
+        const double t = fit_bred(bv * bv, wsum);
+        if (threadIdx.x == 0) {
+            q[D.o5 + blockIdx.x] = t;
+            if (blockIdx.x == 0) D.stop[s] = 0;
+        }
+    }
+}
+
+// CG (re)start at iteration k: r = b - Ap (COLD: r = b, x being zero),
+// p = r; partials of r . r into slot k & 1 and of NW' p.
+template <bool COLD>
+__global__ __launch_bounds__(FB) void kf_restart(const FitDev *__restrict__ Dp, int ws0, int k) {
+    const FitDev &D = *Dp;
+    __shared__ double wsum[4];
+    const int s = ws0 + blockIdx.y;
+    const long na = D.nact;
+    const long a = (long)blockIdx.x * FB + threadIdx.x;
+    const bool on = a < na;
+    double v = 0;
+    if (on) {
+        const long i = (long)s * na + a;
+        v = COLD ? D.b[i] : D.b[i] - D.Ap[i];
+        D.r[i] = v;
+        D.p[i] = v;
+    }
+    double *q = fpart(D, s);
+    const double t = fit_bred(v * v, wsum);
+    if (threadIdx.x == 0) q[D.o4 + (k & 1) * D.nc4 + blockIdx.x] = t;
+    for (int j = 0; j < D.nnw; j++) {
+        __syncthreads();
+        const double c = on ? D.NW[(long)j * na + a] * v : 0.0;
+        const double u = fit_bred(c, wsum);
+        if (threadIdx.x == 0) q[D.o3 + j * D.nc4 + blockIdx.x] = u;
+    }
+}
+
+// Iteration k, first half: alpha = rr_k / p.Ap; x += alpha p; r -= alpha Ap;
+// partials of r . r into slot (k + 1) & 1.  A non-finite alpha stops the
+// system (x keeps its value, the residual is NaN).
+__global__ __launch_bounds__(FB) void kf_upd(const FitDev *__restrict__ Dp, double *__restrict__ x, int ws0,
+                                             int k) {
+    const FitDev &D = *Dp;
+    __shared__ double wsum[4];
+    const int s = ws0 + blockIdx.y;
+    if (D.stop[s]) return;
+    const double *q = fpart(D, s);
+    const double rr = psum(q + D.o4 + (k & 1) * D.nc4, D.nc4);
+    const double pap = psum(q + D.o2, D.nc3);
+    const double ak = rr / pap;
+    if (!isfinite(ak)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            D.stop[s] = 1;
+            D.res[s] = NAN;
+        }
+        return;
+    }
+    const long na = D.nact;
+    const long a = (long)blockIdx.x * FB + threadIdx.x;
+    double v = 0;
+    if (a < na) {
+        const long i = (long)s * na + a;
+        x[(long)blockIdx.y * na + a] += ak * D.p[i];
+        v = D.r[i] - ak * D.Ap[i];
+        D.r[i] = v;
+    }
+    const double t = fit_bred(v * v, wsum);
+    if (threadIdx.x == 0) fpart(D, s)[D.o4 + ((k + 1) & 1) * D.nc4 + blockIdx.x] = t;
+}
+
+// Iteration k, second half: beta = rr_(k+1) / rr_k; p = beta p + r and the
+// partials of NW' p; after the last iteration the residual sqrt(rr / b.b).
+__global__ __launch_bounds__(FB) void kf_p(const FitDev *__restrict__ Dp, int ws0, int k, int last) {
+    const FitDev &D = *Dp;
+    __shared__ double wsum[4];
+    const int s = ws0 + blockIdx.y;
+    if (D.stop[s]) return;
+    const double *qs = fpart(D, s);
+    const double rr2 = psum(qs + D.o4 + ((k + 1) & 1) * D.nc4, D.nc4);
+    if (last) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) D.res[s] = sqrt(rr2 / psum(qs + D.o5, D.nc4));
+        return;
+    }
+    const double rr = psum(qs + D.o4 + (k & 1) * D.nc4, D.nc4);
+    const double bk = rr2 / rr;
+    const long na = D.nact;
+    const long a = (long)blockIdx.x * FB + threadIdx.x;
+    const bool on = a < na;
+    double v = 0;
+    if (on) {
+        const long i = (long)s * na + a;
+        v = bk * D.p[i] + D.r[i];
+        D.p[i] = v;
+    }
+    double *q = fpart(D, s);
+    for (int j = 0; j < D.nnw; j++) {
+        const double c = on ? D.NW[(long)j * na + a] * v : 0.0;
+        const double u = fit_bred(c, wsum);
+        if (threadIdx.x == 0) q[D.o3 + j * D.nc4 + blockIdx.x] = u;
+        __syncthreads();
+    }
 }
 
 // Forward taps of one axis: output coordinates u_k (in source grid units).
@@ -366,11 +477,12 @@ struct gpu_fit {
     FitDev H{};
     DBuf<FitDev> dev;
     DBuf<Tap2> hax, hay, hxx, hxy;
-    DBuf<int> txb, tyb, txp, typ;
+    DBuf<int> txp, typ;
     DBuf<TapT> txe, tye;
     DBuf<double> W0, W1, NW, slx;
     DBuf<long> slp, sli;
-    DBuf<double> opd, opd2, r, p, Ap, b, res, x;
+    DBuf<double> opd, opd2, r, p, Ap, b, part, res, x;
+    DBuf<int> stop;
     DBuf<float> xin;
     std::vector<float> hstage;
 };
@@ -470,8 +582,18 @@ void build_fit_plan(gpu_fit *g, const mt_recon_t *R, const mt_fit_recon_t *F) {
     g->hay.upload(hay);
     g->hxx.upload(hxx);
     g->hxy.upload(hxy);
-    g->txb.upload(txb);
-    g->tyb.upload(tyb);
+    // the pointer arrays' bases follow from the grid sizes
+    D.tpx = D.tpy = 0;
+    for (int d = 0; d < D.ndm; d++) {
+        D.tdx[d] = D.tpx;
+        D.tdy[d] = D.tpy;
+        D.tpx += D.anx[d] + 1;
+        D.tpy += D.any[d] + 1;
+    }
+    for (int f = 0; f < D.nfit; f++)
+        for (int d = 0; d < D.ndm; d++)
+            if (txb[f * D.ndm + d] != f * D.tpx + D.tdx[d] || tyb[f * D.ndm + d] != f * D.tpy + D.tdy[d])
+                ffail(MT_EINVAL, "fitting tables: unexpected layout");
     g->txp.upload(txp);
     g->typ.upload(typ);
     g->txe.upload(txe);
@@ -490,8 +612,6 @@ void build_fit_plan(gpu_fit *g, const mt_recon_t *R, const mt_fit_recon_t *F) {
     D.hay = g->hay.p;
     D.hxx = g->hxx.p;
     D.hxy = g->hxy.p;
-    D.txb = g->txb.p;
-    D.tyb = g->tyb.p;
     D.txp = g->txp.p;
     D.typ = g->typ.p;
     D.txe = g->txe.p;
@@ -511,6 +631,19 @@ void build_fit_plan(gpu_fit *g, const mt_recon_t *R, const mt_fit_recon_t *F) {
     g->p.alloc(ns * D.nact);
     g->Ap.alloc(ns * D.nact);
     g->b.alloc(ns * D.nact);
+    D.nc1 = (int)((D.nfloc + FB - 1) / FB);
+    D.hab = FB / D.nfit;
+    D.nc3 = (int)((D.nact + D.hab - 1) / D.hab);
+    D.nc4 = (int)((D.nact + FB - 1) / FB);
+    D.o1 = 0;
+    D.o2 = D.o1 + D.nfit * D.nc1;
+    D.o3 = D.o2 + D.nc3;
+    D.o4 = D.o3 + D.nnw * D.nc4;
+    D.o5 = D.o4 + 2 * D.nc4;
+    D.pstride = D.o5 + D.nc4;
+    g->part.alloc(ns * D.pstride);
+    g->stop.alloc(ns);
+    FCHK(hipMemset(g->stop.p, 0, sizeof(int) * ns));
     g->res.alloc(ns);
     g->x.alloc(ns * D.nact);
     g->xin.alloc(ns * D.nxtot);
@@ -521,9 +654,72 @@ void build_fit_plan(gpu_fit *g, const mt_recon_t *R, const mt_fit_recon_t *F) {
     D.p = g->p.p;
     D.Ap = g->Ap.p;
     D.b = g->b.p;
+    D.part = g->part.p;
+    D.stop = g->stop.p;
     D.res = g->res.p;
     g->dev.alloc(1);
     FCHK(hipMemcpy(g->dev.p, &D, sizeof(FitDev), hipMemcpyHostToDevice));
+}
+
+// Launch sequences.  ws0: first workspace slot; n: systems; external
+// vectors are given for the first system of the launch, stride nact (actuator
+// vectors) or xstride (layer vectors).
+dim3 grid_pts(const FitDev &D, int n) { return dim3(D.nc1, D.nfit, n); }
+dim3 grid_hat(const FitDev &D, int n) { return dim3(D.nc3, n); }
+dim3 grid_act(const FitDev &D, int n) { return dim3(D.nc4, n); }
+
+// Interpolation from nsrc sources, GS of them per group of loads: 2 for up
+// to two sources, 3 for three, else 4 (a partial last group reloads the
+// last source).
+template <int SRC>
+void launch_fwd(const FitDev &D, int nsrc, dim3 grid, hipStream_t st, const FitDev *dev, const void *src, long sstride,
+                int ws0) {
+    if (nsrc <= 2) hipLaunchKernelGGL((kf_fwd<SRC, 2>), grid, dim3(FB), 0, st, dev, src, sstride, ws0);
+    else if (nsrc == 3) hipLaunchKernelGGL((kf_fwd<SRC, 3>), grid, dim3(FB), 0, st, dev, src, sstride, ws0);
+    else hipLaunchKernelGGL((kf_fwd<SRC, 4>), grid, dim3(FB), 0, st, dev, src, sstride, ws0);
+}
+
+// out = sum_f wt_f HA_f' W HX_f x (FitR)
+void enq_rhs(gpu_fit *g, hipStream_t st, const float *xin, long xstride, double *out, int ws0, int n) {
+    const FitDev &D = g->H;
+    launch_fwd<1>(D, D.npsr, grid_pts(D, n), st, g->dev.p, (const void *)xin, xstride, ws0);
+    hipLaunchKernelGGL(kf_w, grid_pts(D, n), dim3(FB), 0, st, g->dev.p, ws0);
+    hipLaunchKernelGGL(kf_hat<HAT_R>, grid_hat(D, n), dim3(FB), 0, st, g->dev.p, (const double *)nullptr, out, ws0);
+}
+
+// out = FitL p; the partials of NW' p must be in place (kf_init, kf_restart
+// or kf_p wrote them).
+void enq_L(gpu_fit *g, hipStream_t st, const double *p, double *out, int ws0, int n) {
+    const FitDev &D = g->H;
+    launch_fwd<0>(D, D.ndm, grid_pts(D, n), st, g->dev.p, (const void *)p, D.nact, ws0);
+    hipLaunchKernelGGL(kf_w, grid_pts(D, n), dim3(FB), 0, st, g->dev.p, ws0);
+    hipLaunchKernelGGL(kf_hat<HAT_L>, grid_hat(D, n), dim3(FB), 0, st, g->dev.p, p, out, ws0);
+}
+
+// The whole solve of n systems (lib/pcg.c:28-113 with Mmul = NULL: z = r,
+// restart every 100 iterations): 5 launches per iteration.
+void enq_solve(gpu_fit *g, hipStream_t st, const float *xin, long xstride, double *x, int ws0, int n, int warm,
+               int maxit) {
+    const FitDev &D = g->H;
+    const long na = D.nact;
+    double *b = g->b.p + (long)ws0 * na, *p = g->p.p + (long)ws0 * na, *Ap = g->Ap.p + (long)ws0 * na;
+    enq_rhs(g, st, xin, xstride, b, ws0, n);
+    hipLaunchKernelGGL(kf_init<true>, grid_act(D, n), dim3(FB), 0, st, g->dev.p, x, ws0, warm);
+    for (int k = 0; k < maxit; k++) {
+        if (k % 100 == 0) {
+            if (k == 0 && !warm) {
+                hipLaunchKernelGGL(kf_restart<true>, grid_act(D, n), dim3(FB), 0, st, g->dev.p, ws0, k);
+            } else {
+                if (k > 0) hipLaunchKernelGGL(kf_init<false>, grid_act(D, n), dim3(FB), 0, st, g->dev.p, x, ws0, 1);
+                enq_L(g, st, x, Ap, ws0, n);
+                hipLaunchKernelGGL(kf_restart<false>, grid_act(D, n), dim3(FB), 0, st, g->dev.p, ws0, k);
+            }
+        }
+        enq_L(g, st, p, Ap, ws0, n);
+        hipLaunchKernelGGL(kf_upd, grid_act(D, n), dim3(FB), 0, st, g->dev.p, x, ws0, k);
+        hipLaunchKernelGGL(kf_p, grid_act(D, n), dim3(FB), 0, st, g->dev.p, ws0, k, (int)(k + 1 == maxit));
+    }
+    FCHK(hipGetLastError());
 }
 
 void stage_x(gpu_fit *g, int isys, const double *opdr) {
@@ -544,7 +740,7 @@ extern "C" {
 int gpu_fit_setup(const mt_recon_t *recon, const mt_fit_recon_t *fit, int device, int nbatch, gpu_fit_t **out) {
     if (!out) return MT_EINVAL;
     *out = nullptr;
-    if (!recon || !fit || nbatch <= 0 || fit->maxit <= 0 || fit->maxit > 100000) {
+    if (!recon || !fit || nbatch <= 0 || nbatch > 65535 || fit->maxit <= 0 || fit->maxit > 100000) {
         mt::set_error("gpu_fit_setup: invalid arguments");
         return MT_EINVAL;
     }
@@ -574,10 +770,8 @@ int gpu_fit_solve(gpu_fit_t *g, int isys, const double *opdr, double *dmfit, dou
         FCHK(hipSetDevice(g->device));
         stage_x(g, isys, opdr);
         const long na = g->H.nact;
-        hipLaunchKernelGGL(k_fit<FIT_SOLVE>, dim3(1), dim3(FTHREADS), 0, g->stream, g->dev.p,
-                           g->xin.p + (long)isys * g->H.nxtot, g->H.nxtot, g->x.p + (long)isys * na,
-                           (const double *)nullptr, isys, g->warm, g->maxit);
-        FCHK(hipGetLastError());
+        enq_solve(g, g->stream, g->xin.p + (long)isys * g->H.nxtot, g->H.nxtot, g->x.p + (long)isys * na, isys, 1,
+                  g->warm, g->maxit);
         FCHK(hipMemcpyAsync(dmfit, g->x.p + (long)isys * na, sizeof(double) * na, hipMemcpyDeviceToHost, g->stream));
         FCHK(hipMemcpyAsync(&res, g->res.p + isys, sizeof(double), hipMemcpyDeviceToHost, g->stream));
         FCHK(hipStreamSynchronize(g->stream));
@@ -592,9 +786,7 @@ int gpu_fit_solve_dev(gpu_fit_t *g, const float *d_opdr, double *d_dmfit, int ns
     return fguarded([&] {
         FCHK(hipSetDevice(g->device));
         hipStream_t st = stream ? (hipStream_t)stream : g->stream;
-        hipLaunchKernelGGL(k_fit<FIT_SOLVE>, dim3(nsys), dim3(FTHREADS), 0, st, g->dev.p, d_opdr, g->H.nxtot,
-                           d_dmfit, (const double *)nullptr, 0, g->warm, g->maxit);
-        FCHK(hipGetLastError());
+        enq_solve(g, st, d_opdr, g->H.nxtot, d_dmfit, 0, nsys, g->warm, g->maxit);
     });
 }
 
@@ -604,8 +796,7 @@ int gpu_fit_R(gpu_fit_t *g, const double *opdr, double *out, double alpha) {
         FCHK(hipSetDevice(g->device));
         stage_x(g, 0, opdr);
         const long na = g->H.nact;
-        hipLaunchKernelGGL(k_fit<FIT_R>, dim3(1), dim3(FTHREADS), 0, g->stream, g->dev.p, g->xin.p, g->H.nxtot,
-                           g->Ap.p, (const double *)nullptr, 0, 0, 0);
+        enq_rhs(g, g->stream, g->xin.p, g->H.nxtot, g->Ap.p, 0, 1);
         FCHK(hipGetLastError());
         std::vector<double> h(na);
         FCHK(hipMemcpyAsync(h.data(), g->Ap.p, sizeof(double) * na, hipMemcpyDeviceToHost, g->stream));
@@ -623,8 +814,8 @@ int gpu_fit_L(gpu_fit_t *g, const double *a, double *out, double alpha, double b
             if (!std::isfinite(a[i])) ffail(MT_ENOTFINITE, "non-finite input vector");
         // the input goes to the p slot of system 0, the output to its r slot
         FCHK(hipMemcpyAsync(g->p.p, a, sizeof(double) * na, hipMemcpyHostToDevice, g->stream));
-        hipLaunchKernelGGL(k_fit<FIT_L>, dim3(1), dim3(FTHREADS), 0, g->stream, g->dev.p, (const float *)nullptr,
-                           0L, g->r.p, (const double *)g->p.p, 0, 0, 0);
+        hipLaunchKernelGGL(kf_init<false>, grid_act(g->H, 1), dim3(FB), 0, g->stream, g->dev.p, g->p.p, 0, 1);
+        enq_L(g, g->stream, g->p.p, g->r.p, 0, 1);
         FCHK(hipGetLastError());
         std::vector<double> h(na);
         FCHK(hipMemcpyAsync(h.data(), g->r.p, sizeof(double) * na, hipMemcpyDeviceToHost, g->stream));
@@ -659,9 +850,7 @@ int fit_enqueue_cold(gpu_fit_t *g, const float *d_x, long xstride, double *d_dm,
         const long na = g->H.nact;
         for (int s0 = 0; s0 < nsys; s0 += g->nbatch) {
             const int n = std::min(g->nbatch, nsys - s0);
-            hipLaunchKernelGGL(k_fit<FIT_SOLVE>, dim3(n), dim3(FTHREADS), 0, st, g->dev.p, d_x + (long)s0 * xstride,
-                               xstride, d_dm + (long)s0 * na, (const double *)nullptr, 0, 0, g->maxit);
-            FCHK(hipGetLastError());
+            enq_solve(g, st, d_x + (long)s0 * xstride, xstride, d_dm + (long)s0 * na, 0, n, 0, g->maxit);
         }
     });
 }

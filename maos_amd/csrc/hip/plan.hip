@@ -32,6 +32,7 @@
 
 #include "fit_internal.hpp"
 #include "rays.hpp"
+#include "solve64.hpp"
 #include "kernels.hpp"
 #include "launch.hpp"
 #include "maos_tomo.h"
@@ -57,6 +58,10 @@ struct HipError {
     } while (0)
 
 [[noreturn]] void fail(int code, const std::string &m) { throw HipError{code, m}; }
+// Status of an FP64-plan call (which has set the error message) as an exception.
+void chk64(int rc) {
+    if (rc != MT_OK) fail(rc, mt_last_error());
+}
 
 template <typename T>
 struct DevBuf {
@@ -213,6 +218,11 @@ struct gpu_tomo {
     bool colr_ok = false; // the register column kernel applies (per-frequency blocks, ny = shL x shP)
     bool lty_ok = false;  // LTY-row tiles fit every layer of this (matched) plan
     bool lty2_ok = false; // LTY2-row tiles do
+    // precision mode 4 (all FP64): a separate FP64 plan, made from a host
+    // copy of the recon when the mode is selected, which holds the warm-start
+    // state while the mode is active
+    recon64 *rc64 = nullptr;
+    tomo64 *t64 = nullptr;
     // profiling
     int prof = 0;
     std::vector<ProfEv> evs;  // profiling events
@@ -221,6 +231,14 @@ struct gpu_tomo {
 };
 
 namespace {
+
+// Entry points of the FP32-operator path (device buffers, groups, the
+// autotuner, MVM assembly) refuse a plan in precision mode 4.
+void no_mode4(const gpu_tomo *g, const char *what) {
+    if (g->t64)
+        fail(MT_EINVAL, std::string(what) + ": not available in precision mode 4 (all FP64), which runs through "
+                                            "gpu_tomo_solve, gpu_tomo_solve_batch and the operator applies");
+}
 
 
 FDiv make_fdiv(unsigned d) {
@@ -1628,6 +1646,7 @@ int gpu_tomo_setup(const mt_recon_t *recon, int device, int nbatch, int maxit, i
         g->maxit = maxit;
         g->warm = warm ? 1 : 0;
         build_plan(g.get(), recon);
+        chk64(t64_keep(recon, &g->rc64));
         alloc_state(g.get());
         HIPCHK(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
         noalloc_init(g.get());
@@ -1672,6 +1691,8 @@ int gpu_tomo_update_cn2(gpu_tomo_t *g, const mt_recon_t *recon) {
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
         update_cn2_plan(g, recon);
+        chk64(t64_keep_cn2(g->rc64, recon));
+        if (g->t64) chk64(t64_update_cn2(g->t64, recon));
     });
 }
 
@@ -1681,6 +1702,7 @@ int gpu_tomo_reset(gpu_tomo_t *g) {
         HIPCHK(hipSetDevice(g->device));
         HIPCHK(hipMemsetAsync(g->x.p, 0, sizeof(float) * g->x.n, g->stream));
         HIPCHK(hipStreamSynchronize(g->stream));
+        if (g->t64) chk64(t64_reset(g->t64));
     });
 }
 
@@ -1700,6 +1722,8 @@ void gpu_tomo_free(gpu_tomo_t *g) {
     }
     if (g->fork) (void)hipEventDestroy(g->fork);
     if (g->stream) (void)hipStreamDestroy(g->stream);
+    t64_free(g->t64);
+    t64_keep_free(g->rc64);
     delete g;
 }
 
@@ -1725,6 +1749,10 @@ int gpu_tomo_solve(gpu_tomo_t *g, int isys, const double *grad, double *x, doubl
     double res = NAN;
     int rc = guarded([&] {
         HIPCHK(hipSetDevice(g->device));
+        if (g->t64) {
+            chk64(t64_solve(g->t64, isys, 1, grad, x, &res, g->maxit, g->warm, g->stream));
+            return;
+        }
         to_dev_grad(g, isys, grad);
         Rec rec{&g->evs, g->stream, false};
         enqueue_solve(own_batch(g, isys, 1), rec);
@@ -1741,6 +1769,10 @@ int gpu_tomo_solve_batch(gpu_tomo_t *g, int nsys, const double *grad, double *x,
     std::vector<double> res(nsys, NAN);
     int rc = guarded([&] {
         HIPCHK(hipSetDevice(g->device));
+        if (g->t64) {
+            chk64(t64_solve(g->t64, 0, nsys, grad, x, res.data(), g->maxit, g->warm, g->stream));
+            return;
+        }
         for (int s = 0; s < nsys; s++) to_dev_grad(g, s, grad + (long)s * g->ngtot_all);
         Rec rec{&g->evs, g->stream, false};
         enqueue_solve(own_batch(g, 0, nsys), rec);
@@ -1846,6 +1878,7 @@ int gpu_tomo_solve_dev(gpu_tomo_t *g, const float *d_grad, float *d_x, int nbatc
     if (!g || nbatch <= 0 || nbatch > g->nbatch) return MT_EINVAL;
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
+        no_mode4(g, "gpu_tomo_solve_dev");
         hipStream_t st = stream ? (hipStream_t)stream : g->stream;
         const Geom &G = g->G;
         if (d_grad && d_grad != g->g.p) launch_copy(d_grad, g->g.p, (long)nbatch * G.ngtot, st);
@@ -1890,6 +1923,7 @@ int gpu_tomo_solve_dev_noalloc(gpu_tomo_t *g, const float *d_grad, float *d_x, d
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
         if (g->prof) fail(MT_ESTATE, "gpu_tomo_solve_dev_noalloc: profiling is on (it allocates events)");
+        no_mode4(g, "gpu_tomo_solve_dev_noalloc");
         unsigned char *base = nullptr;
         {
             std::lock_guard<std::mutex> lk(scratch_mu);
@@ -1972,6 +2006,7 @@ int gpu_tomo_solve_dev_noalloc(gpu_tomo_t *g, const float *d_grad, float *d_x, d
 // warm-start state) are restored at the end.
 static void mvm_columns(gpu_tomo *g, gpu_fit_t *f, long col0, long ncol, double *out) {
     HIPCHK(hipSetDevice(g->device));
+    no_mode4(g, "MVM assembly");
     const Geom &G = g->G;
     const long nx = g->nxtot;
     const long nrow = f ? fit_nact(f) : nx;  // output rows per column
@@ -2101,6 +2136,10 @@ int gpu_tomo_residuals(gpu_tomo_t *g, double *cgres, int nbatch) {
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
         HIPCHK(hipDeviceSynchronize());
+        if (g->t64) {
+            chk64(t64_residuals(g->t64, cgres, nbatch, g->stream));
+            return;
+        }
         HIPCHK(hipMemcpy(cgres, g->res.p, sizeof(double) * nbatch, hipMemcpyDeviceToHost));
     });
 }
@@ -2109,6 +2148,12 @@ int gpu_tomo_L(gpu_tomo_t *g, const double *xin, double *xout, double alpha, dou
     if (!g || !xin || !xout) return MT_EINVAL;
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
+        if (g->t64) {
+            std::vector<double> tmp(g->nxtot);
+            chk64(t64_L(g->t64, xin, tmp.data(), g->stream));
+            for (long i = 0; i < g->nxtot; i++) xout[i] = beta * xout[i] + alpha * tmp[i];
+            return;
+        }
         to_dev_x(g, g->z.p, xin);
         Rec rec{&g->evs, g->stream, false};
         enqueue_L(own_batch(g, 0, 1), BUF_Z, PART_NONE, rec);
@@ -2122,6 +2167,11 @@ int gpu_tomo_R(gpu_tomo_t *g, const double *grad, double *xout, double alpha) {
     if (!g || !grad || !xout) return MT_EINVAL;
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
+        if (g->t64) {
+            chk64(t64_R(g->t64, grad, xout, g->stream));
+            for (long i = 0; i < g->nxtot; i++) xout[i] *= alpha;
+            return;
+        }
         to_dev_grad(g, 0, grad);
         Rec rec{&g->evs, g->stream, false};
         enqueue_R(own_batch(g, 0, 1), rec);
@@ -2135,6 +2185,10 @@ int gpu_tomo_P(gpu_tomo_t *g, const double *xin, double *xout) {
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
         if (!g->G.precond) fail(MT_EINVAL, "no FDPCG preconditioner in this plan");
+        if (g->t64) {
+            chk64(t64_P(g->t64, xin, xout, g->stream));
+            return;
+        }
         // b <- xin, r <- b through the cold-start row kernel (which also
         // clears its x argument: the p buffer is used as that scratch), then P
         to_dev_rt(g, g->b.p, xin);
@@ -2151,6 +2205,7 @@ int gpu_tomo_set_grad(gpu_tomo_t *g, int isys, const double *grad) {
     if (!g || !grad || isys < 0 || isys >= g->nbatch) return MT_EINVAL;
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
+        no_mode4(g, "gpu_tomo_set_grad");
         to_dev_grad(g, isys, grad);
     });
 }
@@ -2160,6 +2215,10 @@ int gpu_tomo_get_x(gpu_tomo_t *g, int isys, double *x, double *cgres) {
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
         HIPCHK(hipDeviceSynchronize());
+        if (g->t64) {
+            chk64(t64_get_x(g->t64, isys, x, cgres, g->stream));
+            return;
+        }
         from_dev_x(g, x, g->x.p + (long)isys * g->G.nxtot);
         if (cgres) HIPCHK(hipMemcpy(cgres, g->res.p + isys, sizeof(double), hipMemcpyDeviceToHost));
     });
@@ -2189,8 +2248,10 @@ int gpu_tomo_group_create(int n, gpu_tomo_t *const *plans, const int *s0, const 
     if (!grp) return MT_ENOMEM;
     int rc = guarded([&] {
         std::vector<gpu_tomo *> pl(plans, plans + n);
-        for (auto *g : pl)
+        for (auto *g : pl) {
             if (!g) fail(MT_EINVAL, "null plan");
+            no_mode4(g, "gpu_tomo_group_create");
+        }
         grp->device = pl[0]->device;
         HIPCHK(hipSetDevice(grp->device));
         grp->plans = pl;
@@ -2209,6 +2270,7 @@ int gpu_tomo_group_create(int n, gpu_tomo_t *const *plans, const int *s0, const 
 // gpu_tomo_autotune) has a new gen, and the group rebuilds its batch and
 // captures its graph again before solving.
 static void group_refresh(gpu_tomo_group *grp) {
+    for (auto *g : grp->plans) no_mode4(g, "gpu_tomo_group_solve");
     bool moved = false;
     for (size_t j = 0; j < grp->plans.size(); j++) moved |= grp->plans[j]->gen != grp->gens[j];
     if (!moved) return;
@@ -2491,11 +2553,28 @@ int gpu_tomo_autotune(gpu_tomo_t *g, int nsys, int *choice) {
 }
 
 int gpu_tomo_precision(gpu_tomo_t *g, int mode) {
-    if (!g || mode < 0 || mode > 3) return MT_EINVAL;
+    if (!g || mode < 0 || mode > 4) return MT_EINVAL;
     return guarded([&] {
         HIPCHK(hipSetDevice(g->device));
         HIPCHK(hipStreamSynchronize(g->stream));
         HIPCHK(hipDeviceSynchronize());
+        if (mode == 4) {
+            // the FP64 plan takes over the warm-start state
+            if (!g->t64) {
+                tomo64 *t = nullptr;
+                chk64(t64_create(t64_view(g->rc64), g->nbatch, &t));
+                g->t64 = t;
+                chk64(t64_load_x(g->t64, g->x.p, g->stream));
+            }
+            return;
+        }
+        if (g->t64) {
+            // back to an FP32-operator mode: the FP64 warm-start state, rounded
+            const int rc = t64_store_x(g->t64, g->x.p, g->stream);
+            t64_free(g->t64);
+            g->t64 = nullptr;
+            chk64(rc);
+        }
         g->rt64 = mode == 1 || mode == 2;
         g->spec64 = mode == 1;
         g->G.dot32 = mode == 3;

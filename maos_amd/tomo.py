@@ -156,7 +156,9 @@ class GpuTomo:
     def precision(self, mode):
         """0: FP32 residual and spectrum; 1: FP64 residual recurrence and
         forward spectrum (default); 2: FP64 residual, FP32 spectrum; 3: as 0
-        with FP32 dot products and alpha/beta (FP32 everywhere)."""
+        with FP32 dot products and alpha/beta (FP32 everywhere); 4: all FP64
+        (vectors, operators, FDPCG transforms, reductions), through the host
+        calls (tomo, tomo_batch, L, R, P, get_x, residuals) only."""
         _abi.check(_abi.lib().gpu_tomo_precision(self._h, int(mode)), "gpu_tomo_precision")
 
     def info(self):

@@ -6,7 +6,7 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def fp32_spread(o, grad, maxit, draws=2, eps=3e-7, xt=None, dist=None):
+def fp32_spread(o, grad, maxit, draws=2, eps=3e-7, xt=None, dist=None, to32=True):
     """How far the oracle's own PCG (the steps of or_pcg, lib/pcg.c:28-113)
     moves when each preconditioner output is rounded to FP32 with a relative
     error of eps added, the size of the rounding of an FP32 inverse FFT and
@@ -16,7 +16,9 @@ def fp32_spread(o, grad, maxit, draws=2, eps=3e-7, xt=None, dist=None):
     iteration count, although the system is far from converged: b8_w2_l4
     gives 2e-7 at 20 iterations, 2e-6 at 25, 4e-5 to 3e-4 at 30 and 3e-3
     at 40.  The same PCG with eps = 0 reproduces the oracle to 3e-9.
-    With xt and a list dist, the draws' distances to xt are appended."""
+    With xt and a list dist, the draws' distances to xt are appended.
+    to32=False keeps the perturbed outputs in FP64 (with eps ~ 1e-15: the
+    oracle's own spread under FP64 rounding differences)."""
     b = o.tomo_r(grad)
     xo, _ = o.solve(grad, maxit)
     worst = 0.0
@@ -25,14 +27,17 @@ def fp32_spread(o, grad, maxit, draws=2, eps=3e-7, xt=None, dist=None):
 
         def z_of(v):
             z = o.fdpcg(v)
-            return (z * (1 + eps * rng.standard_normal(z.shape))).astype(np.float32).astype(np.float64)
+            z = z * (1 + eps * rng.standard_normal(z.shape))
+            return z.astype(np.float32).astype(np.float64) if to32 else z
 
         x = np.zeros_like(b)
-        rr = b.copy()
-        z = z_of(rr)
-        p = z.copy()
-        rz = rr @ z
         for k in range(maxit):
+            if k % 100 == 0:
+                # (re)start, as lib/pcg.c:55: r = b - A x, p = z
+                rr = b - o.tomo_l(x) if k else b.copy()
+                z = z_of(rr)
+                p = z.copy()
+                rz = rr @ z
             ap = o.tomo_l(p)
             a = rz / (p @ ap)
             x += a * p
