@@ -579,6 +579,31 @@ def bench_fit(steps=20, batch=32, mvm_cols=512, mvm_batch=256):
     return out
 
 
+def fit_profile_run(n, calls=3):
+    """Three cold large30 DM fits of n systems per call (synthetic layer
+    inputs; the work of a fit does not depend on the values), each timed
+    to the host, for a kernel trace of the fitting kernels."""
+    import torch
+    from maos_amd import configs, tomo
+    base = configs.large30()
+    fc = configs.fit_for(base)
+    cfg = configs.with_fit_dirs(base, fc)
+    r = tomo.Recon(cfg)
+    f = tomo.FitSetup(r, fc)
+    g = tomo.GpuFit(r, f, nbatch=n)
+    X = 1e-7 * np.random.default_rng(1).standard_normal((n, r.nx))
+    dx = torch.tensor(X.astype(np.float32), device="cuda:0")
+    da = torch.zeros((n, f.nact), dtype=torch.float64, device="cuda:0")
+    st = torch.cuda.current_stream().cuda_stream
+    for _ in range(calls):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.solve_dev(dx.data_ptr(), da.data_ptr(), nsys=n, stream=st)
+        torch.cuda.synchronize()
+        log("fit of %d systems: %.2f ms" % (n, 1e3 * (time.perf_counter() - t0)))
+    g.close()
+
+
 _STREAM_JOBS = None
 
 
@@ -754,6 +779,8 @@ def main():
     ap.add_argument("--systems-per-workgroup", metavar="W,L,C[,ROWS[,COLK[,FLIGHTS]]]", default=None,
                     help="fix the systems per workgroup of the WFS, layer and column kernels (and the layer "
                          "tile rows) instead of autotuning (profiling runs, whose timing the counters distort)")
+    ap.add_argument("--fit-profile", metavar="N", type=int, default=None,
+                    help="only run 3 cold large30 DM fits of N systems per call (for a rocprofv3 kernel trace)")
     ap.add_argument("--traffic-summary", metavar="DIR", default=None,
                     help="no GPU: fold rocprofv3 FETCH_SIZE / WRITE_SIZE runs under DIR/{fetch,write,"
                          "copy_fetch,copy_write} into profiles/traffic_large30.json")
@@ -770,6 +797,9 @@ def main():
         return
     if args.traffic_summary:
         traffic_summary(args.traffic_summary, args.batch)
+        return
+    if args.fit_profile:
+        fit_profile_run(args.fit_profile)
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (the solve runs on one MI355X; "

@@ -158,11 +158,17 @@ __device__ double fit_bred(double v, double *wsum) {
     return s;
 }
 
-// Sum of the n partials at q in index order.
-__device__ __forceinline__ double psum(const double *q, int n) {
-    double s = 0;
-    for (int i = 0; i < n; i++) s += q[i];
-    return s;
+// Sum of the n partials at q in a fixed order (thread-strided sums, then the
+// lane tree and the four waves in order), returned to every thread.  Every
+// thread of the workgroup must call it; psh is its own LDS (not fit_bred's).
+__device__ double psum(const double *q, int n, double *psh) {
+    double v = 0;
+    for (int i = threadIdx.x; i < n; i += FB) v += q[i];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    __syncthreads();  // the previous call's readers are done with psh
+    if ((threadIdx.x & 63) == 0) psh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return ((psh[0] + psh[1]) + psh[2]) + psh[3];
 }
 
 __device__ __forceinline__ double *fpart(const FitDev &D, int s) { return D.part + (long)s * D.pstride; }
@@ -182,7 +188,9 @@ __global__ __launch_bounds__(FB) void kf_fwd(const FitDev *__restrict__ Dp, cons
     const long pt = (long)blockIdx.x * FB + threadIdx.x;
     const bool on = pt < nf;
     const long ptc = on ? pt : nf - 1;
-    const int py = (int)(ptc / D.fnx), px = (int)(ptc - (long)py * D.fnx);
+    // 32-bit index arithmetic (a 64-bit division is a long instruction
+    // sequence; every grid here is far below 2^31 points)
+    const int py = (int)ptc / D.fnx, px = (int)ptc - py * D.fnx;
     const int nsrc = SRC ? D.npsr : D.ndm;
     const Tap2 *tx = SRC ? D.hxx : D.hax, *ty = SRC ? D.hxy : D.hay;
     const int *snx = SRC ? D.xnx : D.anx;
@@ -232,13 +240,14 @@ __global__ __launch_bounds__(FB) void kf_fwd(const FitDev *__restrict__ Dp, cons
 // opd2_f = wt_f (W0 opd_f - W1 (W1' opd_f)) (applyW, maos/recon_utils.c:155-180).
 __global__ __launch_bounds__(FB) void kf_w(const FitDev *__restrict__ Dp, int ws0) {
     const FitDev &D = *Dp;
+    __shared__ double psh[4];
     const int f = blockIdx.y, s = ws0 + blockIdx.z;
     const long nf = D.nfloc;
     const long i = (long)blockIdx.x * FB + threadIdx.x;
+    const double dot = psum(fpart(D, s) + D.o1 + f * D.nc1, D.nc1, psh);
     if (i >= nf) return;
-    const double dot = psum(fpart(D, s) + D.o1 + f * D.nc1, D.nc1);
     const int fnx = D.fnx, fny = D.fny;
-    const int iy = (int)(i / fnx), ix = (int)(i - (long)iy * fnx);
+    const int iy = (int)i / fnx, ix = (int)i - iy * fnx;
     const double *o = D.opd + ((long)s * D.nfit + f) * nf;
     const double *w0 = D.W0 + 9 * i;
     double acc = 0;
@@ -257,29 +266,35 @@ enum { HAT_R = 0, HAT_L = 1 };
 // tikdiag p and writes the workgroup's partial of p . out.  A workgroup holds
 // hab actuators x nfit directions, one thread per (actuator, direction),
 // gathering through the transposed per-axis lists (no atomics); the
-// directions of an actuator are summed in index order.  Grid (actuator
-// chunks, systems); p and out are per-system vectors of stride nact.
+// directions of an actuator are summed in index order.  Actuators are the
+// fast index of the threads, so a wave's gathers of one direction read
+// neighbouring fitting-grid points (with directions fastest, every lane read
+// another direction's grid: one cache line per lane and load).  Grid
+// (actuator chunks, systems); p and out are per-system vectors of stride nact.
 template <int MODE>
 __global__ __launch_bounds__(FB) void kf_hat(const FitDev *__restrict__ Dp, const double *__restrict__ p,
                                               double *__restrict__ out, int ws0) {
     const FitDev &D = *Dp;
     __shared__ double red[FB];
     __shared__ double tnw[FMAXNW];
-    __shared__ double wsum[4];
+    __shared__ double wsum[4], psh[4];
     const int s = ws0 + blockIdx.y;
     const int nfit = D.nfit, t = threadIdx.x;
-    const int al = t / nfit, f = t - al * nfit;
+    const int f = t / D.hab, al = t - f * D.hab;
     const long na = D.nact, nf = D.nfloc;
     const long a = (long)blockIdx.x * D.hab + al;
-    const bool act = al < D.hab && a < na;
-    if (MODE == HAT_L && t < D.nnw) tnw[t] = psum(fpart(D, s) + D.o3 + t * D.nc4, D.nc4);
+    const bool act = f < nfit && a < na;
+    if (MODE == HAT_L)
+        for (int k = 0; k < D.nnw; k++) {
+            const double v = psum(fpart(D, s) + D.o3 + k * D.nc4, D.nc4, psh);
+            if (t == 0) tnw[k] = v;
+        }
     double acc = 0;
     if (act) {
         int d = 0;
         while (a >= D.aoff[d + 1]) d++;
-        const long k = a - D.aoff[d];
-        const int ja = (int)(k / D.anx[d]), ia = (int)(k - (long)ja * D.anx[d]);
-        const int fd = f * D.ndm + d;
+        const int k = (int)(a - D.aoff[d]);
+        const int ja = k / D.anx[d], ia = k - ja * D.anx[d];
         const int bx = f * D.tpx + D.tdx[d] + ia, by = f * D.tpy + D.tdy[d] + ja;
         const int x0 = D.txp[bx], x1 = D.txp[bx + 1];
         const int y0 = D.typ[by], y1 = D.typ[by + 1];
@@ -299,7 +314,7 @@ __global__ __launch_bounds__(FB) void kf_hat(const FitDev *__restrict__ Dp, cons
     double pap = 0;
     if (act && f == 0) {
         double v = 0;
-        for (int j = 0; j < nfit; j++) v += red[al * nfit + j];
+        for (int j = 0; j < nfit; j++) v += red[j * D.hab + al];
         if (MODE == HAT_L) {
             const double *ps = p + (long)blockIdx.y * na;
             for (int q = 0; q < D.nnw; q++) v += D.NW[(long)q * na + a] * tnw[q];
@@ -389,12 +404,12 @@ __global__ __launch_bounds__(FB) void kf_restart(const FitDev *__restrict__ Dp, 
 __global__ __launch_bounds__(FB) void kf_upd(const FitDev *__restrict__ Dp, double *__restrict__ x, int ws0,
                                              int k) {
     const FitDev &D = *Dp;
-    __shared__ double wsum[4];
+    __shared__ double wsum[4], psh[4];
     const int s = ws0 + blockIdx.y;
     if (D.stop[s]) return;
     const double *q = fpart(D, s);
-    const double rr = psum(q + D.o4 + (k & 1) * D.nc4, D.nc4);
-    const double pap = psum(q + D.o2, D.nc3);
+    const double rr = psum(q + D.o4 + (k & 1) * D.nc4, D.nc4, psh);
+    const double pap = psum(q + D.o2, D.nc3, psh);
     const double ak = rr / pap;
     if (!isfinite(ak)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -420,16 +435,19 @@ __global__ __launch_bounds__(FB) void kf_upd(const FitDev *__restrict__ Dp, doub
 // partials of NW' p; after the last iteration the residual sqrt(rr / b.b).
 __global__ __launch_bounds__(FB) void kf_p(const FitDev *__restrict__ Dp, int ws0, int k, int last) {
     const FitDev &D = *Dp;
-    __shared__ double wsum[4];
+    __shared__ double wsum[4], psh[4];
     const int s = ws0 + blockIdx.y;
     if (D.stop[s]) return;
     const double *qs = fpart(D, s);
-    const double rr2 = psum(qs + D.o4 + ((k + 1) & 1) * D.nc4, D.nc4);
+    const double rr2 = psum(qs + D.o4 + ((k + 1) & 1) * D.nc4, D.nc4, psh);
     if (last) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) D.res[s] = sqrt(rr2 / psum(qs + D.o5, D.nc4));
+        if (blockIdx.x == 0) {
+            const double bb = psum(qs + D.o5, D.nc4, psh);
+            if (threadIdx.x == 0) D.res[s] = sqrt(rr2 / bb);
+        }
         return;
     }
-    const double rr = psum(qs + D.o4 + (k & 1) * D.nc4, D.nc4);
+    const double rr = psum(qs + D.o4 + (k & 1) * D.nc4, D.nc4, psh);
     const double bk = rr2 / rr;
     const long na = D.nact;
     const long a = (long)blockIdx.x * FB + threadIdx.x;
@@ -485,6 +503,18 @@ struct gpu_fit {
     DBuf<int> stop;
     DBuf<float> xin;
     std::vector<float> hstage;
+    // captured solves (the ~500 launches of a 100-iteration solve replay with
+    // graph-node gaps instead of launch gaps), keyed by their arguments
+    struct Graph {
+        const float *xin;
+        long xstride;
+        double *x;
+        int ws0, n, warm, maxit;
+        hipGraph_t graph;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;  // most recently used last
+    hipStream_t cap = nullptr;  // capture stream
 };
 
 namespace {
@@ -577,7 +607,9 @@ void build_fit_plan(gpu_fit *g, const mt_recon_t *R, const mt_fit_recon_t *F) {
             hxy.insert(hxy.end(), ty.begin(), ty.end());
         }
     }
-    if (txe.size() > (size_t)INT32_MAX || tye.size() > (size_t)INT32_MAX) ffail(MT_EINVAL, "fitting tables too large");
+    if (txe.size() > (size_t)INT32_MAX || tye.size() > (size_t)INT32_MAX || D.nfloc * D.nfit >= INT32_MAX ||
+        D.nact >= INT32_MAX)
+        ffail(MT_EINVAL, "fitting tables too large");
     g->hax.upload(hax);
     g->hay.upload(hay);
     g->hxx.upload(hxx);
@@ -722,6 +754,65 @@ void enq_solve(gpu_fit *g, hipStream_t st, const float *xin, long xstride, doubl
     FCHK(hipGetLastError());
 }
 
+constexpr size_t FIT_GRAPHS = 8;  // captured solves kept per plan
+
+void drop_graphs(gpu_fit *g) {
+    for (auto &e : g->graphs) {
+        (void)hipGraphExecDestroy(e.exec);
+        (void)hipGraphDestroy(e.graph);
+    }
+    g->graphs.clear();
+}
+
+// The solve as a replay of a captured graph on st (captured on first use of
+// these arguments); direct launches when st is itself being captured.
+void run_solve(gpu_fit *g, hipStream_t st, const float *xin, long xstride, double *x, int ws0, int n, int warm,
+               int maxit) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    FCHK(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone) {
+        enq_solve(g, st, xin, xstride, x, ws0, n, warm, maxit);
+        return;
+    }
+    size_t k = 0;
+    for (; k < g->graphs.size(); k++) {
+        const auto &e = g->graphs[k];
+        if (e.xin == xin && e.xstride == xstride && e.x == x && e.ws0 == ws0 && e.n == n && e.warm == warm &&
+            e.maxit == maxit)
+            break;
+    }
+    if (k == g->graphs.size()) {
+        if (!g->cap) FCHK(hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking));
+        gpu_fit::Graph e{xin, xstride, x, ws0, n, warm, maxit, nullptr, nullptr};
+        FCHK(hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal));
+        try {
+            enq_solve(g, g->cap, xin, xstride, x, ws0, n, warm, maxit);
+        } catch (...) {
+            hipGraph_t dropped = nullptr;
+            (void)hipStreamEndCapture(g->cap, &dropped);
+            if (dropped) (void)hipGraphDestroy(dropped);
+            throw;
+        }
+        FCHK(hipStreamEndCapture(g->cap, &e.graph));
+        const hipError_t ie = hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(e.graph);
+            FCHK(ie);
+        }
+        if (g->graphs.size() >= FIT_GRAPHS) {
+            (void)hipGraphExecDestroy(g->graphs.front().exec);
+            (void)hipGraphDestroy(g->graphs.front().graph);
+            g->graphs.erase(g->graphs.begin());
+        }
+        g->graphs.push_back(e);
+        k = g->graphs.size() - 1;
+    } else if (k + 1 != g->graphs.size()) {
+        std::rotate(g->graphs.begin() + k, g->graphs.begin() + k + 1, g->graphs.end());
+        k = g->graphs.size() - 1;
+    }
+    FCHK(hipGraphLaunch(g->graphs[k].exec, st));
+}
+
 void stage_x(gpu_fit *g, int isys, const double *opdr) {
     const long n = g->H.nxtot;
     g->hstage.resize(n);
@@ -770,7 +861,7 @@ int gpu_fit_solve(gpu_fit_t *g, int isys, const double *opdr, double *dmfit, dou
         FCHK(hipSetDevice(g->device));
         stage_x(g, isys, opdr);
         const long na = g->H.nact;
-        enq_solve(g, g->stream, g->xin.p + (long)isys * g->H.nxtot, g->H.nxtot, g->x.p + (long)isys * na, isys, 1,
+        run_solve(g, g->stream, g->xin.p + (long)isys * g->H.nxtot, g->H.nxtot, g->x.p + (long)isys * na, isys, 1,
                   g->warm, g->maxit);
         FCHK(hipMemcpyAsync(dmfit, g->x.p + (long)isys * na, sizeof(double) * na, hipMemcpyDeviceToHost, g->stream));
         FCHK(hipMemcpyAsync(&res, g->res.p + isys, sizeof(double), hipMemcpyDeviceToHost, g->stream));
@@ -786,7 +877,7 @@ int gpu_fit_solve_dev(gpu_fit_t *g, const float *d_opdr, double *d_dmfit, int ns
     return fguarded([&] {
         FCHK(hipSetDevice(g->device));
         hipStream_t st = stream ? (hipStream_t)stream : g->stream;
-        enq_solve(g, st, d_opdr, g->H.nxtot, d_dmfit, 0, nsys, g->warm, g->maxit);
+        run_solve(g, st, d_opdr, g->H.nxtot, d_dmfit, 0, nsys, g->warm, g->maxit);
     });
 }
 
@@ -838,6 +929,8 @@ void gpu_fit_free(gpu_fit_t *g) {
     (void)hipSetDevice(g->device);
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     (void)hipDeviceSynchronize();
+    drop_graphs(g);
+    if (g->cap) (void)hipStreamDestroy(g->cap);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
 }
@@ -850,7 +943,7 @@ int fit_enqueue_cold(gpu_fit_t *g, const float *d_x, long xstride, double *d_dm,
         const long na = g->H.nact;
         for (int s0 = 0; s0 < nsys; s0 += g->nbatch) {
             const int n = std::min(g->nbatch, nsys - s0);
-            enq_solve(g, st, d_x + (long)s0 * xstride, xstride, d_dm + (long)s0 * na, 0, n, 0, g->maxit);
+            run_solve(g, st, d_x + (long)s0 * xstride, xstride, d_dm + (long)s0 * na, 0, n, 0, g->maxit);
         }
     });
 }

@@ -149,10 +149,17 @@ __device__ double bred(double v, double *wsum) {
     if (threadIdx.x == 0) s = ((wsum[0] + wsum[1]) + wsum[2]) + wsum[3];
     return s;
 }
-__device__ __forceinline__ double psum(const double *q, int n) {
-    double s = 0;
-    for (int i = 0; i < n; i++) s += q[i];
-    return s;
+// Sum of the n partials at q in a fixed order (thread-strided sums, then the
+// lane tree and the four waves in order), returned to every thread.  Every
+// thread of the workgroup must call it; psh is its own LDS (not bred's).
+__device__ double psum(const double *q, int n, double *psh) {
+    double v = 0;
+    for (int i = threadIdx.x; i < n; i += B64) v += q[i];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    __syncthreads();  // the previous call's readers are done with psh
+    if ((threadIdx.x & 63) == 0) psh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return ((psh[0] + psh[1]) + psh[2]) + psh[3];
 }
 __device__ __forceinline__ double *prt(const D64 &D, int s) { return D.part + (long)s * D.pstride; }
 
@@ -225,16 +232,17 @@ __global__ __launch_bounds__(B64) void k64_grad(const D64 *__restrict__ Dp, int 
 template <bool FROM_OPD>
 __global__ __launch_bounds__(B64) void k64_wg(const D64 *__restrict__ Dp, int ws0) {
     const D64 &D = *Dp;
+    __shared__ double psh[4];
     const int k = blockIdx.y, s = ws0 + blockIdx.z;
     const long nsa = D.nsa[k];
     const long i = (long)blockIdx.x * B64 + threadIdx.x;
-    if (i >= nsa) return;
     double j0 = 0, j1 = 0;
     if (D.ttr[k] && (!FROM_OPD || D.ptt_lhs)) {
         const double *q = prt(D, s) + D.o_tt + (long)k * 2 * D.nchg;
-        j0 = psum(q, D.nchg);
-        j1 = psum(q + D.nchg, D.nchg);
+        j0 = psum(q, D.nchg, psh);
+        j1 = psum(q + D.nchg, D.nchg, psh);
     }
+    if (i >= nsa) return;
     const long o = (long)s * D.ngtot + D.goff[k];
     const double *gsrc = FROM_OPD ? D.g : D.gin;
     const double gx = gsrc[o + i] - j0, gy = gsrc[o + nsa + i] - j1;
@@ -517,12 +525,13 @@ __global__ __launch_bounds__(B64) void k64_pfirst(const D64 *__restrict__ Dp, in
 // without a preconditioner, of r . z = r . r into slot (k + 1) & 1).
 __global__ __launch_bounds__(B64) void k64_upd(const D64 *__restrict__ Dp, int ws0, int k) {
     const D64 &D = *Dp;
-    __shared__ double wsum[4];
+    __shared__ double wsum[4], psh[4];
     const int s = ws0 + blockIdx.y;
     if (D.stop[s]) return;
     const double *q = prt(D, s);
     const int nrzk = D.precond ? D.ncw : D.nch;
-    const double ak = psum(q + D.o_rz + (k & 1) * D.nrz, nrzk) / psum(q + D.o_pap, D.nch);
+    const double rz = psum(q + D.o_rz + (k & 1) * D.nrz, nrzk, psh);
+    const double ak = rz / psum(q + D.o_pap, D.nch, psh);
     if (!isfinite(ak)) {
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             D.stop[s] = 1;
@@ -550,15 +559,21 @@ __global__ __launch_bounds__(B64) void k64_upd(const D64 *__restrict__ Dp, int w
 // preconditioner); after the last iteration the residual sqrt(r.r / b.b).
 __global__ __launch_bounds__(B64) void k64_p(const D64 *__restrict__ Dp, int ws0, int k, int last) {
     const D64 &D = *Dp;
+    __shared__ double psh[4];
     const int s = ws0 + blockIdx.y;
     if (D.stop[s]) return;
     const double *q = prt(D, s);
     if (last) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) D.res[s] = sqrt(psum(q + D.o_rr, D.nch) / psum(q + D.o_bb, D.nch));
+        if (blockIdx.x == 0) {
+            const double rr = psum(q + D.o_rr, D.nch, psh);
+            const double bb = psum(q + D.o_bb, D.nch, psh);
+            if (threadIdx.x == 0) D.res[s] = sqrt(rr / bb);
+        }
         return;
     }
     const int nrzk = D.precond ? D.ncw : D.nch;
-    const double bk = psum(q + D.o_rz + ((k + 1) & 1) * D.nrz, nrzk) / psum(q + D.o_rz + (k & 1) * D.nrz, nrzk);
+    const double rz2 = psum(q + D.o_rz + ((k + 1) & 1) * D.nrz, nrzk, psh);
+    const double bk = rz2 / psum(q + D.o_rz + (k & 1) * D.nrz, nrzk, psh);
     const long n = (long)blockIdx.x * B64 + threadIdx.x;
     if (n < D.nxtot) {
         const long i = (long)s * D.nxtot + n;

@@ -6,7 +6,7 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def fp32_spread(o, grad, maxit, draws=2, eps=3e-7, xt=None, dist=None, to32=True):
+def fp32_spread(o, grad, maxit, draws=2, eps=3e-7, xt=None, dist=None, to32=True, perturb_L=False):
     """How far the oracle's own PCG (the steps of or_pcg, lib/pcg.c:28-113)
     moves when each preconditioner output is rounded to FP32 with a relative
     error of eps added, the size of the rounding of an FP32 inverse FFT and
@@ -18,7 +18,9 @@ def fp32_spread(o, grad, maxit, draws=2, eps=3e-7, xt=None, dist=None, to32=True
     at 40.  The same PCG with eps = 0 reproduces the oracle to 3e-9.
     With xt and a list dist, the draws' distances to xt are appended.
     to32=False keeps the perturbed outputs in FP64 (with eps ~ 1e-15: the
-    oracle's own spread under FP64 rounding differences)."""
+    oracle's own spread under FP64 rounding differences); perturb_L also
+    perturbs each operator output A p by eps (an FP64 GPU solve differs from
+    the oracle by rounding in both)."""
     b = o.tomo_r(grad)
     xo, _ = o.solve(grad, maxit)
     worst = 0.0
@@ -39,6 +41,8 @@ def fp32_spread(o, grad, maxit, draws=2, eps=3e-7, xt=None, dist=None, to32=True
                 p = z.copy()
                 rz = rr @ z
             ap = o.tomo_l(p)
+            if perturb_L:
+                ap = ap * (1 + eps * rng.standard_normal(ap.shape))
             a = rz / (p @ ap)
             x += a * p
             rr -= a * ap

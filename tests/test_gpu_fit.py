@@ -188,3 +188,21 @@ def test_mvm_control_matrix_columns(fitsys):
         ao, _ = o.solve(xo.astype(np.float32).astype(np.float64), f.rec.maxit)
         assert rel(M[:, k], ao) < 1e-4, c
     t.close()
+
+
+def test_fit_graph_replays(fitsys):
+    """Device solves replay a graph captured on first use of their buffers:
+    the same buffers refilled with other layers between replays give each
+    input's own result, bit for bit on every replay, equal to the host call
+    (which replays a graph of its own slot)."""
+    cfg, fc, r, f, g, o = fitsys
+    xs = [layers(cfg, r, 40 + k) for k in range(2)]
+    ref = [g.solve(x, isys=0)[0] for x in xs]
+    dx = DevArray((1, r.nx), np.float32)
+    da = DevArray((1, f.nact), np.float64)
+    for rep in range(3):
+        for k in range(2):
+            dx.upload(xs[k].astype(np.float32)[None, :])
+            g.solve_dev(dx.ptr, da.ptr, nsys=1)
+            got = da.numpy()[0]
+            assert np.array_equal(got, ref[k]), (rep, k)
