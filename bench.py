@@ -770,8 +770,9 @@ def main():
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precision", type=int, default=None,
-                    help="solve precision mode (0, 1, 2; default: the library default)")
+    ap.add_argument("--precision", type=int, default=None, choices=(0, 1, 2, 3),
+                    help="solve precision mode of the timed device-buffer solves (0-3; default: the library "
+                         "default; mode 4, all FP64, serves host calls only and is timed in the precision study)")
     ap.add_argument("--modes", default="plain,graph,batched,uneven,served,mvm,fit,update,precision,sweep",
                     help="comma list of modes to time (profiling runs use one)")
     ap.add_argument("--copy-calibration", action="store_true",
