@@ -35,11 +35,12 @@ def log(msg):
     print("bench: " + msg, file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(r):
+def algorithmic_bytes(r, spec64=True):
     """Unique HBM bytes each kernel must move in one launch of an UPDATE
     iteration: (per system, shared by all systems of the launch).  Vectors:
-    x/p FP32, r/Ap/q FP64, forward spectrum complex128, inverse spectrum
-    complex64; shared: operator geometry and FDPCG blocks, read once."""
+    x/p FP32, r/Ap/q FP64, forward spectrum complex128 (complex64 with
+    spec64=False: precision mode 2), inverse spectrum complex64; shared:
+    operator geometry and FDPCG blocks, read once."""
     rec = r.rec
     nx = r.nx
     nw = sum(1 for w in range(rec.nwfs) if not rec.skip[w])
@@ -49,15 +50,16 @@ def algorithmic_bytes(r):
     nxl, nyl = rec.xnx[0], rec.xny[0]
     nspec = rec.npsr * (nxl // 2 + 1) * nyl
     bs, nbh = rec.fd_bs, rec.fd_nbhalf
+    cf = 16 if spec64 else 8  # bytes per forward-spectrum entry
     return {
         # read p, write q; GP stencils (x, y), NEA and the subaperture map
         "k_wfs": (4 * nx + 8 * nw * nploc, nsa * (2 * np2 * 4 + 16) + 4 * nw * nploc),
         # read q and p (L2), write Ap
         "k_layer": (8 * nw * nploc + 4 * nx + 8 * nx, 0),
         # read r, Ap; write r and the forward row spectrum
-        "k_row": (8 * nx + 8 * nx + 8 * nx + 16 * nspec, 0),
+        "k_row": (8 * nx + 8 * nx + 8 * nx + cf * nspec, 0),
         # read the row spectrum, write the inverse-column spectrum; blocks
-        "k_col": (16 * nspec + 8 * nspec, 8 * nbh * bs * bs),
+        "k_col": (cf * nspec + 8 * nspec, 8 * nbh * bs * bs),
         # read the spectrum, x and p; write x and p
         "k_irow": (8 * nspec + 8 * nx + 8 * nx, 0),
     }
@@ -87,6 +89,13 @@ def algorithmic_bytes_fp32(r):
         "k_col": (8 * nspec + 8 * nspec, 8 * nbh * bs * bs),
         "k_irow": (8 * nspec + 8 * nx + 8 * nx, 0),
     }
+
+
+# compute types of the solve per precision mode (gpu_tomo_precision)
+DTYPES = {0: "f32 operators/vectors/FFTs and residual recurrence, f64 reductions and alpha/beta",
+          1: "f32 operators/vectors/inverse FFT, f64 residual recurrence (r, Ap, b, q)/forward FFT/reductions",
+          2: "f32 operators/vectors/FFTs, f64 residual recurrence (r, Ap, b, q) and reductions",
+          3: "f32 everywhere (reductions too)"}
 
 
 def measured_traffic(kernel, batch):
@@ -226,16 +235,18 @@ def kernel_table(plan, ab, nb, d_g, d_x, stream, ab32=None):
     return out
 
 
-def bench_sweep(r, grads, maxit, sizes, copy_gbs):
+def bench_sweep(r, grads, maxit, sizes, copy_gbs, precision):
     """Batch sweep of the large30 solve: solves/s, p99 and per-kernel achieved
-    algorithmic bandwidth at several batch sizes (autotuned per size)."""
+    algorithmic bandwidth at several batch sizes (autotuned per size), in the
+    headline's precision mode."""
     import torch
     from maos_amd import tomo
-    ab = algorithmic_bytes(r)
+    ab = algorithmic_bytes(r, spec64=precision == 1)
     ab32 = algorithmic_bytes_fp32(r)
     out = {}
     for B in sizes:
         g = tomo.GpuTomo(r, nbatch=B, maxit=maxit)
+        g.precision(precision)
         tuned = g.autotune(B)
         G = torch.tensor(np.stack([grads[i % len(grads)] for i in range(B)]), dtype=torch.float32).cuda()
         X = torch.zeros((B, r.nx), dtype=torch.float32, device="cuda:0")
@@ -770,9 +781,11 @@ def main():
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precision", type=int, default=None, choices=(0, 1, 2, 3),
-                    help="solve precision mode of the timed device-buffer solves (0-3; default: the library "
-                         "default; mode 4, all FP64, serves host calls only and is timed in the precision study)")
+    ap.add_argument("--precision", type=int, default=2, choices=(0, 1, 2, 3),
+                    help="precision mode of the headline, plain, graph and sweep solves (default 2: FP32 "
+                         "operators, vectors and FFTs, FP64 residual recurrence and reductions, within 1e-4 of "
+                         "the FP64 oracle at 30 iterations; the library default is 1; mode 4, all FP64, serves "
+                         "host calls only and is timed in the precision study)")
     ap.add_argument("--modes", default="plain,graph,batched,uneven,served,mvm,fit,update,precision,sweep",
                     help="comma list of modes to time (profiling runs use one)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -847,9 +860,8 @@ def main():
 
     plain = tomo.GpuTomo(r, nbatch=1, maxit=maxit)
     batched = tomo.GpuTomo(r, nbatch=B, maxit=maxit)
-    if args.precision is not None:
-        plain.precision(args.precision)
-        batched.precision(args.precision)
+    plain.precision(args.precision)
+    batched.precision(args.precision)
     if args.systems_per_workgroup:
         tuned = tuple(int(v) for v in args.systems_per_workgroup.split(","))
         batched.tune_set(*tuned)
@@ -878,7 +890,7 @@ def main():
     # per-kernel profile of one batched solve (plain launches, HIP events on
     # the launch stream; the read-only kernel classes launched 8 times back
     # to back per event pair, so the average excludes the event overhead)
-    ab = algorithmic_bytes(r)
+    ab = algorithmic_bytes(r, spec64=args.precision == 1)
     ab32 = algorithmic_bytes_fp32(r)
     kernels = kernel_table(batched, ab, B, d_g.data_ptr(), d_x.data_ptr(), sp, ab32)
     torch.cuda.synchronize()
@@ -899,7 +911,7 @@ def main():
     sweep = None
     if "sweep" in modes:
         try:
-            sweep = bench_sweep(r, grads, maxit, (64, 128, 256), copy_gbs)
+            sweep = bench_sweep(r, grads, maxit, (64, 128, 256), copy_gbs, args.precision)
         except Exception as e:  # reported, not fatal for the headline line
             sweep = {"error": repr(e)}
 
@@ -972,11 +984,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 operators/FFT-inverse, f64 residual/forward-FFT/reductions",
+        "dtype": DTYPES[args.precision],
         "data": "synthetic (seeded von Karman screens, geometric SH slopes + 0.05\" noise)",
         "config": {"workload": "large30: D=30 m, 6 LGS at 90 km, 6 layers 0-16 km, 60x60 subaps, "
                                "xloc 6x96x96, 30 FDPCG-PCG iterations, cold start",
                    "batch": B, "global_batch": B, "parallelism": "batch of independent solves",
+                   "precision_mode": args.precision,
                    "autotuned_systems_per_workgroup": {"wfs": tuned[0], "layer": tuned[1], "col": tuned[2]},
                    "layer_tile_rows": tuned[3] if len(tuned) > 3 else None,
                    "column_fft": {1: "registers", 2: "lds"}.get(tuned[4]) if len(tuned) > 4 else None,
