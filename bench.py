@@ -865,8 +865,11 @@ def main():
     if args.systems_per_workgroup:
         tuned = tuple(int(v) for v in args.systems_per_workgroup.split(","))
         batched.tune_set(*tuned)
+        tuned_plain = None
     else:
         tuned = batched.autotune(B)
+        # the one-at-a-time comparator gets its own best launch shape too
+        tuned_plain = plain.autotune(1) if ("plain" in modes or "graph" in modes) else None
     empty = dict(solves_per_s=float("nan"), p50_ms=float("nan"), p99_ms=float("nan"), ms_per_step=float("nan"))
     # latency percentiles need enough calls: at least 100 per mode
     ncall = max(100, args.steps)
@@ -999,6 +1002,7 @@ def main():
         "latency_calls": ncall,
         "modes": {"plain": res_plain, "graph": res_graph, "batched": res_batch,
                   "batched_latency": res_lat},
+        "plain_launch_shape_autotuned": list(tuned_plain) if tuned_plain is not None else None,
         "speedup_vs_plain": round(res_batch["solves_per_s"] / res_plain["solves_per_s"], 2),
         "accuracy": {"rel_l2_vs_fp64_oracle": err, "cg_residual": float(res_gpu)},
         "roofline": roofline,

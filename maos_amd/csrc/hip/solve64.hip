@@ -676,8 +676,9 @@ void build64(tomo64 *t, const mt_recon_t *R, int nbatch) {
         D.lny[l] = (int)R->xny[l];
         D.loff[l + 1] = D.loff[l] + R->xnx[l] * R->xny[l];
     }
-    // one row or column of the FFT kernels and its work copy fit in 64 KB of LDS
-    if (D.precond && (D.nx > 2048 || D.ny > 2048)) fail64(MT_EINVAL, "FP64 solve: layer grid above 2048");
+    // a row or column of the FFT kernels and its work copy (32 bytes per
+    // point) stay within 32 KB of LDS
+    if (D.precond && (D.nx > 1024 || D.ny > 1024)) fail64(MT_EINVAL, "FP64 solve: layer grid above 1024");
     D.nxl = (long)D.nx * D.ny;
     D.nxtot = D.loff[D.npsr];
     D.pnx = (int)R->pnx;
