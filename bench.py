@@ -236,9 +236,10 @@ def kernel_table(plan, ab, nb, d_g, d_x, stream, ab32=None):
 
 
 def bench_sweep(r, grads, maxit, sizes, copy_gbs, precision):
-    """Batch sweep of the large30 solve: solves/s, p99 and per-kernel achieved
-    algorithmic bandwidth at several batch sizes (autotuned per size), in the
-    headline's precision mode."""
+    """Batch sweep of the large30 solve: solves/s, p50 / p99 per call (100
+    calls) and per-kernel achieved algorithmic bandwidth at several batch
+    sizes (autotuned per size), in the headline's precision mode: the
+    latency / throughput trade between one-at-a-time and large batches."""
     import torch
     from maos_amd import tomo
     ab = algorithmic_bytes(r, spec64=precision == 1)
@@ -255,7 +256,7 @@ def bench_sweep(r, grads, maxit, sizes, copy_gbs, precision):
             g.solve_dev(G.data_ptr(), X.data_ptr(), nbatch=B, mode=1, stream=st)
         torch.cuda.synchronize()
         lat = []
-        for _ in range(10):
+        for _ in range(100):
             t = time.perf_counter()
             g.solve_dev(G.data_ptr(), X.data_ptr(), nbatch=B, mode=1, stream=st)
             torch.cuda.synchronize()
@@ -264,7 +265,8 @@ def bench_sweep(r, grads, maxit, sizes, copy_gbs, precision):
         torch.cuda.synchronize()
         g.close()
         del G, X
-        out[str(B)] = dict(solves_per_s=B * len(lat) / sum(lat), p99_ms=1e3 * float(np.percentile(lat, 99)),
+        out[str(B)] = dict(solves_per_s=B * len(lat) / sum(lat), p50_ms=1e3 * float(np.percentile(lat, 50)),
+                           p99_ms=1e3 * float(np.percentile(lat, 99)), calls=len(lat),
                            systems_per_workgroup=dict(wfs=tuned[0], layer=tuned[1], col=tuned[2]),
                            layer_tile_rows=tuned[3], column_fft={1: "registers", 2: "lds"}.get(tuned[4]),
                            flights=tuned[5],
@@ -914,7 +916,7 @@ def main():
     sweep = None
     if "sweep" in modes:
         try:
-            sweep = bench_sweep(r, grads, maxit, (64, 128, 256), copy_gbs, args.precision)
+            sweep = bench_sweep(r, grads, maxit, (4, 8, 16, 64, 128, 256), copy_gbs, args.precision)
         except Exception as e:  # reported, not fatal for the headline line
             sweep = {"error": repr(e)}
 

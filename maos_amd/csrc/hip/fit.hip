@@ -138,6 +138,7 @@ struct FitDev {
     long pstride;
     // per-system workspace
     double *opd, *opd2;     // [s][nfit][nfloc]
+    double *tnw;            // [s][FMAXNW]: NW' p of the current p
     double *r, *p, *Ap, *b; // [s][nact]
     double *part;           // [s][pstride]
     int *stop;              // [s]: 1 once a step size was not finite
@@ -238,12 +239,20 @@ __global__ __launch_bounds__(FB) void kf_fwd(const FitDev *__restrict__ Dp, cons
 }
 
 // opd2_f = wt_f (W0 opd_f - W1 (W1' opd_f)) (applyW, maos/recon_utils.c:155-180).
-__global__ __launch_bounds__(FB) void kf_w(const FitDev *__restrict__ Dp, int ws0) {
+// With nwdots, the first workgroup of each system also sums the partials of
+// NW' p once for the kf_hat that follows (instead of every kf_hat workgroup:
+// those sums were a quarter of its time).
+__global__ __launch_bounds__(FB) void kf_w(const FitDev *__restrict__ Dp, int ws0, int nwdots) {
     const FitDev &D = *Dp;
     __shared__ double psh[4];
     const int f = blockIdx.y, s = ws0 + blockIdx.z;
     const long nf = D.nfloc;
     const long i = (long)blockIdx.x * FB + threadIdx.x;
+    if (nwdots && blockIdx.x == 0 && blockIdx.y == 0)
+        for (int k = 0; k < D.nnw; k++) {
+            const double v = psum(fpart(D, s) + D.o3 + k * D.nc4, D.nc4, psh);
+            if (threadIdx.x == 0) D.tnw[(long)s * FMAXNW + k] = v;
+        }
     const double dot = psum(fpart(D, s) + D.o1 + f * D.nc1, D.nc1, psh);
     if (i >= nf) return;
     const int fnx = D.fnx, fny = D.fny;
@@ -277,18 +286,14 @@ __global__ __launch_bounds__(FB) void kf_hat(const FitDev *__restrict__ Dp, cons
     const FitDev &D = *Dp;
     __shared__ double red[FB];
     __shared__ double tnw[FMAXNW];
-    __shared__ double wsum[4], psh[4];
+    __shared__ double wsum[4];
     const int s = ws0 + blockIdx.y;
     const int nfit = D.nfit, t = threadIdx.x;
     const int f = t / D.hab, al = t - f * D.hab;
     const long na = D.nact, nf = D.nfloc;
     const long a = (long)blockIdx.x * D.hab + al;
     const bool act = f < nfit && a < na;
-    if (MODE == HAT_L)
-        for (int k = 0; k < D.nnw; k++) {
-            const double v = psum(fpart(D, s) + D.o3 + k * D.nc4, D.nc4, psh);
-            if (t == 0) tnw[k] = v;
-        }
+    if (MODE == HAT_L && t < D.nnw) tnw[t] = D.tnw[(long)s * FMAXNW + t];
     double acc = 0;
     if (act) {
         int d = 0;
@@ -499,7 +504,7 @@ struct gpu_fit {
     DBuf<TapT> txe, tye;
     DBuf<double> W0, W1, NW, slx;
     DBuf<long> slp, sli;
-    DBuf<double> opd, opd2, r, p, Ap, b, part, res, x;
+    DBuf<double> opd, opd2, tnw, r, p, Ap, b, part, res, x;
     DBuf<int> stop;
     DBuf<float> xin;
     std::vector<float> hstage;
@@ -663,6 +668,8 @@ void build_fit_plan(gpu_fit *g, const mt_recon_t *R, const mt_fit_recon_t *F) {
     g->p.alloc(ns * D.nact);
     g->Ap.alloc(ns * D.nact);
     g->b.alloc(ns * D.nact);
+    g->tnw.alloc(ns * FMAXNW);
+    D.tnw = g->tnw.p;
     D.nc1 = (int)((D.nfloc + FB - 1) / FB);
     D.hab = FB / D.nfit;
     D.nc3 = (int)((D.nact + D.hab - 1) / D.hab);
@@ -715,7 +722,7 @@ void launch_fwd(const FitDev &D, int nsrc, dim3 grid, hipStream_t st, const FitD
 void enq_rhs(gpu_fit *g, hipStream_t st, const float *xin, long xstride, double *out, int ws0, int n) {
     const FitDev &D = g->H;
     launch_fwd<1>(D, D.npsr, grid_pts(D, n), st, g->dev.p, (const void *)xin, xstride, ws0);
-    hipLaunchKernelGGL(kf_w, grid_pts(D, n), dim3(FB), 0, st, g->dev.p, ws0);
+    hipLaunchKernelGGL(kf_w, grid_pts(D, n), dim3(FB), 0, st, g->dev.p, ws0, 0);
     hipLaunchKernelGGL(kf_hat<HAT_R>, grid_hat(D, n), dim3(FB), 0, st, g->dev.p, (const double *)nullptr, out, ws0);
 }
 
@@ -724,7 +731,7 @@ void enq_rhs(gpu_fit *g, hipStream_t st, const float *xin, long xstride, double 
 void enq_L(gpu_fit *g, hipStream_t st, const double *p, double *out, int ws0, int n) {
     const FitDev &D = g->H;
     launch_fwd<0>(D, D.ndm, grid_pts(D, n), st, g->dev.p, (const void *)p, D.nact, ws0);
-    hipLaunchKernelGGL(kf_w, grid_pts(D, n), dim3(FB), 0, st, g->dev.p, ws0);
+    hipLaunchKernelGGL(kf_w, grid_pts(D, n), dim3(FB), 0, st, g->dev.p, ws0, 1);
     hipLaunchKernelGGL(kf_hat<HAT_L>, grid_hat(D, n), dim3(FB), 0, st, g->dev.p, p, out, ws0);
 }
 

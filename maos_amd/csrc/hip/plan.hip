@@ -2570,7 +2570,7 @@ int gpu_tomo_precision(gpu_tomo_t *g, int mode) {
         }
         if (g->t64) {
             // back to an FP32-operator mode: the FP64 warm-start state, rounded
-            const int rc = t64_store_x(g->t64, g->x.p, g->stream);
+            const int rc = t64_store_x(g->t64, g->x.p, g->res.p, g->stream);
             t64_free(g->t64);
             g->t64 = nullptr;
             chk64(rc);

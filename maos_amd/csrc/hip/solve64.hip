@@ -1146,11 +1146,13 @@ int t64_load_x(tomo64 *t, const float *d_x, hipStream_t st) {
     });
 }
 
-int t64_store_x(tomo64 *t, float *d_x, hipStream_t st) {
+int t64_store_x(tomo64 *t, float *d_x, double *d_res, hipStream_t st) {
     if (!t || !d_x) return MT_EINVAL;
     return g64([&] {
         hipLaunchKernelGGL(k64_cvt_out, dim3(1024), dim3(256), 0, st, d_x, t->x.p, (long)t->nbatch * t->H.nxtot);
         C64(hipGetLastError());
+        if (d_res)
+            C64(hipMemcpyAsync(d_res, t->res.p, sizeof(double) * t->nbatch, hipMemcpyDeviceToDevice, st));
         C64(hipStreamSynchronize(st));
     });
 }

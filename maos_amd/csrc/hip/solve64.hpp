@@ -24,9 +24,10 @@ void t64_free(tomo64 *t);
 // New L2 coefficients and FDPCG blocks (Cn2 update), same geometry.
 int t64_update_cn2(tomo64 *t, const mt_recon_t *recon);
 // Warm-start state: the nbatch FP32 solutions d_x (stride nxtot) into the
-// FP64 state, or the FP64 state back into d_x (rounded).
+// FP64 state, or the FP64 state back into d_x (rounded) and the last
+// residuals into d_res (nbatch doubles, may be null).
 int t64_load_x(tomo64 *t, const float *d_x, hipStream_t st);
-int t64_store_x(tomo64 *t, float *d_x, hipStream_t st);
+int t64_store_x(tomo64 *t, float *d_x, double *d_res, hipStream_t st);
 // Zero the warm-start state.
 int t64_reset(tomo64 *t);
 // Solve systems [s0, s0 + nsys) from host gradients (caller layout: all WFS,

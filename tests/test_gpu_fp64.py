@@ -123,9 +123,11 @@ def test_warm64_and_mode_switch():
         assert e <= 1e-9
         assert abs(resg - reso) <= 1e-6 * reso
     g.precision(1)
-    # the FP32 plan resumes from the FP64 solution rounded to FP32
-    x_after, _ = g.get_x(0)
+    # the FP32 plan resumes from the FP64 solution rounded to FP32, and
+    # reports the FP64 plan's last residual
+    x_after, res_after = g.get_x(0)
     assert np.array_equal(x_after, xg.astype(np.float32).astype(np.float64))
+    assert res_after == resg
     xg3, _, _ = g.tomo(grads[3])
     xo3, _ = o.solve(grads[3], 10, x0=x_after)
     assert rel(xg3, xo3) <= 1e-4
