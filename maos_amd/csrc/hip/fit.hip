@@ -270,10 +270,13 @@ __global__ __launch_bounds__(FB) void kf_w(const FitDev *__restrict__ Dp, int ws
 }
 
 enum { HAT_R = 0, HAT_L = 1 };
+constexpr int HAH = 2;  // actuator groups per kf_hat workgroup (each thread gathers for HAH actuators)
 
 // out[a] = sum_f HA_f' opd2_f at actuator a; HAT_L adds NW (NW' p) + S p +
 // tikdiag p and writes the workgroup's partial of p . out.  A workgroup holds
-// hab actuators x nfit directions, one thread per (actuator, direction),
+// HAH groups of hab actuators x nfit directions, one thread per (actuator of
+// each group, direction) (two actuators per thread halve the workgroups and
+// their fixed reduction and barrier cost),
 // gathering through the transposed per-axis lists (no atomics); the
 // directions of an actuator are summed in index order.  Actuators are the
 // fast index of the threads, so a wave's gathers of one direction read
@@ -284,52 +287,58 @@ template <int MODE>
 __global__ __launch_bounds__(FB) void kf_hat(const FitDev *__restrict__ Dp, const double *__restrict__ p,
                                               double *__restrict__ out, int ws0) {
     const FitDev &D = *Dp;
-    __shared__ double red[FB];
+    __shared__ double red[HAH * FB];
     __shared__ double tnw[FMAXNW];
     __shared__ double wsum[4];
     const int s = ws0 + blockIdx.y;
-    const int nfit = D.nfit, t = threadIdx.x;
-    const int f = t / D.hab, al = t - f * D.hab;
+    const int nfit = D.nfit, t = threadIdx.x, hab = D.hab;
+    const int f = t / hab, al = t - f * hab;
     const long na = D.nact, nf = D.nfloc;
-    const long a = (long)blockIdx.x * D.hab + al;
-    const bool act = f < nfit && a < na;
+    const long a0 = (long)blockIdx.x * HAH * hab;  // first actuator of the workgroup
     if (MODE == HAT_L && t < D.nnw) tnw[t] = D.tnw[(long)s * FMAXNW + t];
-    double acc = 0;
-    if (act) {
-        int d = 0;
-        while (a >= D.aoff[d + 1]) d++;
-        const int k = (int)(a - D.aoff[d]);
-        const int ja = k / D.anx[d], ia = k - ja * D.anx[d];
-        const int bx = f * D.tpx + D.tdx[d] + ia, by = f * D.tpy + D.tdy[d] + ja;
-        const int x0 = D.txp[bx], x1 = D.txp[bx + 1];
-        const int y0 = D.typ[by], y1 = D.typ[by + 1];
-        const double *o = D.opd2 + ((long)s * nfit + f) * nf;
-        const int fnx = D.fnx;
-        for (int ey = y0; ey < y1; ey++) {
-            const TapT ty = D.tye[ey];
-            for (int ex = x0; ex < x1; ex++) {
-                const TapT tx = D.txe[ex];
-                const double w = tx.w * ty.w;
-                if (w > FEPS) acc += w * o[(long)ty.i * fnx + tx.i];
+    const double *o = D.opd2 + ((long)s * nfit + f) * nf;
+    const int fnx = D.fnx;
+#pragma unroll
+    for (int j = 0; j < HAH; j++) {
+        const long a = a0 + j * hab + al;
+        double acc = 0;
+        if (f < nfit && a < na) {
+            int d = 0;
+            while (a >= D.aoff[d + 1]) d++;
+            const int k = (int)(a - D.aoff[d]);
+            const int ja = k / D.anx[d], ia = k - ja * D.anx[d];
+            const int bx = f * D.tpx + D.tdx[d] + ia, by = f * D.tpy + D.tdy[d] + ja;
+            const int x0 = D.txp[bx], x1 = D.txp[bx + 1];
+            const int y0 = D.typ[by], y1 = D.typ[by + 1];
+            for (int ey = y0; ey < y1; ey++) {
+                const TapT ty = D.tye[ey];
+                for (int ex = x0; ex < x1; ex++) {
+                    const TapT tx = D.txe[ex];
+                    const double w = tx.w * ty.w;
+                    if (w > FEPS) acc += w * o[(long)ty.i * fnx + tx.i];
+                }
             }
         }
+        red[j * FB + t] = acc;
     }
-    red[t] = acc;
     __syncthreads();
     double pap = 0;
-    if (act && f == 0) {
+    for (int e = t; e < HAH * hab; e += FB) {
+        const int j = e / hab, el = e - j * hab;
+        const long a = a0 + e;
+        if (a >= na) break;
         double v = 0;
-        for (int j = 0; j < nfit; j++) v += red[j * D.hab + al];
+        for (int q = 0; q < nfit; q++) v += red[j * FB + q * hab + el];
         if (MODE == HAT_L) {
             const double *ps = p + (long)blockIdx.y * na;
             for (int q = 0; q < D.nnw; q++) v += D.NW[(long)q * na + a] * tnw[q];
             if (D.slp) {
                 double sl = 0;
-                for (long e = D.slp[a]; e < D.slp[a + 1]; e++) sl += D.slx[e] * ps[D.sli[e]];
+                for (long e2 = D.slp[a]; e2 < D.slp[a + 1]; e2++) sl += D.slx[e2] * ps[D.sli[e2]];
                 v += sl;
             }
             v += D.tikdiag * ps[a];
-            pap = ps[a] * v;
+            pap += ps[a] * v;
         }
         out[(long)blockIdx.y * na + a] = v;
     }
@@ -672,7 +681,7 @@ void build_fit_plan(gpu_fit *g, const mt_recon_t *R, const mt_fit_recon_t *F) {
     D.tnw = g->tnw.p;
     D.nc1 = (int)((D.nfloc + FB - 1) / FB);
     D.hab = FB / D.nfit;
-    D.nc3 = (int)((D.nact + D.hab - 1) / D.hab);
+    D.nc3 = (int)((D.nact + (long)HAH * D.hab - 1) / ((long)HAH * D.hab));
     D.nc4 = (int)((D.nact + FB - 1) / FB);
     D.o1 = 0;
     D.o2 = D.o1 + D.nfit * D.nc1;
