@@ -8,6 +8,8 @@ independent library routine:
                  coordinates, linear-ramp closed form, adjoint identity
   GP             full-subaperture closed forms (pos 1 and 2), exact slope of a
                  plane wave on every subaperture (partial ones included)
+  saloc / saa    covered fraction of each subaperture square by the annular
+                 pupil (exact chord integral), and the kept set by saat
   L2'L2          Fourier eigenvalue cf^2 (1 - cos/2 - cos/2)^2
   TomoL / TomoR  bilinear-form and adjoint identities built from the pinned HX,
                  GP, W, PTT and L2 pieces
@@ -239,6 +241,51 @@ def test_gp_plane_wave_every_subaperture(case, request):
         assert np.abs(g[:nsa] - a).max() < 1e-9 * abs(a)
         assert np.abs(g[nsa:] - b).max() < 1e-9 * abs(a)
         assert np.abs(G @ np.ones(rec.pnx * rec.pny)).max() < 1e-9
+
+
+def _annulus_square_fraction(x0, y0, d, R, Rin, m=4000):
+    """Exact covered fraction of the squares [x0, x0+d] x [y0, y0+d] by the
+    annulus Rin <= r <= R about the origin: the chord length of each disk in
+    y, integrated over x by the midpoint rule (m points)."""
+    t = (np.arange(m) + 0.5) / m
+
+    def disk(rad):
+        x = x0[:, None] + d * t[None, :]
+        s = np.sqrt(np.clip(rad * rad - x * x, 0, None))
+        lo, hi = np.maximum(y0[:, None], -s), np.minimum(y0[:, None] + d, s)
+        return np.clip(hi - lo, 0, None).sum(axis=1) * d / m
+
+    return (disk(R) - (disk(Rin) if Rin > 0 else 0)) / (d * d)
+
+
+@pytest.mark.parametrize("case", ["small8", "large30"])
+def test_subapertures_against_pupil_area(case, request):
+    """saloc and saa (maos/setup_powfs.c:92-404) against the geometry they
+    sample: each subaperture's saa is the covered fraction of its square by
+    the annular pupil, to 0.01 (the reference's anti-aliased rim pixels enter
+    squared: 0.006 and 0.007 here); squares wholly inside have saa exactly 1;
+    the subapertures kept are those of the order x order grid covering more
+    than saat.  Squares moved by half a fine pixel (dsa / 64) already miss the
+    0.01 bound, so a misplaced grid fails this."""
+    cfg, r = request.getfixturevalue(case)
+    pw = cfg["powfs"][0]
+    D, Din, dsa, saat = cfg["aper_d"], cfg["aper_din"], pw["dsa"], pw["saat"]
+    for w in range(r.nwfs):
+        x, y = r.saloc(w)
+        saa = r.saa(w)
+        ex = _annulus_square_fraction(x, y, dsa, D / 2, Din / 2)
+        assert np.abs(saa - ex).max() < 0.01
+        assert np.array_equal(saa == 1, ex > 1 - 1e-12)
+        moved = _annulus_square_fraction(x + dsa / 64, y, dsa, D / 2, Din / 2)
+        assert np.abs(saa - moved).max() > 0.01
+    order = int(math.ceil(D / dsa))
+    ii = np.arange(-(order // 2), (order - 1) // 2 + 1) + (-0.5 if order & 1 else 0.0)
+    gx, gy = (a.ravel() * dsa for a in np.meshgrid(ii, ii))
+    frac = _annulus_square_fraction(gx, gy, dsa, D / 2, Din / 2)
+    assert np.abs(frac - saat).min() > 0.01  # no square at the threshold
+    want = {(round(2 * a / dsa), round(2 * b / dsa)) for a, b, f in zip(gx, gy, frac) if f > saat}
+    x, y = r.saloc(0)
+    assert {(round(2 * a / dsa), round(2 * b / dsa)) for a, b in zip(x, y)} == want
 
 
 # ------------------------------------------------------------- L2, W, PTT
