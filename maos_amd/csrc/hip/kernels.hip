@@ -75,6 +75,8 @@ __device__ __forceinline__ MRay mray(CG &G, int i) {
     m.wy = r.wy;
     m.fx = r.fx;
     m.fy = r.fy;
+    m.nrow = r.nrow;
+    m.qoff = r.qoff;
     return m;
 }
 __device__ __forceinline__ FDiv ldf(const CONSTAS FDiv &f) { return FDiv{f.d, f.m}; }
@@ -847,10 +849,15 @@ __global__ __launch_bounds__(256) void k_wfs_m(KARGS, int xsel) {
 #pragma unroll
         for (int si = 0; si < CS; si++) v[r][si] = 0;
     if (item) {
+        const float *xs = x + (long)s0 * G.nxtot;
         for (int l = 0; l < G.npsr; l++) {
             const MRay m = mray(G, w * MAXL + l);
             const int nxl = G.xnx[l], nyl = G.xny[l];
-            const float *xl = x + (long)s0 * G.nxtot + G.xoff[l];
+            const long xo = G.xoff[l];
+            // every scalar of this layer in one round trip (the compiler would
+            // otherwise load those used only past a branch separately)
+            asm volatile("" ::"s"(m.wx), "s"(m.wy), "s"(nyl), "s"(xo));
+            const float *xl = xs + xo;
             const RT fx = m.fx, fy = m.fy;
             const bool fast = tx0 - POS >= m.ax0 && tx0 + TILE + POS <= (m.wx ? m.ax1 - 1 : m.ax1) &&
                               ty0 - POS >= m.ay0 && ty0 + TILE + POS <= (m.wy ? m.ay1 - 1 : m.ay1);
@@ -1156,10 +1163,15 @@ __global__ __launch_bounds__(128) void k_layer_m8(KARGS, int psel_, int osel, in
     // reads 0 with no test per row; an invalid ray column starts from BOOB,
     // past the end for every row.
     if (nr > 0 && nc > 0) {
-        const int pnx = G.pnx;
-        const unsigned rstep = (unsigned)pnx * (unsigned)sizeof(RT);
+        const unsigned rstep = (unsigned)G.pnx * (unsigned)sizeof(RT);
+        const RT *qs[CS];  // the WFS fields of each system
+#pragma unroll
+        for (int si = 0; si < CS; si++) qs[si] = q + (long)(s0 + min(si, ns - 1)) * G.nwfs * G.nploc;
         for (int w = 0; w < G.nwfs; w++) {
             const MRay m = mray(G, w * MAXL + l);
+            // the whole record in one scalar round trip (the compiler would
+            // otherwise load the fields used after the tile test separately)
+            asm volatile("" ::"s"(m.fx), "s"(m.fy), "s"(m.nrow), "s"(m.qoff));
             const bool edgex = m.wx && jx0 == 0, edgey = m.wy && jy0 == 0;
             if (!edgex && (jx0 + tw - 1 - m.cx < m.ax0 || jx0 - m.cx - 1 >= m.ax1)) continue;
             if (!edgey && (jy0 + th - 1 - m.cy < m.ay0 || jy0 - m.cy - 1 >= m.ay1)) continue;
@@ -1170,17 +1182,16 @@ __global__ __launch_bounds__(128) void k_layer_m8(KARGS, int psel_, int osel, in
             const unsigned ca = va ? (unsigned)ixa * (unsigned)sizeof(RT) : BOOB;
             const unsigned cb = vb ? (unsigned)ixb * (unsigned)sizeof(RT) : BOOB;
             const unsigned cc = vc ? (unsigned)ixc * (unsigned)sizeof(RT) : BOOB;
-            const unsigned rw = (m.wy && jyb == 0) ? (unsigned)(m.ay1 - 1 - m.ay0) * rstep
-                                                   : (unsigned)(jyb - m.cy - 1 - m.ay0) * rstep;
+            const int iyw = (m.wy && jyb == 0) ? m.ay1 - 1 : jyb - m.cy - 1;  // sample row 0
             const unsigned r1 = (unsigned)(jyb - m.cy - m.ay0) * rstep;
             unsigned ro[R + 1];
-            ro[0] = rw;
+            ro[0] = (unsigned)(iyw - m.ay0) * rstep;
 #pragma unroll
             for (int k = 1; k <= R; k++) ro[k] = r1 + (unsigned)(k - 1) * rstep;
-            const unsigned nrec = (unsigned)max(m.ay1 - m.ay0, 0) * rstep;
+            const unsigned nrec = (unsigned)m.nrow * rstep;
 #pragma unroll
             for (int si = 0; si < CS; si++) {
-                const auto rs = brsrc(q + ((long)(s0 + min(si, ns - 1)) * G.nwfs + w) * G.nploc + (long)m.ay0 * pnx, nrec);
+                const auto rs = brsrc(qs[si] + m.qoff, nrec);
                 RT h0[R + 1], h1[R + 1];
 #pragma unroll
                 for (int k = 0; k <= R; k++) {

@@ -33,10 +33,14 @@ constexpr int MAXN = 1024;    // largest FFT length (layer grid edge)
 // Matched-sampling ray of one (WFS, layer) pair: ploc index i maps to layer
 // nodes i + c and i + c + 1 with a constant fraction f, for i in [a0, a1)
 // (per axis).  wrap: the last ray's second node is node 0 (edge case).
+// qoff = w * nploc + ay0 * pnx: where the rows [ay0, ay1) of this WFS's
+// field start within a system's WFS fields; nrow = ay1 - ay0 (0 if none).
 struct MRay {
     int cx, cy, ax0, ax1, ay0, ay1;
     int wx, wy;
     float fx, fy;
+    int nrow, pad_;
+    long qoff;
 };
 
 // Division by a runtime constant d >= 1 as a multiply-high: m = ceil(2^32/d)

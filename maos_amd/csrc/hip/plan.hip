@@ -589,6 +589,9 @@ void build_plan(gpu_tomo *g, const mt_recon_t *R) {
             if (!simple_axis(hnx, hfx, wl * pnx, pnx, R->xnx[l], m.cx, m.fx, m.ax0, m.ax1, m.wx) ||
                 !simple_axis(hny, hfy, wl * pny, pny, R->xny[l], m.cy, m.fy, m.ay0, m.ay1, m.wy))
                 matched = 0;
+            m.nrow = std::max(m.ay1 - m.ay0, 0);
+            m.pad_ = 0;
+            m.qoff = (long)k * pnx * pny + (long)m.ay0 * pnx;
         }
     // the matched kernels wrap the L2 window once: layers of at least TILE + 2
     // (16 x 16 tiles), LTY2 + 2 (16 x 32) or LTY + 2 rows (16 x 64 tiles)
