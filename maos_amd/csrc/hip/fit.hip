@@ -526,6 +526,7 @@ struct gpu_fit {
         int ws0, n, warm, maxit;
         hipGraph_t graph;
         hipGraphExec_t exec;
+        hipEvent_t done;  // recorded after each launch
     };
     std::vector<Graph> graphs;  // most recently used last
     hipStream_t cap = nullptr;  // capture stream
@@ -772,11 +773,18 @@ void enq_solve(gpu_fit *g, hipStream_t st, const float *xin, long xstride, doubl
 
 constexpr size_t FIT_GRAPHS = 8;  // captured solves kept per plan
 
-void drop_graphs(gpu_fit *g) {
-    for (auto &e : g->graphs) {
-        (void)hipGraphExecDestroy(e.exec);
-        (void)hipGraphDestroy(e.graph);
+// Destroy a captured solve once its last launch (which an asynchronous
+// caller may not have waited for) has completed.
+void drop_graph(gpu_fit::Graph &e) {
+    if (e.done) {
+        (void)hipEventSynchronize(e.done);
+        (void)hipEventDestroy(e.done);
     }
+    if (e.exec) (void)hipGraphExecDestroy(e.exec);
+    if (e.graph) (void)hipGraphDestroy(e.graph);
+}
+void drop_graphs(gpu_fit *g) {
+    for (auto &e : g->graphs) drop_graph(e);
     g->graphs.clear();
 }
 
@@ -799,7 +807,7 @@ void run_solve(gpu_fit *g, hipStream_t st, const float *xin, long xstride, doubl
     }
     if (k == g->graphs.size()) {
         if (!g->cap) FCHK(hipStreamCreateWithFlags(&g->cap, hipStreamNonBlocking));
-        gpu_fit::Graph e{xin, xstride, x, ws0, n, warm, maxit, nullptr, nullptr};
+        gpu_fit::Graph e{xin, xstride, x, ws0, n, warm, maxit, nullptr, nullptr, nullptr};
         FCHK(hipStreamBeginCapture(g->cap, hipStreamCaptureModeThreadLocal));
         try {
             enq_solve(g, g->cap, xin, xstride, x, ws0, n, warm, maxit);
@@ -810,14 +818,14 @@ void run_solve(gpu_fit *g, hipStream_t st, const float *xin, long xstride, doubl
             throw;
         }
         FCHK(hipStreamEndCapture(g->cap, &e.graph));
-        const hipError_t ie = hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0);
+        hipError_t ie = hipGraphInstantiate(&e.exec, e.graph, nullptr, nullptr, 0);
+        if (ie == hipSuccess) ie = hipEventCreateWithFlags(&e.done, hipEventDisableTiming);
         if (ie != hipSuccess) {
-            (void)hipGraphDestroy(e.graph);
+            drop_graph(e);
             FCHK(ie);
         }
         if (g->graphs.size() >= FIT_GRAPHS) {
-            (void)hipGraphExecDestroy(g->graphs.front().exec);
-            (void)hipGraphDestroy(g->graphs.front().graph);
+            drop_graph(g->graphs.front());
             g->graphs.erase(g->graphs.begin());
         }
         g->graphs.push_back(e);
@@ -827,6 +835,7 @@ void run_solve(gpu_fit *g, hipStream_t st, const float *xin, long xstride, doubl
         k = g->graphs.size() - 1;
     }
     FCHK(hipGraphLaunch(g->graphs[k].exec, st));
+    FCHK(hipEventRecord(g->graphs[k].done, st));
 }
 
 void stage_x(gpu_fit *g, int isys, const double *opdr) {

@@ -192,6 +192,7 @@ struct gpu_tomo {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_nsys = 0;
+    hipEvent_t gdone = nullptr;  // recorded after each launch of gexec
     // flights: a graph solve of n systems as k graphs over consecutive
     // system ranges, each on its own stream, forked from and joined to the
     // caller's stream (the kernels of one flight fill the ramps and tails of
@@ -1324,20 +1325,34 @@ Batch &own_batch(gpu_tomo *g, int s0, int n) {
     g->own[key] = std::move(B);
     return ref;
 }
-void drop_batches(gpu_tomo *g) {
-    g->gen++;
-    g->own.clear();
-    if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+// Destroy the plan's solve graph / flight graphs once their last launches
+// (which an asynchronous caller may not have waited for) have completed.
+void drop_graph(gpu_tomo *g) {
+    if (g->gexec) {
+        if (g->gdone) (void)hipEventSynchronize(g->gdone);
+        (void)hipGraphExecDestroy(g->gexec);
+    }
     if (g->graph) (void)hipGraphDestroy(g->graph);
     g->gexec = nullptr;
     g->graph = nullptr;
+}
+void drop_flights(gpu_tomo *g) {
     for (auto &f : g->fl) {
-        if (f.exec) (void)hipGraphExecDestroy(f.exec);
+        if (f.exec) {
+            if (f.done) (void)hipEventSynchronize(f.done);
+            (void)hipGraphExecDestroy(f.exec);
+        }
         if (f.graph) (void)hipGraphDestroy(f.graph);
         f.exec = nullptr;
         f.graph = nullptr;
     }
     g->fl_nsys = g->fl_k = 0;
+}
+void drop_batches(gpu_tomo *g) {
+    g->gen++;
+    drop_graph(g);
+    drop_flights(g);
+    g->own.clear();
 }
 
 LaunchTab tab(const Batch &B, const Sub &s) { return LaunchTab{B.jobs_over ? B.jobs_over : B.jobs.p, s.wgi.p, s.nwg}; }
@@ -1724,6 +1739,7 @@ void gpu_tomo_free(gpu_tomo_t *g) {
         if (f.done) (void)hipEventDestroy(f.done);
     }
     if (g->fork) (void)hipEventDestroy(g->fork);
+    if (g->gdone) (void)hipEventDestroy(g->gdone);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     t64_free(g->t64);
     t64_keep_free(g->rc64);
@@ -1831,26 +1847,19 @@ void graph_solve(gpu_tomo *g, int n, hipStream_t st) {
     const int k = flights_for(g, n);
     if (k == 1) {
         if (!g->gexec || g->graph_nsys != n) {
-            if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
-            if (g->graph) (void)hipGraphDestroy(g->graph);
-            g->gexec = nullptr;
-            g->graph = nullptr;
+            drop_graph(g);
             capture_solve(own_batch(g, 0, n), &g->graph, &g->gexec, &g->evs);
             g->graph_nsys = n;
         }
+        if (!g->gdone) HIPCHK(hipEventCreateWithFlags(&g->gdone, hipEventDisableTiming));
         HIPCHK(hipGraphLaunch(g->gexec, st));
+        HIPCHK(hipEventRecord(g->gdone, st));
         return;
     }
     if (!g->fork) HIPCHK(hipEventCreateWithFlags(&g->fork, hipEventDisableTiming));
     if ((int)g->fl.size() < k) g->fl.resize(k);
     if (g->fl_nsys != n || g->fl_k != k) {
-        for (auto &f : g->fl) {
-            if (f.exec) (void)hipGraphExecDestroy(f.exec);
-            if (f.graph) (void)hipGraphDestroy(f.graph);
-            f.exec = nullptr;
-            f.graph = nullptr;
-        }
-        g->fl_nsys = g->fl_k = 0;
+        drop_flights(g);
         // the flights' streams first, back to back (no other stream created
         // or destroyed between them, as the captures below do), so that the
         // runtime's least-used choice gives them different hardware queues
@@ -2237,9 +2246,23 @@ struct gpu_tomo_group {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;  // recorded after each solve (of B, by graph or launches)
     int device = 0;
     EvList evs;
 };
+namespace {
+// Replace the group's batch (and drop its graph) once its last solve, which
+// an asynchronous caller may not have waited for, has completed.
+void group_rebuild(gpu_tomo_group *grp) {
+    if (grp->done) HIPCHK(hipEventSynchronize(grp->done));
+    if (grp->gexec) (void)hipGraphExecDestroy(grp->gexec);
+    if (grp->graph) (void)hipGraphDestroy(grp->graph);
+    grp->gexec = nullptr;
+    grp->graph = nullptr;
+    grp->B = make_batch(grp->plans, grp->s0, grp->nsys);
+    for (size_t j = 0; j < grp->plans.size(); j++) grp->gens[j] = grp->plans[j]->gen;
+}
+}  // namespace
 
 extern "C" {
 
@@ -2277,13 +2300,7 @@ static void group_refresh(gpu_tomo_group *grp) {
     bool moved = false;
     for (size_t j = 0; j < grp->plans.size(); j++) moved |= grp->plans[j]->gen != grp->gens[j];
     if (!moved) return;
-    HIPCHK(hipStreamSynchronize(grp->stream));
-    if (grp->gexec) (void)hipGraphExecDestroy(grp->gexec);
-    if (grp->graph) (void)hipGraphDestroy(grp->graph);
-    grp->gexec = nullptr;
-    grp->graph = nullptr;
-    grp->B = make_batch(grp->plans, grp->s0, grp->nsys);
-    for (size_t j = 0; j < grp->plans.size(); j++) grp->gens[j] = grp->plans[j]->gen;
+    group_rebuild(grp);
 }
 
 int gpu_tomo_group_solve(gpu_tomo_group_t *grp, int mode, void *stream) {
@@ -2292,6 +2309,7 @@ int gpu_tomo_group_solve(gpu_tomo_group_t *grp, int mode, void *stream) {
         HIPCHK(hipSetDevice(grp->device));
         group_refresh(grp);
         hipStream_t st = stream ? (hipStream_t)stream : grp->stream;
+        if (!grp->done) HIPCHK(hipEventCreateWithFlags(&grp->done, hipEventDisableTiming));
         if (mode == 1) {
             if (!grp->gexec) capture_solve(*grp->B, &grp->graph, &grp->gexec, &grp->evs);
             HIPCHK(hipGraphLaunch(grp->gexec, st));
@@ -2299,6 +2317,7 @@ int gpu_tomo_group_solve(gpu_tomo_group_t *grp, int mode, void *stream) {
             Rec rec{&grp->evs, st, false};
             enqueue_solve(*grp->B, rec);
         }
+        HIPCHK(hipEventRecord(grp->done, st));
         if (!stream) HIPCHK(hipStreamSynchronize(st));
     });
 }
@@ -2318,12 +2337,7 @@ void group_apply(gpu_tomo_group *grp, const int *c) {
         apply_lty(g);
         noalloc_init(g);
     }
-    if (grp->gexec) (void)hipGraphExecDestroy(grp->gexec);
-    if (grp->graph) (void)hipGraphDestroy(grp->graph);
-    grp->gexec = nullptr;
-    grp->graph = nullptr;
-    grp->B = make_batch(grp->plans, grp->s0, grp->nsys);
-    for (size_t j = 0; j < grp->plans.size(); j++) grp->gens[j] = grp->plans[j]->gen;
+    group_rebuild(grp);
 }
 }  // namespace
 
@@ -2386,6 +2400,7 @@ void gpu_tomo_group_free(gpu_tomo_group_t *grp) {
     (void)hipDeviceSynchronize();
     if (grp->gexec) (void)hipGraphExecDestroy(grp->gexec);
     if (grp->graph) (void)hipGraphDestroy(grp->graph);
+    if (grp->done) (void)hipEventDestroy(grp->done);
     if (grp->stream) (void)hipStreamDestroy(grp->stream);
     delete grp;
 }

@@ -206,3 +206,25 @@ def test_fit_graph_replays(fitsys):
             g.solve_dev(dx.ptr, da.ptr, nsys=1)
             got = da.numpy()[0]
             assert np.array_equal(got, ref[k]), (rep, k)
+
+
+def test_fit_graph_cache_eviction_in_flight(fitsys):
+    """Device solves into ten output buffers queued back to back with no
+    wait: the ninth and tenth capture new graphs and evict the oldest of the
+    eight kept, whose launches may still be queued; the plan waits for an
+    evicted graph's last launch before destroying it, so every buffer holds
+    the host call's result bit for bit."""
+    cfg, fc, r, f, g, o = fitsys
+    x = layers(cfg, r, 45)
+    ref = g.solve(x, isys=0)[0]
+    dx = DevArray.of(x.astype(np.float32)[None, :])
+    outs = [DevArray.of(np.zeros((1, f.nact))) for _ in range(10)]
+    for rep in range(2):
+        for da in outs:
+            g.solve_dev(dx.ptr, da.ptr, nsys=1)
+        sync()
+        for k, da in enumerate(outs):
+            assert np.array_equal(da.numpy()[0], ref), (rep, k)
+    for da in outs:
+        da.free()
+    dx.free()

@@ -43,6 +43,35 @@ def test_systems_per_workgroup_bit_exact(large30, mode, colk, rows):
     g.close()
 
 
+def test_graph_resized_while_in_flight(large30):
+    """Graph solves of 5 and then 3 systems queued back to back on the
+    plan's stream with no wait: the second call replaces the captured
+    5-system graph while its launch may still be queued.  The plan waits for
+    that launch before destroying the graph, so both results equal the same
+    calls made one at a time (a system's bits do not depend on how many
+    systems share the call)."""
+    from hipmem import DevArray, sync
+    cfg, r = large30
+    grads = _batch(r, cfg, 5).astype(np.float32)
+    g = tomo.GpuTomo(r, nbatch=5, maxit=cfg["maxit"])
+    dg = DevArray.of(grads)
+    d5, d3 = DevArray((5, g.nx), np.float32), DevArray((3, g.nx), np.float32)
+    g.solve_dev(dg.ptr, d5.ptr, nbatch=5, mode=1)
+    ref5 = d5.numpy()
+    g.solve_dev(dg.ptr, d3.ptr, nbatch=3, mode=1)
+    ref3 = d3.numpy()
+    assert np.array_equal(ref3, ref5[:3])
+    for rep in range(3):
+        g.solve_dev(dg.ptr, d5.ptr, nbatch=5, mode=1)
+        g.solve_dev(dg.ptr, d3.ptr, nbatch=3, mode=1)
+        sync()
+        assert np.array_equal(d5.numpy(), ref5), rep
+        assert np.array_equal(d3.numpy(), ref3), rep
+    for d in (dg, d5, d3):
+        d.free()
+    g.close()
+
+
 def test_tune_set_rejects_bad_choice(large30):
     cfg, r = large30
     g = tomo.GpuTomo(r, nbatch=1, maxit=5)
